@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""SOCS aerial-image throughput on one MI355X.
+
+Workload (the production clip of the reference's SOCS path): 1024 x 1024
+binary masks at 4 nm/pixel (period 4096 nm), seeded Manhattan rectangles with
+sides of 12..45 pixels (48..180 nm) to 40 % fill; lambda 193 nm, NA 0.85, zero
+defocus, 41 x 41 annular source sigma 0.6-0.9 (572 points); Nx = 34 (69 x 69
+window), K = 24 kernels extracted on the GPU; 256 x 256 conv grid.
+
+A step is one pass of the hot path over one batch of B masks already resident
+in HBM: mask spectrum window -> SOCS accumulate -> Fourier interpolation ->
+B aerial images in HBM.  `value` = images per second over the timed steps.
+The CPU baseline is the project's FP64 C restatement of the reference CPU path
+(the oracle), timed on a bounded sample on the host cores, and doubles as the
+error check of the GPU images against FP64.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fpga-litho-accel_amd"))
+
+import litho_amd as L  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+N = 1024
+PIX_NM = 4
+LAMBDA, NA, DEFOCUS = 193.0, 0.85, 0.0
+NK = 24
+WMIN, WMAX, FILL = 12, 45, 0.40
+STAGES = ["mask_rows_dft", "mask_cols", "socs_accum", "fi_rows_dft", "fi_cols", "image_rows"]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_alg_bytes(N, Nx, G, K, B):
+    """Algorithmic HBM bytes per launch of each stage: what the stage must
+    read and write at minimum (its inputs and outputs, intermediates of this
+    design counted once each way)."""
+    W = 2 * Nx + 1
+    nb = 2 * Nx + 1
+    f2 = 8
+    return {
+        "mask_rows_dft": B * (4 * N * N + N * W * f2),
+        "mask_cols": B * (N * W * f2 + W * W * f2),
+        "socs_accum": B * (W * W * f2 + 4 * G * G) + K * W * W * f2,
+        "fi_rows_dft": B * (4 * G * G + G * nb * f2),
+        "fi_cols": B * (G * nb * f2 + N * nb * f2),
+        "image_rows": B * (N * nb * f2 + 4 * N * N),
+    }
+
+
+def cpu_baseline(masks_host, K, sc, Nx, threads, budget_s, gpu_images):
+    """Oracle (FP64 C, OpenMP over kernels like litho_thread.cpp L897-961) on
+    a bounded sample: Module 3 (mask spectrum) + Module 7 (SOCS + FI) per image."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.lib()
+    done, t0, errs = 0, time.perf_counter(), []
+    Kd = K.astype(np.complex128)
+    scf = sc.astype(np.float32)
+    while done < len(masks_host):
+        img = O.calc_socs(O.mask_spectrum(masks_host[done]), Kd, scf, Nx, Nx, nthreads=threads)
+        g = gpu_images[done]
+        errs.append(float(np.abs(g.astype(np.float64) - img).max() / np.abs(img).max()))
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt, max(errs)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    B = args.batch
+
+    # ---- setup (one time): source, Nx, kernels on the GPU ----------------------
+    src = L.createSource(41, "Annular", 0.6, 0.9)
+    sigma = L.getOuterSigma(src)
+    period = PIX_NM * N
+    Nx = L.compute_nx(NA, period, sigma, LAMBDA)
+    G = 1
+    while G < 4 * Nx + 1:
+        G *= 2
+    log(f"N={N} Nx={Nx} K={NK} G={G} batch={B}")
+    t0 = time.perf_counter()
+    sc, K, flags = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx, Nx, NK)
+    setup_s = time.perf_counter() - t0
+    log(f"kernels (TCC Gram route + FP64 Jacobi on GPU) {setup_s:.3f}s flags={flags}")
+
+    ctx = L.Context()
+    bank = ctx.bank(K, sc, Nx, Nx)
+    masks = np.stack([L.manhattan_mask(N, 1000 + i, WMIN, WMAX, FILL) for i in range(B)])
+    d_mask = ctx.alloc(masks.nbytes)
+    d_img = ctx.alloc(masks.nbytes)
+    ctx.h2d(d_mask, masks)
+
+    # ---- warmup + timed steps ---------------------------------------------------
+    for _ in range(args.warmup):
+        ctx.socs_image_batch(bank, d_mask, B, N, d_img)
+    ctx.sync()
+    log(f"warmup {args.warmup} steps done")
+    stage_acc = np.zeros(len(STAGES))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.socs_image_batch(bank, d_mask, B, N, d_img)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    # per-stage device time from HIP events on the library stream (extra steps)
+    nprobe = min(5, args.steps)
+    for _ in range(nprobe):
+        ctx.socs_image_batch(bank, d_mask, B, N, d_img)
+        tot, st = ctx.last_timing(len(STAGES))
+        stage_acc += np.array(st)
+    stage_ms = stage_acc / nprobe
+    ms_per_step = wall * 1e3 / args.steps
+    images_per_s = B * args.steps / wall
+    log(f"{images_per_s:.1f} images/s, {ms_per_step:.3f} ms/step, stages(ms) "
+        + ", ".join(f"{n}={t:.3f}" for n, t in zip(STAGES, stage_ms)))
+
+    alg = stage_alg_bytes(N, Nx, G, NK, B)
+    kernels = {n: {"ms": round(float(t), 4),
+                   "alg_bytes": int(alg[n]),
+                   "GBps": round(alg[n] / (t * 1e-3) / 1e9, 1)}
+               for n, t in zip(STAGES, stage_ms)}
+    dom = STAGES[int(np.argmax(stage_ms))]
+    path_bytes = 8 * N * N  # mask read + image write per image
+    step_dev_ms = float(stage_ms.sum())
+    achieved = path_bytes * B / (step_dev_ms * 1e-3) / 1e9
+
+    out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
+
+    cpu = None
+    max_err = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, n_done, dt, max_err = cpu_baseline(masks[:8], K, sc, Nx, threads, args.cpu_budget,
+                                                 out_imgs)
+        cpu = {"value": round(rate, 4), "unit": "images/s", "cores": threads, "kind": "port",
+               "sample": f"{n_done} of the same 1024^2 clips, Module 3 + Module 7 (mask FFT, "
+                         f"24-kernel SOCS on the 256^2 grid, FI) in FP64 with own FFT, "
+                         f"{dt:.1f}s"}
+        log(f"cpu baseline {rate:.3f} images/s on {threads} threads; GPU vs FP64 max rel err "
+            f"{max_err:.2e}")
+
+    line = {
+        "metric": "1024^2 aerial images/s, plain vs chosen form at <=1e-3 max rel err; % HBM peak",
+        "value": round(images_per_s, 2),
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain FP32 SOCS, 1024^2 Manhattan clips at 4 nm, NA 0.85, "
+                               "41^2 annulus 0.6-0.9, Nx=34, K=24",
+                   "N": N, "Nx": Nx, "K": NK, "conv_grid": G, "batch": B,
+                   "form": "plain"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": None,
+                     "scope": "whole path (6 launches per step); algorithmic bytes = 8 N^2 per "
+                              "image (mask read + image write)",
+                     "dominant_kernel": dom},
+        "kernels": kernels,
+        "max_rel_err_vs_fp64": max_err,
+        "setup_s": round(setup_s, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    L.lib().litho_bank_destroy(bank)
+    ctx.free(d_mask)
+    ctx.free(d_img)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,810 @@
+// C-ABI of the MI355X SOCS engine (declared in include/litho_socs.h).
+//
+// Host-pointer entry points are synchronous conveniences that stage data
+// through a per-thread default context; the device-pointer batch entry point
+// is asynchronous on the caller's context stream.  No entry point falls back
+// to the CPU: without a usable GPU every compute call returns LITHO_ENODEV.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "eigen_kernels.hpp"
+#include "fft_lds.hpp"
+#include "socs_kernels.hpp"
+#include "tcc_kernels.hpp"
+
+namespace litho {
+
+static thread_local char g_last_err[256] = "";
+
+void set_last_hip_error(hipError_t e, const char* what, const char* file, int line) {
+    snprintf(g_last_err, sizeof(g_last_err), "%s: %s (%s:%d)", hipGetErrorString(e), what, file,
+             line);
+}
+
+}  // namespace litho
+
+using namespace litho;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+enum { NSTAGE = 6 };
+
+struct litho_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float2* tw = nullptr;
+    DevBuf ws[8];
+    hipEvent_t ev[NSTAGE + 1] = {};
+    bool timed = false;
+};
+
+struct litho_bank {
+    litho_ctx* ctx = nullptr;
+    float2* d_bank = nullptr;
+    float* d_scales = nullptr;
+    int nk = 0, Nx = 0, Ny = 0;
+};
+
+static int ctx_ws(litho_ctx* c, int slot, size_t bytes, void** out) {
+    DevBuf& b = c->ws[slot];
+    if (b.bytes < bytes) {
+        if (b.p) LITHO_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+        if (hipMalloc(&b.p, bytes) != hipSuccess) {
+            b.p = nullptr;
+            return LITHO_ENOMEM;
+        }
+        b.bytes = bytes;
+    }
+    *out = b.p;
+    return LITHO_OK;
+}
+
+extern "C" {
+
+const char* litho_status_string(int status) {
+    switch (status) {
+        case LITHO_OK: return "ok";
+        case LITHO_EARG: return "invalid argument or unsupported shape";
+        case LITHO_ENOMEM: return "allocation failed";
+        case LITHO_EHIP: return g_last_err[0] ? g_last_err : "HIP runtime error";
+        case LITHO_ENOCONV: return "eigensolver did not converge";
+        case LITHO_ENODEV: return "no usable GPU";
+        case LITHO_ERANGE: return "pupil support leaves the kernel window";
+        default: return "unknown status";
+    }
+}
+
+int litho_version(void) { return 100; }
+
+int litho_ctx_create(litho_ctx** out, int device) {
+    if (!out) return LITHO_EARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LITHO_ENODEV;
+    if (device < 0 || device >= ndev) return LITHO_EARG;
+    std::unique_ptr<litho_ctx> c(new litho_ctx());
+    c->device = device;
+    LITHO_HIP(hipSetDevice(device));
+    LITHO_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    std::vector<float2> tw(TW_MAX);
+    for (int j = 0; j < TW_MAX; j++) {
+        double a = -2.0 * M_PI * (double)j / (double)TW_MAX;
+        tw[j] = make_float2((float)cos(a), (float)sin(a));
+    }
+    LITHO_HIP(hipMalloc(&c->tw, sizeof(float2) * TW_MAX));
+    LITHO_HIP(hipMemcpy(c->tw, tw.data(), sizeof(float2) * TW_MAX, hipMemcpyHostToDevice));
+    for (int i = 0; i <= NSTAGE; i++) LITHO_HIP(hipEventCreate(&c->ev[i]));
+    *out = c.release();
+    return LITHO_OK;
+}
+
+int litho_ctx_destroy(litho_ctx* c) {
+    if (!c) return LITHO_EARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& b : c->ws)
+        if (b.p) (void)hipFree(b.p);
+    if (c->tw) (void)hipFree(c->tw);
+    for (int i = 0; i <= NSTAGE; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LITHO_OK;
+}
+
+int litho_ctx_sync(litho_ctx* c) {
+    if (!c) return LITHO_EARG;
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    return LITHO_OK;
+}
+
+int litho_dev_alloc(litho_ctx* c, void** dptr, size_t bytes) {
+    if (!c || !dptr || !bytes) return LITHO_EARG;
+    LITHO_HIP(hipSetDevice(c->device));
+    if (hipMalloc(dptr, bytes) != hipSuccess) return LITHO_ENOMEM;
+    return LITHO_OK;
+}
+
+int litho_dev_free(litho_ctx* c, void* dptr) {
+    if (!c) return LITHO_EARG;
+    LITHO_HIP(hipFree(dptr));
+    return LITHO_OK;
+}
+
+int litho_memcpy_h2d(litho_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) return LITHO_EARG;
+    LITHO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    return LITHO_OK;
+}
+
+int litho_memcpy_d2h(litho_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) return LITHO_EARG;
+    LITHO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    return LITHO_OK;
+}
+
+int litho_memset_dev(litho_ctx* c, void* dst, int value, size_t bytes) {
+    if (!c || !dst) return LITHO_EARG;
+    LITHO_HIP(hipMemsetAsync(dst, value, bytes, c->stream));
+    return LITHO_OK;
+}
+
+}  // extern "C"
+
+// per-thread default context for the host-pointer entry points
+static int default_ctx(litho_ctx** out) {
+    static thread_local litho_ctx* ctx = nullptr;
+    if (!ctx) {
+        int s = litho_ctx_create(&ctx, 0);
+        if (s != LITHO_OK) {
+            ctx = nullptr;
+            return s;
+        }
+    }
+    *out = ctx;
+    return LITHO_OK;
+}
+
+// RAII device buffer for the synchronous entry points
+struct Dev {
+    void* p = nullptr;
+    ~Dev() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+            p = nullptr;
+            return LITHO_ENOMEM;
+        }
+        return LITHO_OK;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Module 1 helpers and the synthetic mask generator
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Pcg32 {
+    uint64_t state = 0, inc = 0;
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+    void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+    // uniform integer in [0, bound) without modulo bias
+    uint32_t bounded(uint32_t bound) {
+        uint32_t threshold = (uint32_t)(-bound) % bound;
+        for (;;) {
+            uint32_t r = next();
+            if (r >= threshold) return r % bound;
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+static void normalize_source(float* m, int count) {
+    float sum = 0.0f;
+    for (int i = 0; i < count; ++i) sum += m[i];
+    if (sum != 0.0f)
+        for (int i = 0; i < count; ++i) m[i] /= sum;
+}
+
+int litho_source_annular(float* src, int srcSize, float inner, float outer) {
+    if (!src || srcSize <= 0) return LITHO_EARG;
+    const int half = (srcSize - 1) / 2;
+    const int rin = (int)(inner * (float)half), rout = (int)(outer * (float)half);
+    for (int y = 0; y < srcSize; ++y)
+        for (int x = 0; x < srcSize; ++x) {
+            double d = std::sqrt(std::pow((double)(x - half), 2) + std::pow((double)(y - half), 2));
+            src[y * srcSize + x] = (d >= rin && d <= rout) ? 1.0f : 0.0f;
+        }
+    normalize_source(src, srcSize * srcSize);
+    return LITHO_OK;
+}
+
+static int circles_source(float* src, int srcSize, const int (*c)[2], int nc, int rad) {
+    for (int y = 0; y < srcSize; ++y)
+        for (int x = 0; x < srcSize; ++x) {
+            bool in = false;
+            for (int i = 0; i < nc && !in; ++i) {
+                double dx = x - c[i][0], dy = y - c[i][1];
+                in = std::sqrt(dx * dx + dy * dy) <= rad;
+            }
+            src[y * srcSize + x] = in ? 1.0f : 0.0f;
+        }
+    normalize_source(src, srcSize * srcSize);
+    return LITHO_OK;
+}
+
+int litho_source_quadrupole(float* src, int srcSize, float radius, float offset) {
+    if (!src || srcSize <= 0) return LITHO_EARG;
+    const int half = (srcSize - 1) / 2;
+    const int off = (int)(offset * (float)half), rad = (int)(radius * (float)half);
+    const int c[4][2] = {{half, half + off}, {half, half - off}, {half + off, half}, {half - off, half}};
+    return circles_source(src, srcSize, c, 4, rad);
+}
+
+int litho_source_dipole(float* src, int srcSize, float radius, float offset, int onXAxis) {
+    if (!src || srcSize <= 0) return LITHO_EARG;
+    const int half = (srcSize - 1) / 2;
+    const int off = (int)(offset * (float)half), rad = (int)(radius * (float)half);
+    const int cx = srcSize / 2, cy = srcSize / 2;
+    const int c[2][2] = {{cx + (onXAxis ? off : 0), cy + (onXAxis ? 0 : off)},
+                         {cx - (onXAxis ? off : 0), cy - (onXAxis ? 0 : off)}};
+    return circles_source(src, srcSize, c, 2, rad);
+}
+
+int litho_source_point(float* src, int srcSize, float px, float py) {
+    if (!src || srcSize <= 0) return LITHO_EARG;
+    std::memset(src, 0, sizeof(float) * (size_t)srcSize * srcSize);
+    int ix = (int)std::round((double)(px * (float)srcSize) / 2.0 + srcSize / 2.0);
+    int iy = (int)std::round((double)(-1.0f * py * (float)srcSize) / 2.0 + srcSize / 2.0);
+    if (px >= 0) ix -= 1;
+    if (-1.0f * py >= 0) iy -= 1;
+    if (ix < 0 || iy < 0 || ix >= srcSize || iy >= srcSize) return LITHO_ERANGE;
+    src[iy * srcSize + ix] = 1.0f;
+    normalize_source(src, srcSize * srcSize);
+    return LITHO_OK;
+}
+
+float litho_outer_sigma(const float* src, int srcSize) {
+    if (!src || srcSize <= 0) return 0.0f;
+    int center = (srcSize - 1) / 2;
+    float maxd = 0.0f;
+    for (int y = 0; y < srcSize; y++)
+        for (int x = 0; x < srcSize; x++)
+            if (src[y * srcSize + x] != 0.0f) {
+                float d = (float)std::sqrt((double)((y - center) * (y - center) +
+                                                    (x - center) * (x - center)));
+                if (d > maxd) maxd = d;
+            }
+    float s = center == 0 ? 0.0f : maxd / (float)center;
+    return s > 1.0f ? 1.0f : s;
+}
+
+int litho_compute_nx(float NA, int period, float outerSigma, float lambda) {
+    volatile float v = NA * (float)period;
+    v = v * (1.0f + outerSigma);
+    v = v / lambda;
+    return (int)std::floor((float)v);
+}
+
+int litho_manhattan_mask(float* mask, int N, unsigned long long seed, int wmin, int wmax,
+                         float fill) {
+    if (!mask || N <= 0 || wmin <= 0 || wmax < wmin || wmax > N || fill < 0.f || fill > 0.95f)
+        return LITHO_EARG;
+    std::memset(mask, 0, sizeof(float) * (size_t)N * N);
+    Pcg32 rng;
+    rng.seed(seed, 0x5eedULL);
+    const double target = (double)fill * (double)N * (double)N;
+    double filled = 0.0;
+    long guard = 64L * N * N / ((long)wmin * wmin) + 1000;
+    while (filled < target && guard-- > 0) {
+        int w = wmin + (int)rng.bounded((uint32_t)(wmax - wmin + 1));
+        int h = wmin + (int)rng.bounded((uint32_t)(wmax - wmin + 1));
+        int x0 = (int)rng.bounded((uint32_t)(N - w + 1));
+        int y0 = (int)rng.bounded((uint32_t)(N - h + 1));
+        for (int y = y0; y < y0 + h; y++)
+            for (int x = x0; x < x0 + w; x++) {
+                float& m = mask[(size_t)y * N + x];
+                if (m == 0.0f) {
+                    m = 1.0f;
+                    filled += 1.0;
+                }
+            }
+    }
+    return LITHO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Module 3
+// ---------------------------------------------------------------------------
+int litho_mask_spectrum(const float* mask, int N, float* mskf_r, float* mskf_i) {
+    if (!mask || !mskf_r || !mskf_i || !is_pow2(N) || N < 32 || N > 4096) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    size_t NN = (size_t)N * N;
+    Dev dm, dR, dM;
+    LITHO_TRY(dm.alloc(NN * sizeof(float)));
+    LITHO_TRY(dR.alloc(NN * sizeof(float2)));
+    LITHO_TRY(dM.alloc(NN * sizeof(float2)));
+    LITHO_HIP(hipMemcpyAsync(dm.p, mask, NN * sizeof(float), hipMemcpyHostToDevice, st));
+    LITHO_TRY(launch_rows_dft(N, dm.as<float>(), dR.as<float2>(), N, -N / 2, N, true, c->tw, st));
+    LITHO_TRY(launch_mask_cols(N, dR.as<float2>(), dM.as<float2>(), 1, N, -N / 2, N, c->tw, st));
+    std::vector<float2> h(NN);
+    LITHO_HIP(hipMemcpyAsync(h.data(), dM.p, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < NN; i++) {
+        mskf_r[i] = h[i].x;
+        mskf_i[i] = h[i].y;
+    }
+    return LITHO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Module 4 / 6 internals
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LiveSources {
+    std::vector<int2> pq;
+    std::vector<double> S;
+    PupilGeom g;
+};
+
+// Live source points in the reference's loop order (calcTCC L458-466) and the
+// calcTCC scalars (L443-448).
+int collect_sources(const LithoOptics* op, const float* src, int srcSize, LiveSources& L) {
+    if (!op || !src || srcSize <= 0 || op->Nx < 0 || op->Ny < 0 || op->NA <= 0.f ||
+        op->lambda <= 0.f)
+        return LITHO_EARG;
+    PupilGeom& g = L.g;
+    g.sh = (int)((srcSize - 1) / 2.0);
+    const int oSgm = (int)std::ceil((double)((float)g.sh * op->outerSigma));
+    volatile float na2l = op->NA * op->NA;
+    na2l = na2l / op->lambda;
+    g.dz = op->defocus / (float)na2l;
+    g.k = (float)(2 * M_PI / (double)op->lambda);
+    volatile float lx = (float)op->Lx * op->NA;
+    volatile float ly = (float)op->Ly * op->NA;
+    g.Lx_norm = (float)lx / op->lambda;
+    g.Ly_norm = (float)ly / op->lambda;
+    g.NA = op->NA;
+    g.Nx = op->Nx;
+    g.Ny = op->Ny;
+    if (g.sh <= 0) return LITHO_EARG;
+    for (int q = -oSgm; q <= oSgm; q++)
+        for (int p = -oSgm; p <= oSgm; p++) {
+            if (q + g.sh < 0 || q + g.sh >= srcSize || p + g.sh < 0 || p + g.sh >= srcSize)
+                continue;
+            int id = (q + g.sh) * srcSize + p + g.sh;
+            if (src[id] != 0.0f && (p * p + q * q) <= g.sh * g.sh) {
+                L.pq.push_back(make_int2(p, q));
+                L.S.push_back((double)src[id]);
+            }
+        }
+    return L.pq.empty() ? LITHO_EARG : LITHO_OK;
+}
+
+struct Pupil {
+    Dev A, S, srcs, err;
+    int n = 0, m = 0;
+    bool imag = false;
+};
+
+int build_pupil(litho_ctx* c, const LithoOptics* op, const float* src, int srcSize, Pupil& P) {
+    LiveSources L;
+    LITHO_TRY(collect_sources(op, src, srcSize, L));
+    hipStream_t st = c->stream;
+    P.n = (2 * op->Nx + 1) * (2 * op->Ny + 1);
+    P.m = (int)L.pq.size();
+    P.imag = op->defocus != 0.0f;
+    LITHO_TRY(P.A.alloc((size_t)P.n * P.m * sizeof(double2)));
+    LITHO_TRY(P.S.alloc((size_t)P.m * sizeof(double)));
+    LITHO_TRY(P.srcs.alloc((size_t)P.m * sizeof(int2)));
+    LITHO_TRY(P.err.alloc(sizeof(int)));
+    LITHO_HIP(hipMemcpyAsync(P.S.p, L.S.data(), P.m * sizeof(double), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(P.srcs.p, L.pq.data(), P.m * sizeof(int2), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemsetAsync(P.err.p, 0, sizeof(int), st));
+    LITHO_TRY(launch_pupil(P.srcs.as<int2>(), P.m, L.g, P.A.as<double2>(), P.err.as<int>(), st));
+    int herr = 0;
+    LITHO_HIP(hipMemcpyAsync(&herr, P.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    return herr ? LITHO_ERANGE : LITHO_OK;
+}
+
+// Jacobi to convergence on the device matrix H (n x n); V receives vectors.
+int jacobi_solve(litho_ctx* c, double2* H, double2* V, int n) {
+    hipStream_t st = c->stream;
+    const int npart = 256;
+    Dev part, tol, rots, cnt;
+    LITHO_TRY(part.alloc(npart * sizeof(double)));
+    LITHO_TRY(tol.alloc(sizeof(double)));
+    LITHO_TRY(rots.alloc(((n + 1) / 2 + 1) * sizeof(JacobiRot)));
+    LITHO_TRY(cnt.alloc(sizeof(int)));
+    LITHO_TRY(jacobi_init(H, V, n, part.as<double>(), npart, tol.as<double>(), st));
+    if (n == 1) return LITHO_OK;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        LITHO_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
+        LITHO_TRY(jacobi_sweep(H, V, n, tol.as<double>(), rots.as<JacobiRot>(), cnt.as<int>(), st));
+        int h = 0;
+        LITHO_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipStreamSynchronize(st));
+        if (h == 0) return LITHO_OK;
+    }
+    return LITHO_ENOCONV;
+}
+
+// From descending eigenvalues: how many vectors to canonicalise and the
+// clusters among them (same rule as the oracle: consecutive gaps <= rtol*|w0|).
+void plan_clusters(const std::vector<double>& wd, int nk, int* mtop, int* split,
+                   std::vector<int2>& clusters) {
+    const double rtol = 1e-10;
+    const int m = (int)wd.size();
+    const double scale = std::fabs(wd[0]);
+    *split = (nk < m && std::fabs(wd[nk - 1] - wd[nk]) <= rtol * scale) ? 1 : 0;
+    int mc = nk;
+    while (mc < m && std::fabs(wd[mc - 1] - wd[mc]) <= rtol * scale) mc++;
+    *mtop = mc;
+    int i = 0;
+    while (i < mc) {
+        int j = i + 1;
+        while (j < mc && std::fabs(wd[j - 1] - wd[j]) <= rtol * scale) j++;
+        if (j - i > 1 && j - i <= CANON_MAX) clusters.push_back(make_int2(i, j - i));
+        i = j;
+    }
+}
+
+// Shared tail: Vt (mtop x n vector-major, descending) -> canonical, phase, flip.
+int finish_kernels(litho_ctx* c, double2* Vt, int n, int mtop, int nk, int Nx, int Ny,
+                   const std::vector<int2>& clusters, float* kr, float* ki, double2* kd_dev) {
+    hipStream_t st = c->stream;
+    Dev dcl, dkr, dki;
+    if (!clusters.empty()) {
+        LITHO_TRY(dcl.alloc(clusters.size() * sizeof(int2)));
+        LITHO_HIP(hipMemcpyAsync(dcl.p, clusters.data(), clusters.size() * sizeof(int2),
+                                 hipMemcpyHostToDevice, st));
+        LITHO_TRY(launch_canon(Vt, n, Nx, Ny, dcl.as<int2>(), (int)clusters.size(), st));
+    }
+    float *pr = nullptr, *pi = nullptr;
+    if (kr) {
+        LITHO_TRY(dkr.alloc((size_t)nk * n * sizeof(float)));
+        LITHO_TRY(dki.alloc((size_t)nk * n * sizeof(float)));
+        pr = dkr.as<float>();
+        pi = dki.as<float>();
+    }
+    LITHO_TRY(launch_phase_flip(Vt, n, mtop, nk, kd_dev, pr, pi, st));
+    if (kr) {
+        LITHO_HIP(hipMemcpyAsync(kr, pr, (size_t)nk * n * sizeof(float), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipMemcpyAsync(ki, pi, (size_t)nk * n * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    LITHO_HIP(hipStreamSynchronize(st));
+    return LITHO_OK;
+}
+
+// Eigen of device H (n x n, upper triangle valid) -> top kernels.
+int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int nk,
+                            double* scales, float* kr, float* ki, int* flags) {
+    hipStream_t st = c->stream;
+    Dev V, w, order, Vt, wd;
+    LITHO_TRY(V.alloc((size_t)n * n * sizeof(double2)));
+    LITHO_TRY(jacobi_solve(c, H, V.as<double2>(), n));
+    LITHO_TRY(w.alloc(n * sizeof(double)));
+    LITHO_TRY(order.alloc(n * sizeof(int)));
+    LITHO_TRY(launch_diag_rank(H, n, w.as<double>(), order.as<int>(), st));
+    std::vector<double> hw(n);
+    std::vector<int> ho(n);
+    LITHO_HIP(hipMemcpyAsync(hw.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipMemcpyAsync(ho.data(), order.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    std::vector<double> wdesc(n);
+    for (int r = 0; r < n; r++) wdesc[r] = hw[ho[r]];
+    int mtop, split;
+    std::vector<int2> clusters;
+    plan_clusters(wdesc, nk, &mtop, &split, clusters);
+    LITHO_TRY(Vt.alloc((size_t)mtop * n * sizeof(double2)));
+    LITHO_TRY(wd.alloc(mtop * sizeof(double)));
+    LITHO_TRY(launch_gather_top(V.as<double2>(), w.as<double>(), order.as<int>(), n, mtop,
+                                Vt.as<double2>(), wd.as<double>(), st));
+    LITHO_TRY(finish_kernels(c, Vt.as<double2>(), n, mtop, nk, Nx, Ny, clusters, kr, ki, nullptr));
+    for (int k = 0; k < nk; k++) scales[k] = wdesc[k];
+    if (flags) *flags = split ? LITHO_FLAG_CLUSTER_SPLIT : 0;
+    return LITHO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize, double* tcc_re,
+                    double* tcc_im) {
+    if (!op || !src || !tcc_re || !tcc_im) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    Pupil P;
+    LITHO_TRY(build_pupil(c, op, src, srcSize, P));
+    const size_t nn = (size_t)P.n * P.n;
+    Dev C;
+    LITHO_TRY(C.alloc(nn * sizeof(double2)));
+    LITHO_TRY(launch_herm_gemm(P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), P.n, P.m,
+                               P.imag, C.as<double2>(), st));
+    std::vector<double2> h(nn);
+    LITHO_HIP(hipMemcpyAsync(h.data(), C.p, nn * sizeof(double2), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < nn; i++) {
+        tcc_re[i] = h[i].x;
+        tcc_im[i] = h[i].y;
+    }
+    return LITHO_OK;
+}
+
+int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx, int Ny, int nk,
+                       double* scales, float* krn_r, float* krn_i, int* flags) {
+    if (!tcc_re || !tcc_im || !scales || !krn_r || !krn_i || nk <= 0 || nk > n ||
+        n != (2 * Nx + 1) * (2 * Ny + 1))
+        return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    const size_t nn = (size_t)n * n;
+    std::vector<double2> h(nn);
+    for (size_t i = 0; i < nn; i++) h[i] = make_double2(tcc_re[i], tcc_im[i]);
+    Dev H;
+    LITHO_TRY(H.alloc(nn * sizeof(double2)));
+    LITHO_HIP(hipMemcpyAsync(H.p, h.data(), nn * sizeof(double2), hipMemcpyHostToDevice, st));
+    return kernels_from_device_tcc(c, H.as<double2>(), n, Nx, Ny, nk, scales, krn_r, krn_i, flags);
+}
+
+int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSize, int nk,
+                              double* scales, float* krn_r, float* krn_i, int* flags) {
+    if (!op || !src || !scales || !krn_r || !krn_i || nk <= 0) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    Pupil P;
+    LITHO_TRY(build_pupil(c, op, src, srcSize, P));
+    const int n = P.n, m = P.m;
+    if (nk > n || nk > m) return LITHO_EARG;
+    if (n <= m) {
+        Dev H;
+        LITHO_TRY(H.alloc((size_t)n * n * sizeof(double2)));
+        LITHO_TRY(launch_herm_gemm(P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), n, m,
+                                   P.imag, H.as<double2>(), st));
+        return kernels_from_device_tcc(c, H.as<double2>(), n, op->Nx, op->Ny, nk, scales, krn_r,
+                                       krn_i, flags);
+    }
+    // Gram route: G = B^H B (m x m), B = A diag(sqrt S)
+    Dev Bt, G, U, w, order, Vt, wd;
+    LITHO_TRY(Bt.alloc((size_t)m * n * sizeof(double2)));
+    LITHO_TRY(G.alloc((size_t)m * m * sizeof(double2)));
+    LITHO_TRY(U.alloc((size_t)m * m * sizeof(double2)));
+    LITHO_TRY(launch_conj_transpose_scale(P.A.as<double2>(), P.S.as<double>(), n, m,
+                                          Bt.as<double2>(), st));
+    LITHO_TRY(launch_herm_gemm(Bt.as<double2>(), nullptr, Bt.as<double2>(), m, n, P.imag,
+                               G.as<double2>(), st));
+    LITHO_TRY(jacobi_solve(c, G.as<double2>(), U.as<double2>(), m));
+    LITHO_TRY(w.alloc(m * sizeof(double)));
+    LITHO_TRY(order.alloc(m * sizeof(int)));
+    LITHO_TRY(launch_diag_rank(G.as<double2>(), m, w.as<double>(), order.as<int>(), st));
+    std::vector<double> hw(m);
+    std::vector<int> ho(m);
+    LITHO_HIP(hipMemcpyAsync(hw.data(), w.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipMemcpyAsync(ho.data(), order.p, m * sizeof(int), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    std::vector<double> wdesc(m);
+    for (int r = 0; r < m; r++) wdesc[r] = hw[ho[r]];
+    int mtop, split;
+    std::vector<int2> clusters;
+    plan_clusters(wdesc, nk, &mtop, &split, clusters);
+    LITHO_TRY(Vt.alloc((size_t)mtop * n * sizeof(double2)));
+    LITHO_TRY(wd.alloc(mtop * sizeof(double)));
+    LITHO_TRY(launch_lift(P.A.as<double2>(), P.S.as<double>(), U.as<double2>(), w.as<double>(),
+                          order.as<int>(), n, m, mtop, Vt.as<double2>(), wd.as<double>(), st));
+    LITHO_TRY(finish_kernels(c, Vt.as<double2>(), n, mtop, nk, op->Nx, op->Ny, clusters, krn_r,
+                             krn_i, nullptr));
+    for (int k = 0; k < nk; k++) scales[k] = wdesc[k];
+    if (flags) *flags = (split ? LITHO_FLAG_CLUSTER_SPLIT : 0) | LITHO_FLAG_GRAM_ROUTE;
+    return LITHO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Module 7
+// ---------------------------------------------------------------------------
+int litho_bank_create(litho_ctx* c, const float* krn_r, const float* krn_i, const float* scales,
+                      int nk, int Nx, int Ny, litho_bank** out) {
+    if (!c || !krn_r || !krn_i || !scales || !out || nk <= 0 || Nx < 0 || Ny != Nx)
+        return LITHO_EARG;
+    *out = nullptr;
+    const int W = 2 * Nx + 1;
+    const size_t per = (size_t)W * W;
+    std::vector<float2> h((size_t)nk * per);
+    for (size_t i = 0; i < h.size(); i++) h[i] = make_float2(krn_r[i], krn_i[i]);
+    std::unique_ptr<litho_bank> b(new litho_bank());
+    b->ctx = c;
+    b->nk = nk;
+    b->Nx = Nx;
+    b->Ny = Ny;
+    LITHO_HIP(hipSetDevice(c->device));
+    if (hipMalloc(&b->d_bank, h.size() * sizeof(float2)) != hipSuccess) return LITHO_ENOMEM;
+    if (hipMalloc(&b->d_scales, nk * sizeof(float)) != hipSuccess) {
+        (void)hipFree(b->d_bank);
+        return LITHO_ENOMEM;
+    }
+    LITHO_HIP(hipMemcpyAsync(b->d_bank, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice,
+                             c->stream));
+    LITHO_HIP(hipMemcpyAsync(b->d_scales, scales, nk * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    *out = b.release();
+    return LITHO_OK;
+}
+
+int litho_bank_destroy(litho_bank* b) {
+    if (!b) return LITHO_EARG;
+    if (b->d_bank) (void)hipFree(b->d_bank);
+    if (b->d_scales) (void)hipFree(b->d_scales);
+    delete b;
+    return LITHO_OK;
+}
+
+int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
+                           int N, float* d_image) {
+    if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
+    const int Nx = bank->Nx;
+    const int W = 2 * Nx + 1;
+    const int G = next_pow2(4 * Nx + 1);
+    const int band = 2 * Nx;
+    if (!is_pow2(N) || N < 32 || N > 4096 || G > N || G > 1024 || W > N) return LITHO_EARG;
+    hipStream_t st = c->stream;
+    void *R, *M, *T, *P, *U;
+    LITHO_TRY(ctx_ws(c, 0, (size_t)B * N * W * sizeof(float2), &R));
+    LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
+    LITHO_TRY(ctx_ws(c, 2, (size_t)B * G * G * sizeof(float), &T));
+    LITHO_TRY(ctx_ws(c, 3, (size_t)B * G * (band + 1) * sizeof(float2), &P));
+    LITHO_TRY(ctx_ws(c, 4, (size_t)B * N * (band + 1) * sizeof(float2), &U));
+    LITHO_HIP(hipEventRecord(c->ev[0], st));
+    LITHO_TRY(launch_rows_dft(N, d_mask, (float2*)R, (long)B * N, -Nx, W, true, c->tw, st));
+    LITHO_HIP(hipEventRecord(c->ev[1], st));
+    LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
+    LITHO_HIP(hipEventRecord(c->ev[2], st));
+    LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->d_bank, bank->d_scales, bank->nk, Nx, B,
+                                (float*)T, 0, c->tw, st));
+    LITHO_HIP(hipEventRecord(c->ev[3], st));
+    LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
+    LITHO_HIP(hipEventRecord(c->ev[4], st));
+    LITHO_TRY(launch_fi_cols(G, N, (float2*)P, (float2*)U, B, band, 0, c->tw, st));
+    LITHO_HIP(hipEventRecord(c->ev[5], st));
+    LITHO_TRY(launch_image_rows(N, (float2*)U, d_image, (long)B * N, band, c->tw, st));
+    LITHO_HIP(hipEventRecord(c->ev[6], st));
+    c->timed = true;
+    return LITHO_OK;
+}
+
+int litho_last_timing(litho_ctx* c, float* total_ms, float* stage_ms, int nstage) {
+    if (!c || !c->timed) return LITHO_EARG;
+    LITHO_HIP(hipEventSynchronize(c->ev[NSTAGE]));
+    if (total_ms) LITHO_HIP(hipEventElapsedTime(total_ms, c->ev[0], c->ev[NSTAGE]));
+    for (int i = 0; i < nstage && i < NSTAGE; i++)
+        LITHO_HIP(hipEventElapsedTime(&stage_ms[i], c->ev[i], c->ev[i + 1]));
+    return LITHO_OK;
+}
+
+int litho_socs_image(const float* mask, int N, const float* krn_r, const float* krn_i,
+                     const float* scales, int nk, int Nx, int Ny, float* image) {
+    if (!mask || !image) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    litho_bank* bank = nullptr;
+    LITHO_TRY(litho_bank_create(c, krn_r, krn_i, scales, nk, Nx, Ny, &bank));
+    std::unique_ptr<litho_bank, int (*)(litho_bank*)> guard(bank, litho_bank_destroy);
+    const size_t NN = (size_t)N * N;
+    Dev dm, di;
+    LITHO_TRY(dm.alloc(NN * sizeof(float)));
+    LITHO_TRY(di.alloc(NN * sizeof(float)));
+    LITHO_HIP(hipMemcpyAsync(dm.p, mask, NN * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    LITHO_TRY(litho_socs_image_batch(c, bank, dm.as<float>(), 1, N, di.as<float>()));
+    LITHO_HIP(hipMemcpyAsync(image, di.p, NN * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    return LITHO_OK;
+}
+
+int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn_r,
+                      const float* krn_i, const float* scales, float* tmpImg, float* output,
+                      int nk, int nx, int ny, int Lx, int Ly) {
+    if (!mskf_r || !mskf_i || !krn_r || !krn_i || !scales || nk <= 0 || nx < 0 || ny != nx ||
+        Lx <= 0 || Ly <= 0)
+        return LITHO_EARG;
+    const int W = 2 * nx + 1;
+    const int G = next_pow2(4 * nx + 1);
+    if (G > 1024) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    // window of the centred spectrum (the embed's bounds check, L909-936)
+    std::vector<float2> win((size_t)W * W, make_float2(0.f, 0.f));
+    for (int y = -ny; y <= ny; y++)
+        for (int x = -nx; x <= nx; x++) {
+            int my = Ly / 2 + y, mx = Lx / 2 + x;
+            if (my >= 0 && my < Ly && mx >= 0 && mx < Lx)
+                win[(size_t)(y + ny) * W + x + nx] =
+                    make_float2(mskf_r[(size_t)my * Lx + mx], mskf_i[(size_t)my * Lx + mx]);
+        }
+    litho_bank* bank = nullptr;
+    LITHO_TRY(litho_bank_create(c, krn_r, krn_i, scales, nk, nx, ny, &bank));
+    std::unique_ptr<litho_bank, int (*)(litho_bank*)> guard(bank, litho_bank_destroy);
+    Dev dM, dT;
+    LITHO_TRY(dM.alloc(win.size() * sizeof(float2)));
+    LITHO_TRY(dT.alloc((size_t)G * G * sizeof(float)));
+    LITHO_HIP(hipMemcpyAsync(dM.p, win.data(), win.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+    LITHO_TRY(launch_socs_accum(G, dM.as<float2>(), bank->d_bank, bank->d_scales, nk, nx, 1,
+                                dT.as<float>(), 1, c->tw, st));
+    std::vector<float> h((size_t)G * G);
+    LITHO_HIP(hipMemcpyAsync(h.data(), dT.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    if (tmpImg) std::memcpy(tmpImg, h.data(), h.size() * sizeof(float));
+    if (output) std::memcpy(output, h.data(), h.size() * sizeof(float));
+    return LITHO_OK;
+}
+
+int litho_fi(const float* tmpImgp, int G, int N, int band, float* image) {
+    if (!tmpImgp || !image || !is_pow2(G) || !is_pow2(N) || G > N || G > 1024 || N > 4096 ||
+        band < 0 || 2 * band + 1 > G || G < 32)
+        return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    Dev dT, dP, dU, dI;
+    LITHO_TRY(dT.alloc((size_t)G * G * sizeof(float)));
+    LITHO_TRY(dP.alloc((size_t)G * (band + 1) * sizeof(float2)));
+    LITHO_TRY(dU.alloc((size_t)N * (band + 1) * sizeof(float2)));
+    LITHO_TRY(dI.alloc((size_t)N * N * sizeof(float)));
+    LITHO_HIP(hipMemcpyAsync(dT.p, tmpImgp, (size_t)G * G * sizeof(float), hipMemcpyHostToDevice, st));
+    // input is centred (tmpImgp): its DFT carries (-1)^(kx+ky), undone by the
+    // alternating-sign variants of both FI axes
+    LITHO_TRY(launch_rows_dft(G, dT.as<float>(), dP.as<float2>(), G, 0, band + 1, true, c->tw, st));
+    LITHO_TRY(launch_fi_cols(G, N, dP.as<float2>(), dU.as<float2>(), 1, band, 1, c->tw, st));
+    LITHO_TRY(launch_image_rows(N, dU.as<float2>(), dI.as<float>(), N, band, c->tw, st));
+    LITHO_HIP(hipMemcpyAsync(image, dI.p, (size_t)N * N * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    return LITHO_OK;
+}
+
+}  // extern "C"

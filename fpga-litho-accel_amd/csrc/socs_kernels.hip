@@ -1,0 +1,491 @@
+// SOCS image pipeline kernels (plain FP32 path) for gfx950.
+//
+//   rows_dft     : real rows -> a window of their DFT (two rows per complex
+//                  FFT).  Used for the mask rows (Module 3, first axis) and
+//                  for the rows of the conv-grid intensity (FI R2C, first axis).
+//   mask_cols    : mask spectrum along y, keeps the (2Ny+1) window, /N^2 and
+//                  the centring signs (FT_r2c_padded, litho_thread.cpp L255-331).
+//   socs_accum   : for every kernel k: K_k * M on the window, pruned 2-D
+//                  inverse FFT on the G x G conv grid, lambda_k |.|^2
+//                  accumulated in registers (calcSOCS L897-961).
+//   fi_cols      : FI second axis: G-point forward FFT of each kept column,
+//                  band-limited embed into N, N-point inverse FFT (FI L692-772).
+//   image_rows   : FI last axis: C2R of two rows per complex FFT, writes the
+//                  final image rows once.
+//
+// Every kernel takes its geometry as arguments and is launched with a grid
+// that the host checks against the buffers it was given.
+#include "fft_lds.hpp"
+#include "socs_kernels.hpp"
+
+namespace litho {
+
+// ---------------------------------------------------------------------------
+// rows_dft: in  [nrows][L] f32 (nrows even), out [nrows][count] float2
+// out[r][j] = sgn(k) * X_r[k mod L], k = kmin + j, X_r = DFT_L(row r) forward,
+// sgn = (-1)^k when ALT.  FPB complex FFTs (2*FPB rows) per block.
+// ---------------------------------------------------------------------------
+template <int L, int FPB, bool ALT>
+__global__ __launch_bounds__(256) void rows_dft_kernel(const float* __restrict__ in,
+                                                       float2* __restrict__ out, int nrows,
+                                                       int kmin, int count,
+                                                       const float2* __restrict__ tw) {
+    __shared__ float2 sa[FPB * L];
+    __shared__ float2 sb[FPB * L];
+    const int tid = threadIdx.x;
+    const long pair0 = (long)blockIdx.x * FPB;  // first row pair of this block
+    const long npairs = nrows / 2;
+    for (int i = tid; i < FPB * L; i += 256) {
+        int f = i / L, x = i - f * L;
+        long p = pair0 + f;
+        float2 v = make_float2(0.f, 0.f);
+        if (p < npairs) {
+            const float* r0 = in + (2 * p) * (long)L;
+            v = make_float2(r0[x], r0[L + x]);
+        }
+        sa[i] = v;
+    }
+    __syncthreads();
+    float2* res = fft_lds<L, false>(sa, sb, FPB, tw, tid, 256);
+    for (int i = tid; i < FPB * count; i += 256) {
+        int f = i / count, j = i - f * count;
+        long p = pair0 + f;
+        if (p >= npairs) continue;
+        int k = kmin + j;
+        int kp = ((k % L) + L) % L;
+        int km = ((-k % L) + L) % L;
+        float2 z1 = res[f * L + kp];
+        float2 z2 = cconj(res[f * L + km]);
+        // X1 = (z1 + z2)/2, X2 = -i (z1 - z2)/2
+        float2 x1 = cscale(cadd(z1, z2), 0.5f);
+        float2 d = csub(z1, z2);
+        float2 x2 = make_float2(0.5f * d.y, -0.5f * d.x);
+        if (ALT && (k & 1)) {
+            x1 = cscale(x1, -1.f);
+            x2 = cscale(x2, -1.f);
+        }
+        out[(2 * p) * (long)count + j] = x1;
+        out[(2 * p + 1) * (long)count + j] = x2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mask_cols: R [B][N][cx] float2 (row DFT window) -> M [B][cy][cx] float2
+// M[b][j][c] = (1/N^2) (-1)^ky Y_c[ky mod N], ky = kymin + j, Y_c = DFT_N of
+// column c of R[b].  CG columns per block; grid (ceil(cx/CG), B).
+// ---------------------------------------------------------------------------
+template <int N, int CG>
+__global__ __launch_bounds__(256) void mask_cols_kernel(const float2* __restrict__ R,
+                                                        float2* __restrict__ M, int cx,
+                                                        int kymin, int cy,
+                                                        const float2* __restrict__ tw) {
+    __shared__ float2 sa[CG * N];
+    __shared__ float2 sb[CG * N];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * CG;
+    const long b = blockIdx.y;
+    const float2* Rb = R + b * (long)N * cx;
+    for (int i = tid; i < CG * N; i += 256) {
+        int y = i / CG, c = i - y * CG;
+        int col = c0 + c;
+        sa[c * N + y] = col < cx ? Rb[(long)y * cx + col] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2* res = fft_lds<N, false>(sa, sb, CG, tw, tid, 256);
+    const float inv = 1.0f / ((float)N * (float)N);
+    float2* Mb = M + b * (long)cy * cx;
+    for (int i = tid; i < CG * cy; i += 256) {
+        int j = i / CG, c = i - j * CG;
+        int col = c0 + c;
+        if (col >= cx) continue;
+        int ky = kymin + j;
+        int kp = ((ky % N) + N) % N;
+        float s = (ky & 1) ? -inv : inv;
+        Mb[(long)j * cx + col] = cscale(res[c * N + kp], s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// socs_accum: one block = one image x GR consecutive conv-grid rows.
+//   M    [B][W][W] float2 (mask spectrum window, W = 2Nx+1, square)
+//   bank [nk][W][W] float2 (flipped kernels), scales[nk] f32
+//   out  [B][G][G] f32 intensity (fftshifted when SHIFT, i.e. tmpImgp)
+// Kernel k: Z = K_k * M placed at natural frequency indices (kx mod G),
+// column IFFTs over ky for every window column, keep rows [g0, g0+GR) in LDS,
+// row IFFTs over kx, acc += lambda_k |.|^2.  |IFFT|^2 does not depend on where
+// the window sits in the grid (a frequency shift is a phase ramp), so this
+// equals the reference's centred embed (L847-848, L909-936).
+// ---------------------------------------------------------------------------
+template <int G, int GR>
+__global__ __launch_bounds__(512) void socs_accum_kernel(const float2* __restrict__ M,
+                                                         const float2* __restrict__ bank,
+                                                         const float* __restrict__ scales,
+                                                         int nk, int Nx, float* __restrict__ out,
+                                                         int shift,
+                                                         const float2* __restrict__ tw) {
+    constexpr int NTH = 512;
+    constexpr int NF = (2048 / G) < GR ? (2048 / G) : GR;  // FFTs in flight (<= 32 KB LDS)
+    constexpr int PER = NF * G / NTH;       // outputs per thread per row batch
+    constexpr int NB = GR / NF;             // row batches per kernel
+    static_assert(NF >= 1 && PER >= 1 && NB >= 1, "geometry");
+    extern __shared__ float2 lds[];
+    float2* fa = lds;
+    float2* fb = lds + NF * G;
+    float2* Y = lds + 2 * NF * G;           // [GR][W]
+    const int W = 2 * Nx + 1;
+    const int tid = threadIdx.x;
+    const int nblk = G / GR;
+    const long b = blockIdx.x / nblk;
+    const int g0 = (blockIdx.x - (int)b * nblk) * GR;
+    const float2* Mb = M + b * (long)W * W;
+
+    float acc[NB * PER];
+#pragma unroll
+    for (int i = 0; i < NB * PER; i++) acc[i] = 0.f;
+
+    for (int k = 0; k < nk; k++) {
+        const float2* Kk = bank + (long)k * W * W;
+        const float lam = scales[k];
+        // column pass: NF window columns at a time
+        for (int c0 = 0; c0 < W; c0 += NF) {
+            for (int i = tid; i < NF * G; i += NTH) {
+                int c = i / G, g = i - c * G;
+                int col = c0 + c;
+                int ky = g < G / 2 ? g : g - G;
+                float2 v = make_float2(0.f, 0.f);
+                if (col < W && ky >= -Nx && ky <= Nx) {
+                    int j = (ky + Nx) * W + col;
+                    v = cmul(Kk[j], Mb[j]);
+                }
+                fa[i] = v;
+            }
+            __syncthreads();
+            float2* res = fft_lds<G, true>(fa, fb, NF, tw, tid, NTH);
+            for (int i = tid; i < NF * GR; i += NTH) {
+                int c = i / GR, r = i - c * GR;
+                int col = c0 + c;
+                if (col < W) Y[r * W + col] = res[c * G + g0 + r];
+            }
+            __syncthreads();
+        }
+        // row pass: NF conv-grid rows at a time, accumulate intensity
+#pragma unroll
+        for (int rb = 0; rb < NB; rb++) {
+            for (int i = tid; i < NF * G; i += NTH) {
+                int rr = i / G, h = i - rr * G;
+                int kx = h < G / 2 ? h : h - G;
+                fa[i] = (kx >= -Nx && kx <= Nx) ? Y[(rb * NF + rr) * W + kx + Nx]
+                                                : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+            float2* res = fft_lds<G, true>(fa, fb, NF, tw, tid, NTH);
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                float2 a = res[tid + q * NTH];
+                acc[rb * PER + q] += lam * (a.x * a.x + a.y * a.y);
+            }
+            __syncthreads();
+        }
+    }
+    float* ob = out + b * (long)G * G;
+#pragma unroll
+    for (int rb = 0; rb < NB; rb++)
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int i = tid + q * NTH;
+            int rr = i / G, h = i - rr * G;
+            int row = g0 + rb * NF + rr;
+            if (shift) ob[(long)((row + G / 2) % G) * G + (h + G / 2) % G] = acc[rb * PER + q];
+            else ob[(long)row * G + h] = acc[rb * PER + q];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// fi_cols: P [B][G][nb] float2 (row DFT of the unshifted conv-grid intensity,
+// kx = 0..band) -> U [B][N][nb] float2.  For column kx:
+//   C[ky] = (1/G^2) DFT_G(P[:,kx])[ky mod G], |ky| <= band
+//   U[y][kx] = (-1)^kx sum_ky (-1)^ky C[ky] e^{+2 pi i ky y / N}
+// CG columns per block, grid (ceil(nb/CG), B).  yalt = 1 when the intensity
+// rows came from a centred (fftshifted) grid: its (-1)^ky cancels the
+// centring sign, so no sign is applied along ky.
+// ---------------------------------------------------------------------------
+template <int G, int N, int CG>
+__global__ __launch_bounds__(256) void fi_cols_kernel(const float2* __restrict__ P,
+                                                      float2* __restrict__ U, int band, int yalt,
+                                                      const float2* __restrict__ tw) {
+    __shared__ float2 sa[CG * N];
+    __shared__ float2 sb[CG * N];
+    const int tid = threadIdx.x;
+    const int nb = band + 1;
+    const int c0 = blockIdx.x * CG;
+    const long b = blockIdx.y;
+    const float2* Pb = P + b * (long)G * nb;
+    for (int i = tid; i < CG * G; i += 256) {
+        int g = i / CG, c = i - g * CG;
+        int col = c0 + c;
+        sa[c * G + g] = col < nb ? Pb[(long)g * nb + col] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2* r1 = fft_lds<G, false>(sa, sb, CG, tw, tid, 256);
+    float2* other = (r1 == sa) ? sb : sa;
+    // embed the band into an N-point spectrum with the centring signs
+    const float inv = 1.0f / ((float)G * (float)G);
+    for (int i = tid; i < CG * N; i += 256) {
+        int c = i / N, y = i - c * N;
+        int ky = y < N / 2 ? y : y - N;
+        float2 v = make_float2(0.f, 0.f);
+        if (ky >= -band && ky <= band) {
+            int src = ((ky % G) + G) % G;
+            float s = (!yalt && (ky & 1)) ? -inv : inv;
+            v = cscale(r1[c * G + src], s);
+        }
+        other[i] = v;
+    }
+    __syncthreads();
+    float2* spare = (other == sa) ? sb : sa;
+    float2* r2 = fft_lds<N, true>(other, spare, CG, tw, tid, 256);
+    float2* Ub = U + b * (long)N * nb;
+    for (int i = tid; i < CG * N; i += 256) {
+        int y = i / CG, c = i - y * CG;
+        int col = c0 + c;
+        if (col >= nb) continue;
+        float2 v = r2[c * N + y];
+        if (col & 1) v = cscale(v, -1.f);
+        Ub[(long)y * nb + col] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// image_rows: U [B*N][nb] float2 -> image [B*N][N] f32.  Two rows per complex
+// FFT: Z[kx] = U1[kx] + i U2[kx], Z[-kx] = conj(U1[kx]) + i conj(U2[kx]),
+// DC taken real (the C2R ignores its imaginary part); inverse FFT gives row 1
+// in the real part and row 2 in the imaginary part.
+// ---------------------------------------------------------------------------
+template <int N, int FPB>
+__global__ __launch_bounds__(256) void image_rows_kernel(const float2* __restrict__ U,
+                                                         float* __restrict__ img, int nrows,
+                                                         int band,
+                                                         const float2* __restrict__ tw) {
+    __shared__ float2 sa[FPB * N];
+    __shared__ float2 sb[FPB * N];
+    const int tid = threadIdx.x;
+    const int nb = band + 1;
+    const long pair0 = (long)blockIdx.x * FPB;
+    const long npairs = nrows / 2;
+    for (int i = tid; i < FPB * N; i += 256) sa[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    for (int i = tid; i < FPB * nb; i += 256) {
+        int f = i / nb, kx = i - f * nb;
+        long p = pair0 + f;
+        if (p >= npairs) continue;
+        float2 u1 = U[(2 * p) * (long)nb + kx];
+        float2 u2 = U[(2 * p + 1) * (long)nb + kx];
+        if (kx == 0) {
+            sa[f * N] = make_float2(u1.x, u2.x);
+        } else {
+            // u1 + i u2 and conj(u1) + i conj(u2)
+            sa[f * N + kx] = make_float2(u1.x - u2.y, u1.y + u2.x);
+            sa[f * N + N - kx] = make_float2(u1.x + u2.y, u2.x - u1.y);
+        }
+    }
+    __syncthreads();
+    float2* res = fft_lds<N, true>(sa, sb, FPB, tw, tid, 256);
+    for (int i = tid; i < FPB * N; i += 256) {
+        int f = i / N, x = i - f * N;
+        long p = pair0 + f;
+        if (p >= npairs) continue;
+        float2 v = res[i];
+        img[(2 * p) * (long)N + x] = v.x;
+        img[(2 * p + 1) * (long)N + x] = v.y;
+    }
+}
+
+// ===========================================================================
+// host-side launchers with size dispatch
+// ===========================================================================
+
+template <int L>
+static int launch_rows_dft_L(const float* in, float2* out, long nrows, int kmin, int count,
+                             bool alt, const float2* tw, hipStream_t st) {
+    constexpr int FPB = (2048 / L) >= 1 ? (2048 / L) : 1;
+    unsigned grid = cdiv((size_t)(nrows / 2), FPB);
+    if (alt)
+        hipLaunchKernelGGL((rows_dft_kernel<L, FPB, true>), dim3(grid), dim3(256), 0, st, in, out,
+                           (int)nrows, kmin, count, tw);
+    else
+        hipLaunchKernelGGL((rows_dft_kernel<L, FPB, false>), dim3(grid), dim3(256), 0, st, in,
+                           out, (int)nrows, kmin, count, tw);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_rows_dft(int L, const float* in, float2* out, long nrows, int kmin, int count,
+                    bool alt, const float2* tw, hipStream_t st) {
+    if (nrows % 2 || count <= 0 || nrows > 0x7fffffffL) return LITHO_EARG;
+    switch (L) {
+        case 32: return launch_rows_dft_L<32>(in, out, nrows, kmin, count, alt, tw, st);
+        case 64: return launch_rows_dft_L<64>(in, out, nrows, kmin, count, alt, tw, st);
+        case 128: return launch_rows_dft_L<128>(in, out, nrows, kmin, count, alt, tw, st);
+        case 256: return launch_rows_dft_L<256>(in, out, nrows, kmin, count, alt, tw, st);
+        case 512: return launch_rows_dft_L<512>(in, out, nrows, kmin, count, alt, tw, st);
+        case 1024: return launch_rows_dft_L<1024>(in, out, nrows, kmin, count, alt, tw, st);
+        case 2048: return launch_rows_dft_L<2048>(in, out, nrows, kmin, count, alt, tw, st);
+        case 4096: return launch_rows_dft_L<4096>(in, out, nrows, kmin, count, alt, tw, st);
+        default: return LITHO_EARG;
+    }
+}
+
+template <int N>
+static int launch_mask_cols_N(const float2* R, float2* M, int B, int cx, int kymin, int cy,
+                              const float2* tw, hipStream_t st) {
+    constexpr int CG = (2048 / N) >= 1 ? (2048 / N) : 1;
+    dim3 grid(cdiv(cx, CG), B);
+    hipLaunchKernelGGL((mask_cols_kernel<N, CG>), grid, dim3(256), 0, st, R, M, cx, kymin, cy, tw);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_mask_cols(int N, const float2* R, float2* M, int B, int cx, int kymin, int cy,
+                     const float2* tw, hipStream_t st) {
+    if (B <= 0 || B > 65535 || cx <= 0 || cy <= 0) return LITHO_EARG;
+    switch (N) {
+        case 32: return launch_mask_cols_N<32>(R, M, B, cx, kymin, cy, tw, st);
+        case 64: return launch_mask_cols_N<64>(R, M, B, cx, kymin, cy, tw, st);
+        case 128: return launch_mask_cols_N<128>(R, M, B, cx, kymin, cy, tw, st);
+        case 256: return launch_mask_cols_N<256>(R, M, B, cx, kymin, cy, tw, st);
+        case 512: return launch_mask_cols_N<512>(R, M, B, cx, kymin, cy, tw, st);
+        case 1024: return launch_mask_cols_N<1024>(R, M, B, cx, kymin, cy, tw, st);
+        case 2048: return launch_mask_cols_N<2048>(R, M, B, cx, kymin, cy, tw, st);
+        case 4096: return launch_mask_cols_N<4096>(R, M, B, cx, kymin, cy, tw, st);
+        default: return LITHO_EARG;
+    }
+}
+
+template <int G, int GR>
+static int launch_socs_GR(const float2* M, const float2* bank, const float* scales, int nk,
+                          int Nx, int B, float* out, int shift, const float2* tw,
+                          hipStream_t st) {
+    const int W = 2 * Nx + 1;
+    constexpr int NF = (2048 / G) < GR ? (2048 / G) : GR;
+    size_t lds = (size_t)(2 * NF * G + GR * W) * sizeof(float2);
+    if (lds > 160 * 1024) return LITHO_EARG;
+    auto kern = socs_accum_kernel<G, GR>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024) != hipSuccess)
+            return LITHO_EHIP;
+        attr_set = true;
+    }
+    unsigned grid = (unsigned)((size_t)B * (G / GR));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, st, M, bank, scales, nk, Nx, out, shift,
+                       tw);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int socs_rows_per_block(int G, int Nx) {
+    const int W = 2 * Nx + 1;
+    int cands[3];
+    int nc = 0;
+    switch (G) {
+        case 32: cands[nc++] = 32; break;
+        case 64: cands[nc++] = 64; break;
+        case 128: cands[nc++] = 128; cands[nc++] = 64; break;
+        case 256: cands[nc++] = 128; cands[nc++] = 64; cands[nc++] = 32; break;
+        case 512: cands[nc++] = 64; cands[nc++] = 32; cands[nc++] = 16; break;
+        case 1024: cands[nc++] = 32; cands[nc++] = 16; break;
+        default: return 0;
+    }
+    for (int i = 0; i < nc; i++)
+        if ((size_t)(2 * 2048 + cands[i] * W) * sizeof(float2) <= 160 * 1024) return cands[i];
+    return 0;
+}
+
+int launch_socs_accum(int G, const float2* M, const float2* bank, const float* scales, int nk,
+                      int Nx, int B, float* out, int shift, const float2* tw, hipStream_t st) {
+    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G) return LITHO_EARG;
+    int GR = socs_rows_per_block(G, Nx);
+    if (!GR) return LITHO_EARG;
+#define LITHO_SOCS_CASE(g, gr) \
+    if (G == g && GR == gr) return launch_socs_GR<g, gr>(M, bank, scales, nk, Nx, B, out, shift, tw, st);
+    LITHO_SOCS_CASE(32, 32)
+    LITHO_SOCS_CASE(64, 64)
+    LITHO_SOCS_CASE(128, 128)
+    LITHO_SOCS_CASE(128, 64)
+    LITHO_SOCS_CASE(256, 128)
+    LITHO_SOCS_CASE(256, 64)
+    LITHO_SOCS_CASE(256, 32)
+    LITHO_SOCS_CASE(512, 64)
+    LITHO_SOCS_CASE(512, 32)
+    LITHO_SOCS_CASE(512, 16)
+    LITHO_SOCS_CASE(1024, 32)
+    LITHO_SOCS_CASE(1024, 16)
+#undef LITHO_SOCS_CASE
+    return LITHO_EARG;
+}
+
+template <int G, int N>
+static int launch_fi_cols_GN(const float2* P, float2* U, int B, int band, int yalt,
+                             const float2* tw, hipStream_t st) {
+    constexpr int CG = (2048 / N) >= 1 ? (2048 / N) : 1;
+    dim3 grid(cdiv(band + 1, CG), B);
+    hipLaunchKernelGGL((fi_cols_kernel<G, N, CG>), grid, dim3(256), 0, st, P, U, band, yalt, tw);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+template <int G>
+static int launch_fi_cols_G(int N, const float2* P, float2* U, int B, int band, int yalt,
+                            const float2* tw, hipStream_t st) {
+    switch (N) {
+        case 32: if constexpr (G <= 32) return launch_fi_cols_GN<G, 32>(P, U, B, band, yalt, tw, st); break;
+        case 64: if constexpr (G <= 64) return launch_fi_cols_GN<G, 64>(P, U, B, band, yalt, tw, st); break;
+        case 128: if constexpr (G <= 128) return launch_fi_cols_GN<G, 128>(P, U, B, band, yalt, tw, st); break;
+        case 256: if constexpr (G <= 256) return launch_fi_cols_GN<G, 256>(P, U, B, band, yalt, tw, st); break;
+        case 512: if constexpr (G <= 512) return launch_fi_cols_GN<G, 512>(P, U, B, band, yalt, tw, st); break;
+        case 1024: if constexpr (G <= 1024) return launch_fi_cols_GN<G, 1024>(P, U, B, band, yalt, tw, st); break;
+        case 2048: return launch_fi_cols_GN<G, 2048>(P, U, B, band, yalt, tw, st);
+        case 4096: return launch_fi_cols_GN<G, 4096>(P, U, B, band, yalt, tw, st);
+        default: break;
+    }
+    return LITHO_EARG;
+}
+
+int launch_fi_cols(int G, int N, const float2* P, float2* U, int B, int band, int yalt,
+                   const float2* tw, hipStream_t st) {
+    if (B <= 0 || B > 65535 || band < 0 || 2 * band + 1 > G || G > N) return LITHO_EARG;
+    switch (G) {
+        case 32: return launch_fi_cols_G<32>(N, P, U, B, band, yalt, tw, st);
+        case 64: return launch_fi_cols_G<64>(N, P, U, B, band, yalt, tw, st);
+        case 128: return launch_fi_cols_G<128>(N, P, U, B, band, yalt, tw, st);
+        case 256: return launch_fi_cols_G<256>(N, P, U, B, band, yalt, tw, st);
+        case 512: return launch_fi_cols_G<512>(N, P, U, B, band, yalt, tw, st);
+        case 1024: return launch_fi_cols_G<1024>(N, P, U, B, band, yalt, tw, st);
+        default: return LITHO_EARG;
+    }
+}
+
+template <int N>
+static int launch_image_rows_N(const float2* U, float* img, long nrows, int band,
+                               const float2* tw, hipStream_t st) {
+    constexpr int FPB = (2048 / N) >= 1 ? (2048 / N) : 1;
+    unsigned grid = cdiv((size_t)(nrows / 2), FPB);
+    hipLaunchKernelGGL((image_rows_kernel<N, FPB>), dim3(grid), dim3(256), 0, st, U, img,
+                       (int)nrows, band, tw);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, const float2* tw,
+                      hipStream_t st) {
+    if (nrows % 2 || 2 * band + 1 > N || nrows > 0x7fffffffL) return LITHO_EARG;
+    switch (N) {
+        case 32: return launch_image_rows_N<32>(U, img, nrows, band, tw, st);
+        case 64: return launch_image_rows_N<64>(U, img, nrows, band, tw, st);
+        case 128: return launch_image_rows_N<128>(U, img, nrows, band, tw, st);
+        case 256: return launch_image_rows_N<256>(U, img, nrows, band, tw, st);
+        case 512: return launch_image_rows_N<512>(U, img, nrows, band, tw, st);
+        case 1024: return launch_image_rows_N<1024>(U, img, nrows, band, tw, st);
+        case 2048: return launch_image_rows_N<2048>(U, img, nrows, band, tw, st);
+        case 4096: return launch_image_rows_N<4096>(U, img, nrows, band, tw, st);
+        default: return LITHO_EARG;
+    }
+}
+
+}  // namespace litho

@@ -1,0 +1,210 @@
+// TCC build kernels for gfx950 (Module 4, calcTCC, litho_thread.cpp L441-532).
+//
+//   pupil_kernel   : A[ID][s] = P(f_ID + s) for every live source point, with
+//                    the reference's float arithmetic (L458-484), no FMA
+//                    contraction, so the 0/1 support is bit-identical.
+//   herm_gemm_mfma : C = (X diag(sx)) Y^H in FP64 on v_mfma_f64_16x16x4f64,
+//                    upper block triangle only, lower mirrored as conj (the
+//                    reference fills its lower triangle the same way, L527-532).
+//                    TCC = A diag(S) A^H; Gram = Bt Bt^H with Bt = conj(A)^T
+//                    diag(sqrt S).
+//   conj_transpose_scale : Bt[s][i] = conj(A[i][s]) * sqrt(S[s]).
+#include "common.hpp"
+#include "tcc_kernels.hpp"
+
+namespace litho {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// pupil: one thread per (ID, s).  srcs[s] = {p, q}; geometry in `g`.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pupil_kernel(const int2* __restrict__ srcs, int nsrc,
+                                                    PupilGeom g, double2* __restrict__ A,
+                                                    int* __restrict__ range_err) {
+#pragma clang fp contract(off)
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = 2 * g.Nx + 1;
+    const long n = (long)W * (2 * g.Ny + 1);
+    if (idx >= n * nsrc) return;
+    const int s = (int)(idx % nsrc);
+    const int ID = (int)(idx / nsrc);
+    const int nx = ID % W - g.Nx;
+    const int ny = ID / W - g.Ny;
+    const int p = srcs[s].x, q = srcs[s].y;
+    const float sx = (float)p / (float)g.sh;
+    const float sy = (float)q / (float)g.sh;
+    const int nxMin = (int)((-1.0f - sx) * g.Lx_norm);
+    const int nxMax = (int)((1.0f - sx) * g.Lx_norm);
+    const int nyMin = (int)((-1.0f - sy) * g.Ly_norm);
+    const int nyMax = (int)((1.0f - sy) * g.Ly_norm);
+    double2 v = make_double2(0.0, 0.0);
+    if (nx >= nxMin && nx <= nxMax && ny >= nyMin && ny <= nyMax) {
+        const float fy = (float)ny / g.Ly_norm + sy;
+        if (fy * fy <= 1.0f) {
+            const float fx = (float)nx / g.Lx_norm + sx;
+            const float rho2 = fx * fx + fy * fy;
+            if (rho2 <= 1.0f) {
+                const float na2 = rho2 * g.NA * g.NA;
+                const float ph = (float)((double)(g.dz * g.k) * sqrt(1.0 - (double)na2));
+                v = make_double2((double)cosf(ph), (double)sinf(ph));
+            }
+        }
+    }
+    A[(long)ID * nsrc + s] = v;
+    // The reference's loops can reach (nx,ny) outside the window; detect that
+    // (only the edge IDs can see it, so test the box against the window).
+    if (ID == 0 && (nxMin < -g.Nx || nxMax > g.Nx || nyMin < -g.Ny || nyMax > g.Ny)) {
+        // out-of-window points matter only if they pass the disk test; check them
+        for (int yy = nyMin; yy <= nyMax; yy++) {
+            const float fy = (float)yy / g.Ly_norm + sy;
+            if (!(fy * fy <= 1.0f)) continue;
+            for (int xx = nxMin; xx <= nxMax; xx++) {
+                if (xx >= -g.Nx && xx <= g.Nx && yy >= -g.Ny && yy <= g.Ny) continue;
+                const float fx = (float)xx / g.Lx_norm + sx;
+                if (fx * fx + fy * fy <= 1.0f) atomicOr(range_err, 1);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bt[s][i] = conj(A[i][s]) * sqrt(S[s])  (A: [n][m], Bt: [m][n])
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conj_transpose_scale_kernel(const double2* __restrict__ A,
+                                                                   const double* __restrict__ S,
+                                                                   int n, int m,
+                                                                   double2* __restrict__ Bt) {
+    __shared__ double2 tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int i0 = blockIdx.y * 32, s0 = blockIdx.x * 32;
+    for (int r = ty; r < 32; r += 8) {
+        int i = i0 + r, s = s0 + tx;
+        tile[r][tx] = (i < n && s < m) ? A[(long)i * m + s] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        int s = s0 + r, i = i0 + tx;
+        if (s < m && i < n) {
+            double2 a = tile[tx][r];
+            double f = sqrt(S[s]);
+            Bt[(long)s * n + i] = make_double2(a.x * f, -a.y * f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C = (X diag(sx)) Y^H, X: [n][m], Y: [n][m] complex double, C: [n][n].
+// Block = 4 waves computing a 32 x 32 tile (each wave 16 x 16), K staged in
+// LDS 16 at a time.  Only blocks with tj >= ti run; they also write the
+// conjugate mirror.  MFMA f64 16x16x4 lane layout (measured on gfx950): A
+// operand lane l holds A[l%16][l/16], B operand B[l/16][l%16], result
+// register v of lane l is C[l/16 + 4v][l%16].
+// ---------------------------------------------------------------------------
+constexpr int TG_T = 32;  // tile
+constexpr int TG_K = 16;  // K stage
+
+__global__ __launch_bounds__(256) void herm_gemm_mfma_kernel(const double2* __restrict__ X,
+                                                             const double* __restrict__ sx,
+                                                             const double2* __restrict__ Y, int n,
+                                                             int m, int has_imag,
+                                                             double2* __restrict__ C) {
+    const int nt = (n + TG_T - 1) / TG_T;
+    // linear block index -> (ti, tj) with tj >= ti
+    long bid = blockIdx.x;
+    int ti = 0;
+    {
+        long rowlen = nt;
+        while (bid >= rowlen) {
+            bid -= rowlen;
+            ti++;
+            rowlen--;
+        }
+    }
+    const int tj = ti + (int)bid;
+    __shared__ double2 Xs[TG_T][TG_K + 1];
+    __shared__ double2 Ys[TG_T][TG_K + 1];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int i0 = ti * TG_T, j0 = tj * TG_T;
+    d4 cr = {0.0, 0.0, 0.0, 0.0};
+    d4 ci = {0.0, 0.0, 0.0, 0.0};
+    for (int s0 = 0; s0 < m; s0 += TG_K) {
+        for (int e = tid; e < TG_T * TG_K; e += 256) {
+            int r = e / TG_K, c = e - r * TG_K;
+            int s = s0 + c;
+            double2 xv = make_double2(0.0, 0.0), yv = make_double2(0.0, 0.0);
+            if (s < m) {
+                if (i0 + r < n) {
+                    xv = X[(long)(i0 + r) * m + s];
+                    if (sx) {
+                        double f = sx[s];
+                        xv.x *= f;
+                        xv.y *= f;
+                    }
+                }
+                if (j0 + r < n) yv = Y[(long)(j0 + r) * m + s];
+            }
+            Xs[r][c] = xv;
+            Ys[r][c] = yv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TG_K; kk += 4) {
+            double2 a = Xs[wi * 16 + (lane & 15)][kk + (lane >> 4)];
+            double2 b = Ys[wj * 16 + (lane & 15)][kk + (lane >> 4)];
+            // C += X Y^H : Cr += Xr Yr + Xi Yi ; Ci += Xi Yr - Xr Yi
+            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr, 0, 0, 0);
+            if (has_imag) {
+                cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, cr, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, b.y, ci, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        int i = i0 + wi * 16 + (lane >> 4) + 4 * v;
+        int j = j0 + wj * 16 + (lane & 15);
+        if (i >= n || j >= n) continue;
+        if (ti == tj && j < i) continue;  // diagonal tile: upper part only
+        double re = cr[v], im = ci[v];
+        if (i == j) {
+            C[(long)i * n + j] = make_double2(re, 0.0);
+        } else {
+            C[(long)i * n + j] = make_double2(re, im);
+            C[(long)j * n + i] = make_double2(re, -im);
+        }
+    }
+}
+
+int launch_pupil(const int2* srcs, int nsrc, const PupilGeom& g, double2* A, int* range_err,
+                 hipStream_t st) {
+    long n = (long)(2 * g.Nx + 1) * (2 * g.Ny + 1);
+    long total = n * nsrc;
+    if (total <= 0) return LITHO_EARG;
+    hipLaunchKernelGGL(pupil_kernel, dim3(cdiv(total, 256)), dim3(256), 0, st, srcs, nsrc, g, A,
+                       range_err);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m, double2* Bt,
+                                hipStream_t st) {
+    dim3 grid(cdiv(m, 32), cdiv(n, 32));
+    hipLaunchKernelGGL(conj_transpose_scale_kernel, grid, dim3(256), 0, st, A, S, n, m, Bt);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
+                     int has_imag, double2* C, hipStream_t st) {
+    if (n <= 0 || m <= 0) return LITHO_EARG;
+    long nt = (n + TG_T - 1) / TG_T;
+    long nblocks = nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(herm_gemm_mfma_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, X, sx, Y,
+                       n, m, has_imag, C);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+}  // namespace litho

@@ -1,0 +1,27 @@
+// Host launchers of the TCC build kernels (tcc_kernels.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace litho {
+
+// calcTCC scalars (litho_thread.cpp L443-448), all in the reference's types.
+struct PupilGeom {
+    int sh;         // (srcSize-1)/2
+    int Nx, Ny;
+    float dz;       // defocus / (NA*NA/lambda)
+    float k;        // (float)(2*pi/lambda)
+    float Lx_norm;  // Lx*NA/lambda
+    float Ly_norm;
+    float NA;
+};
+
+int launch_pupil(const int2* srcs, int nsrc, const PupilGeom& g, double2* A, int* range_err,
+                 hipStream_t st);
+int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m, double2* Bt,
+                                hipStream_t st);
+// C = (X diag(sx)) Y^H (sx may be null), Hermitian output, n x n
+int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
+                     int has_imag, double2* C, hipStream_t st);
+
+}  // namespace litho

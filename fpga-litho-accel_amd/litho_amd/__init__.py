@@ -1,0 +1,304 @@
+"""Python mirror of the reference's SOCS call surface over the MI355X C-ABI.
+
+The functions keep the names and argument meaning of the reference's CPU
+oracle (validation/golden/src/litho_thread.cpp): FT_r2c_padded (L255),
+calcTCC (L441), calcKernels (L597), FI (L692), calcSOCS (L813), plus the HLS
+accelerator top calc_socs_2048_hls_stream_refactored_v18
+(source/SOCS_HLS/src/socs_2048_stream_refactored_v18_hls_fft.cpp L287).
+Everything runs in liblitho_socs.so on the GPU; a missing library or GPU
+raises LithoError -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(os.path.dirname(_HERE), "liblitho_socs.so")
+_LIB = None
+
+LITHO_FLAG_CLUSTER_SPLIT = 1
+LITHO_FLAG_GRAM_ROUTE = 2
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+# every symbol include/litho_socs.h declares
+EXPORTS = [
+    "litho_status_string", "litho_version", "litho_ctx_create", "litho_ctx_destroy",
+    "litho_ctx_sync", "litho_dev_alloc", "litho_dev_free", "litho_memcpy_h2d",
+    "litho_memcpy_d2h", "litho_memset_dev", "litho_source_annular", "litho_source_quadrupole",
+    "litho_source_dipole", "litho_source_point", "litho_outer_sigma", "litho_compute_nx",
+    "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
+    "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
+    "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_last_timing",
+]
+
+
+class LithoError(RuntimeError):
+    def __init__(self, status, what):
+        msg = lib().litho_status_string(status).decode() if _LIB is not None else str(status)
+        super().__init__(f"{what}: status {status} ({msg})")
+        self.status = status
+
+
+class LithoOptics(C.Structure):
+    _fields_ = [("lambda_", C.c_float), ("NA", C.c_float), ("defocus", C.c_float),
+                ("outerSigma", C.c_float), ("Lx", C.c_int), ("Ly", C.c_int),
+                ("Nx", C.c_int), ("Ny", C.c_int)]
+
+
+def library_path():
+    return _LIBPATH
+
+
+def lib():
+    """Load liblitho_socs.so (built in-tree); raises if it is missing."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIBPATH):
+            raise RuntimeError(f"{_LIBPATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(_LIBPATH)
+        i, f, vp, sz = C.c_int, C.c_float, C.c_void_p, C.c_size_t
+        L.litho_status_string.argtypes = [i]
+        L.litho_status_string.restype = C.c_char_p
+        L.litho_version.restype = i
+        L.litho_ctx_create.argtypes = [C.POINTER(vp), i]
+        L.litho_ctx_destroy.argtypes = [vp]
+        L.litho_ctx_sync.argtypes = [vp]
+        L.litho_dev_alloc.argtypes = [vp, C.POINTER(vp), sz]
+        L.litho_dev_free.argtypes = [vp, vp]
+        L.litho_memcpy_h2d.argtypes = [vp, vp, vp, sz]
+        L.litho_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+        L.litho_memset_dev.argtypes = [vp, vp, i, sz]
+        L.litho_source_annular.argtypes = [_f32p, i, f, f]
+        L.litho_source_quadrupole.argtypes = [_f32p, i, f, f]
+        L.litho_source_dipole.argtypes = [_f32p, i, f, f, i]
+        L.litho_source_point.argtypes = [_f32p, i, f, f]
+        L.litho_outer_sigma.argtypes = [_f32p, i]
+        L.litho_outer_sigma.restype = f
+        L.litho_compute_nx.argtypes = [f, i, f, f]
+        L.litho_manhattan_mask.argtypes = [_f32p, i, C.c_ulonglong, i, i, f]
+        L.litho_mask_spectrum.argtypes = [_f32p, i, _f32p, _f32p]
+        L.litho_tcc_build.argtypes = [C.POINTER(LithoOptics), _f32p, i, _f64p, _f64p]
+        L.litho_kernels_topk.argtypes = [_f64p, _f64p, i, i, i, i, _f64p, _f32p, _f32p,
+                                         C.POINTER(i)]
+        L.litho_kernels_from_source.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p,
+                                                _f32p, _f32p, C.POINTER(i)]
+        L.litho_socs_tmpimg.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                        i, i, i, i, i]
+        L.litho_socs_image.argtypes = [_f32p, i, _f32p, _f32p, _f32p, i, i, i, _f32p]
+        L.litho_fi.argtypes = [_f32p, i, i, i, _f32p]
+        L.litho_bank_create.argtypes = [vp, _f32p, _f32p, _f32p, i, i, i, C.POINTER(vp)]
+        L.litho_bank_destroy.argtypes = [vp]
+        L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
+        L.litho_last_timing.argtypes = [vp, C.POINTER(f), C.POINTER(f), i]
+        for name in EXPORTS:
+            fn = getattr(L, name)
+            if name not in ("litho_status_string", "litho_outer_sigma"):
+                fn.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _check(status, what):
+    if status != 0:
+        raise LithoError(status, what)
+
+
+def _planes(z):
+    z = np.asarray(z)
+    return (np.ascontiguousarray(z.real, dtype=np.float32).ravel(),
+            np.ascontiguousarray(z.imag, dtype=np.float32).ravel())
+
+
+# ---- Module 1 ---------------------------------------------------------------
+def createSource(size, kind="Annular", inner=0.6, outer=0.9, radius=0.5, offset=0.3,
+                 on_x=True, px=0.0, py=0.0):
+    """createSource (common/source.cpp L105-141) incl. normalizeMatrix."""
+    s = np.zeros(size * size, np.float32)
+    L = lib()
+    if kind == "Annular":
+        rc = L.litho_source_annular(s, size, inner, outer)
+    elif kind == "CrossQuadrupole":
+        rc = L.litho_source_quadrupole(s, size, radius, offset)
+    elif kind == "Dipole":
+        rc = L.litho_source_dipole(s, size, radius, offset, int(on_x))
+    elif kind == "Point":
+        rc = L.litho_source_point(s, size, px, py)
+    else:
+        raise ValueError(kind)
+    _check(rc, "createSource")
+    return s.reshape(size, size)
+
+
+def getOuterSigma(src):
+    """getOuterSigma (L427-439), clamped to 1 as main does (L1344)."""
+    src = np.ascontiguousarray(src, np.float32)
+    return float(lib().litho_outer_sigma(src.ravel(), src.shape[0]))
+
+
+def compute_nx(NA, period, outer_sigma, wavelength):
+    """Nx = floor(NA*period*(1+sigma)/lambda) in float arithmetic (L1346)."""
+    return int(lib().litho_compute_nx(NA, int(period), outer_sigma, wavelength))
+
+
+def manhattan_mask(N, seed, wmin=12, wmax=45, fill=0.40):
+    """Seeded synthetic Manhattan mask (PCG32), float32 0/1, N x N."""
+    m = np.zeros(N * N, np.float32)
+    _check(lib().litho_manhattan_mask(m, N, seed, wmin, wmax, fill), "litho_manhattan_mask")
+    return m.reshape(N, N)
+
+
+def optics(wavelength, NA, defocus, outer_sigma, Lx, Ly, Nx, Ny):
+    return LithoOptics(wavelength, NA, defocus, outer_sigma, int(Lx), int(Ly), int(Nx), int(Ny))
+
+
+# ---- Module 3 ---------------------------------------------------------------
+def FT_r2c_padded(mask):
+    """Full centred mask spectrum / N^2 (L255-331), complex64 N x N."""
+    mask = np.ascontiguousarray(mask, np.float32)
+    N = mask.shape[0]
+    r = np.zeros(N * N, np.float32)
+    i = np.zeros(N * N, np.float32)
+    _check(lib().litho_mask_spectrum(mask.ravel(), N, r, i), "litho_mask_spectrum")
+    return (r + 1j * i).astype(np.complex64).reshape(N, N)
+
+
+# ---- Module 4 ---------------------------------------------------------------
+def calcTCC(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny):
+    """TCC = A diag(S) A^H (L441-532), complex128 n x n, built on the GPU."""
+    src = np.ascontiguousarray(src, np.float32)
+    n = (2 * Nx + 1) * (2 * Ny + 1)
+    re = np.zeros(n * n)
+    im = np.zeros(n * n)
+    op = optics(wavelength, NA, defocus, outerSigma, Lx, Ly, Nx, Ny)
+    _check(lib().litho_tcc_build(C.byref(op), src.ravel(), src.shape[0], re, im),
+           "litho_tcc_build")
+    return (re + 1j * im).reshape(n, n)
+
+
+# ---- Module 6 ---------------------------------------------------------------
+def calcKernels(tcc, nk, Nx, Ny):
+    """Top-nk kernels (L597-663): (scales float64 desc, kernels complex64 nk x n, flags)."""
+    tcc = np.asarray(tcc, np.complex128)
+    n = tcc.shape[0]
+    sc = np.zeros(nk)
+    kr = np.zeros(nk * n, np.float32)
+    ki = np.zeros(nk * n, np.float32)
+    fl = C.c_int(0)
+    _check(lib().litho_kernels_topk(np.ascontiguousarray(tcc.real).ravel(),
+                                    np.ascontiguousarray(tcc.imag).ravel(), n, Nx, Ny, nk, sc,
+                                    kr, ki, C.byref(fl)), "litho_kernels_topk")
+    return sc, (kr + 1j * ki).astype(np.complex64).reshape(nk, n), fl.value
+
+
+def kernels_from_source(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny, nk):
+    """Source -> top-nk kernels on the GPU (explicit or Gram route)."""
+    src = np.ascontiguousarray(src, np.float32)
+    n = (2 * Nx + 1) * (2 * Ny + 1)
+    sc = np.zeros(nk)
+    kr = np.zeros(nk * n, np.float32)
+    ki = np.zeros(nk * n, np.float32)
+    fl = C.c_int(0)
+    op = optics(wavelength, NA, defocus, outerSigma, Lx, Ly, Nx, Ny)
+    _check(lib().litho_kernels_from_source(C.byref(op), src.ravel(), src.shape[0], nk, sc, kr,
+                                           ki, C.byref(fl)), "litho_kernels_from_source")
+    return sc, (kr + 1j * ki).astype(np.complex64).reshape(nk, n), fl.value
+
+
+# ---- Module 7 ---------------------------------------------------------------
+def calc_socs_hls(mskf, krns, scales, nx, ny):
+    """HLS-v18-shaped SOCS (v18 .cpp L287-388): fftshifted G x G tmpImgp."""
+    mskf = np.asarray(mskf)
+    Ly, Lx = mskf.shape
+    mr, mi = _planes(mskf)
+    kr, ki = _planes(krns)
+    sc = np.ascontiguousarray(scales, np.float32)
+    nk = sc.size
+    G = 1
+    while G < 4 * nx + 1:
+        G *= 2
+    out = np.zeros(G * G, np.float32)
+    _check(lib().litho_socs_tmpimg(mr, mi, kr, ki, sc, out, out, nk, nx, ny, Lx, Ly),
+           "litho_socs_tmpimg")
+    return out.reshape(G, G)
+
+
+def FI(tmpImgp, N, band):
+    """Fourier interpolation (L692-772) of a centred G x G grid to N x N."""
+    t = np.ascontiguousarray(tmpImgp, np.float32)
+    G = t.shape[0]
+    out = np.zeros(N * N, np.float32)
+    _check(lib().litho_fi(t.ravel(), G, N, band, out), "litho_fi")
+    return out.reshape(N, N)
+
+
+def calcSOCS(mskf, krns, scales, Nx, Ny):
+    """calcSOCS (L813-1040) from a full centred spectrum: SOCS loop + FI."""
+    mskf = np.asarray(mskf)
+    t = calc_socs_hls(mskf, krns, scales, Nx, Ny)
+    return FI(t, mskf.shape[0], 2 * Nx)
+
+
+def socs_image(mask, krns, scales, Nx, Ny):
+    """Mask -> aerial image on the GPU (Modules 3 + 7 fused pipeline)."""
+    mask = np.ascontiguousarray(mask, np.float32)
+    N = mask.shape[0]
+    kr, ki = _planes(krns)
+    sc = np.ascontiguousarray(scales, np.float32)
+    out = np.zeros(N * N, np.float32)
+    _check(lib().litho_socs_image(mask.ravel(), N, kr, ki, sc, sc.size, Nx, Ny, out),
+           "litho_socs_image")
+    return out.reshape(N, N)
+
+
+# ---- device-resident batch path ----------------------------------------------
+class Context:
+    """One HIP stream + workspaces (litho_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().litho_ctx_create(C.byref(self.h), device), "litho_ctx_create")
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        _check(lib().litho_dev_alloc(self.h, C.byref(p), nbytes), "litho_dev_alloc")
+        return p
+
+    def free(self, p):
+        _check(lib().litho_dev_free(self.h, p), "litho_dev_free")
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().litho_memcpy_h2d(self.h, dst, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def d2h(self, arr, src):
+        _check(lib().litho_memcpy_d2h(self.h, arr.ctypes.data, src, arr.nbytes), "d2h")
+        return arr
+
+    def sync(self):
+        _check(lib().litho_ctx_sync(self.h), "litho_ctx_sync")
+
+    def bank(self, krns, scales, Nx, Ny):
+        kr, ki = _planes(krns)
+        sc = np.ascontiguousarray(scales, np.float32)
+        b = C.c_void_p()
+        _check(lib().litho_bank_create(self.h, kr, ki, sc, sc.size, Nx, Ny, C.byref(b)),
+               "litho_bank_create")
+        return b
+
+    def socs_image_batch(self, bank, d_mask, B, N, d_image):
+        _check(lib().litho_socs_image_batch(self.h, bank, d_mask, B, N, d_image),
+               "litho_socs_image_batch")
+
+    def last_timing(self, nstage=6):
+        tot = C.c_float()
+        st = (C.c_float * nstage)()
+        _check(lib().litho_last_timing(self.h, C.byref(tot), st, nstage), "litho_last_timing")
+        return tot.value, list(st)
+
+    def close(self):
+        if self.h:
+            lib().litho_ctx_destroy(self.h)
+            self.h = None

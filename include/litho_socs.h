@@ -1,0 +1,146 @@
+/*
+ * litho_socs.h -- C-ABI of the MI355X SOCS aerial-image engine.
+ *
+ * Plain C, no HIP or torch types: every buffer is a pointer plus sizes.  The
+ * entry points replace the reference's calcTCC / calcKernels / calcSOCS /
+ * FT_r2c_padded / FI (validation/golden/src/litho_thread.cpp) and the HLS
+ * accelerator top calc_socs_2048_hls_stream_refactored_v18
+ * (source/SOCS_HLS/src/socs_2048_stream_refactored_v18_hls_fft.h L86-97).
+ *
+ * Conventions (identical to the reference):
+ *   - row-major arrays; complex data as separate real / imaginary planes on
+ *     the host side (the BIN layout), interleaved float2 on the device;
+ *   - kernel index ID(nx,ny) = (ny+Ny)*(2Nx+1) + (nx+Nx), kernels stored
+ *     index-flipped K[ID] = v[n-1-ID] (litho_thread.cpp L643-652);
+ *   - mask spectrum centred, DC at [N/2][N/2], divided by N^2;
+ *   - SOCS inverse FFT unnormalised, FI divides its R2C by G^2.
+ *
+ * Errors: every function returns an int status, LITHO_OK (0) or a negative
+ * LITHO_E* code; nothing exits or throws across the boundary.  Host-pointer
+ * entry points are synchronous.  A litho_ctx owns one HIP stream and its
+ * workspaces: use one context per host thread.
+ */
+#ifndef LITHO_SOCS_H
+#define LITHO_SOCS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LITHO_OK 0
+#define LITHO_EARG -1      /* bad size / null pointer / unsupported shape */
+#define LITHO_ENOMEM -2    /* device or host allocation failed */
+#define LITHO_EHIP -3      /* HIP runtime error */
+#define LITHO_ENOCONV -4   /* eigensolver did not converge */
+#define LITHO_ENODEV -5    /* no usable GPU */
+#define LITHO_ERANGE -6    /* pupil support leaves the (2Nx+1)x(2Ny+1) window */
+
+/* flag bits returned through `int* flags` */
+#define LITHO_FLAG_CLUSTER_SPLIT 1 /* nk cuts a degenerate eigenvalue cluster */
+#define LITHO_FLAG_GRAM_ROUTE 2    /* kernels came from the source-space Gram route */
+
+/* Optics and grid of one simulation (the reference's FullConfig subset used
+ * by calcTCC, litho_thread.cpp L441-442).  Lx/Ly are the mask periods in nm
+ * (the reference conflates period and pixel count; the two are separate here). */
+typedef struct {
+    float lambda;      /* nm */
+    float NA;
+    float defocus;     /* nm */
+    float outerSigma;  /* getOuterSigma clamped to 1 (litho_thread.cpp L1343-1344) */
+    int Lx, Ly;        /* period, nm */
+    int Nx, Ny;        /* floor(NA*L*(1+sigma)/lambda) (litho_thread.cpp L1346-1347) */
+} LithoOptics;
+
+typedef struct litho_ctx litho_ctx;   /* device, stream, workspaces, twiddle tables */
+typedef struct litho_bank litho_bank; /* one kernel set resident in HBM */
+
+const char* litho_status_string(int status);
+int litho_version(void);
+
+/* ---- context and device memory (for callers without a HIP toolchain) ---- */
+int litho_ctx_create(litho_ctx** ctx, int device);
+int litho_ctx_destroy(litho_ctx* ctx);
+int litho_ctx_sync(litho_ctx* ctx);
+int litho_dev_alloc(litho_ctx* ctx, void** dptr, size_t bytes);
+int litho_dev_free(litho_ctx* ctx, void* dptr);
+int litho_memcpy_h2d(litho_ctx* ctx, void* dst, const void* src, size_t bytes);
+int litho_memcpy_d2h(litho_ctx* ctx, void* dst, const void* src, size_t bytes);
+int litho_memset_dev(litho_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* ---- Module 1 helpers (host) --------------------------------------------- */
+/* createSource (common/source.cpp L105-141): annular L5-23, cross quadrupole
+ * L25-55, dipole L58-87 (note its size/2 centre), point L122-129, each
+ * followed by normalizeMatrix L89-103 (float sum).  src is srcSize^2 f32. */
+int litho_source_annular(float* src, int srcSize, float inner, float outer);
+int litho_source_quadrupole(float* src, int srcSize, float radius, float offset);
+int litho_source_dipole(float* src, int srcSize, float radius, float offset, int onXAxis);
+int litho_source_point(float* src, int srcSize, float px, float py);
+/* getOuterSigma, litho_thread.cpp L427-439, clamped to 1 as main L1344. */
+float litho_outer_sigma(const float* src, int srcSize);
+/* Nx = floor(NA*period*(1+sigma)/lambda) in float, litho_thread.cpp L1346. */
+int litho_compute_nx(float NA, int period, float outerSigma, float lambda);
+/* Seeded synthetic Manhattan mask (no counterpart in the reference, which
+ * only generates LineSpace, common/mask.cpp L5-29): axis-aligned rectangles
+ * with sides U[wmin,wmax] pixels added until the 0/1 fill reaches `fill`.
+ * RNG: PCG32 (O'Neill), state seeded from `seed`. */
+int litho_manhattan_mask(float* mask, int N, unsigned long long seed, int wmin, int wmax,
+                         float fill);
+
+/* ---- Module 3: FT_r2c_padded, litho_thread.cpp L255-331 ------------------- */
+/* Full centred spectrum of an N x N real mask (N even, <= 4096). */
+int litho_mask_spectrum(const float* mask, int N, float* mskf_r, float* mskf_i);
+
+/* ---- Module 4: calcTCC, litho_thread.cpp L441-532 ------------------------- */
+/* TCC = A diag(S) A^H with the reference's float pupil arithmetic, built on
+ * the GPU (FP64 MFMA contraction).  tcc_* are n x n, n = (2Nx+1)(2Ny+1). */
+int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize,
+                    double* tcc_re, double* tcc_im);
+
+/* ---- Module 6: calcKernels, litho_thread.cpp L597-663 ---------------------- */
+/* Top-nk eigenpairs of a Hermitian TCC (upper triangle read, as zheevr 'U'),
+ * FP64 Jacobi on the GPU.  scales descending (FP64; the reference casts to
+ * f32 at L645), kernels flipped (nk x n f32 planes), degenerate clusters in
+ * the canonical mirror basis, every vector phase-fixed.  flags may be NULL. */
+int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx, int Ny,
+                       int nk, double* scales, float* krn_r, float* krn_i, int* flags);
+/* Source -> kernels without forming the TCC on the host: explicit TCC route
+ * when n <= nsrc, source-space Gram route (nsrc x nsrc eigenproblem, lifted)
+ * otherwise.  Same outputs as litho_kernels_topk(litho_tcc_build(...)). */
+int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSize, int nk,
+                              double* scales, float* krn_r, float* krn_i, int* flags);
+
+/* ---- Module 7: calcSOCS, litho_thread.cpp L813-1040 ------------------------ */
+/* HLS-v18-shaped entry (socs_2048_stream_refactored_v18_hls_fft.cpp L287-388):
+ * full Lx x Ly spectrum planes, kernels nk x (2ny+1)(2nx+1) concatenated,
+ * scales[nk]; writes the fftshifted G x G intensity tmpImgp (G =
+ * nextPow2(4n+1) per axis, litho_thread.cpp L833-834) to tmpImg and to
+ * output (either may be NULL).  Lx/Ly here are the spectrum sizes in pixels. */
+int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn_r,
+                      const float* krn_i, const float* scales, float* tmpImg, float* output,
+                      int nk, int nx, int ny, int Lx, int Ly);
+/* Mask -> aerial image: mask spectrum, SOCS loop and FI (Modules 3 + 7). */
+int litho_socs_image(const float* mask, int N, const float* krn_r, const float* krn_i,
+                     const float* scales, int nk, int Nx, int Ny, float* image);
+/* FI alone, litho_thread.cpp L692-772: G x G (zero frequency centred, as
+ * tmpImgp) -> N x N band-limited interpolation. */
+int litho_fi(const float* tmpImgp, int G, int N, int band, float* image);
+
+/* ---- device-resident batched path (the bench path) ------------------------- */
+/* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin). */
+int litho_bank_create(litho_ctx* ctx, const float* krn_r, const float* krn_i,
+                      const float* scales, int nk, int Nx, int Ny, litho_bank** bank);
+int litho_bank_destroy(litho_bank* bank);
+/* B masks (device, B x N x N f32) -> B images (device), asynchronous on the
+ * context stream.  Workspaces are grown by the context as needed. */
+int litho_socs_image_batch(litho_ctx* ctx, const litho_bank* bank, const float* d_mask, int B,
+                           int N, float* d_image);
+/* Timing on the context stream: records the stream time of the last
+ * litho_socs_image_batch call's kernels (ms) and per-stage durations. */
+int litho_last_timing(litho_ctx* ctx, float* total_ms, float* stage_ms, int nstage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,85 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+header declares, validates arguments before touching the device, and its host
+helpers (source sigma, Nx, synthetic masks) agree with the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "litho_socs.h")).read()
+    return sorted(set(re.findall(r"\b(litho_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(gpu):
+    L = gpu.lib()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), s
+    assert sorted(gpu.EXPORTS) == syms
+
+
+def test_status_strings(gpu):
+    L = gpu.lib()
+    assert L.litho_status_string(0) == b"ok"
+    for code in (-1, -2, -4, -5, -6):
+        assert L.litho_status_string(code) != b"unknown status"
+    assert L.litho_version() >= 100
+
+
+def test_arguments_rejected_before_device_work(gpu):
+    L = gpu.lib()
+    m = np.zeros(100 * 100, np.float32)
+    r = np.zeros_like(m)
+    assert L.litho_mask_spectrum(m, 100, r, r) == -1  # not a power of two
+    t = np.zeros(9 * 9)
+    k = np.zeros(9 * 9, np.float32)
+    sc = np.zeros(100)
+    assert L.litho_kernels_topk(t, t, 81, 4, 4, 100, sc, k, k, None) == -1  # nk > n
+    assert L.litho_kernels_topk(t, t, 80, 4, 4, 5, sc, k, k, None) == -1  # n != W^2
+    assert L.litho_fi(m, 48, 100, 3, m) == -1
+    assert L.litho_socs_image_batch(None, None, None, 1, 512, None) == -1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_context_reports_missing_gpu(gpu):
+    h = C.c_void_p()
+    assert gpu.lib().litho_ctx_create(C.byref(h), 0) == -5
+
+
+def test_host_helpers_match_oracle(gpu, orc):
+    for size, a, b in ((41, 0.6, 0.9), (101, 0.6, 0.9), (41, 0.3, 0.5)):
+        src = orc.source_annular(size, a, b)
+        assert gpu.getOuterSigma(src) == orc.outer_sigma(src)
+        s = orc.outer_sigma(src)
+        for NA, P in ((0.85, 2048), (0.85, 4096), (0.8, 512), (0.85, 16384)):
+            assert gpu.compute_nx(NA, P, s, 193.0) == orc.compute_nx(NA, P, s, 193.0)
+
+
+def test_source_generators_match_oracle(gpu, orc):
+    assert np.array_equal(gpu.createSource(41), orc.source_annular(41, 0.6, 0.9))
+    assert np.array_equal(gpu.createSource(101), orc.source_annular(101, 0.6, 0.9))
+    assert np.array_equal(gpu.createSource(41, "Dipole", radius=0.2, offset=0.5),
+                          orc.source_dipole(41, 0.2, 0.5, True))
+    assert np.array_equal(gpu.createSource(41, "CrossQuadrupole", radius=0.2, offset=0.5),
+                          orc.source_quadrupole(41, 0.2, 0.5))
+    assert np.array_equal(gpu.createSource(41, "Point", px=0.3, py=-0.2),
+                          orc.source_point(41, 0.3, -0.2))
+
+
+def test_manhattan_mask_is_seeded_and_filled(gpu):
+    a = gpu.manhattan_mask(512, 7)
+    b = gpu.manhattan_mask(512, 7)
+    c = gpu.manhattan_mask(512, 8)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+    assert set(np.unique(a)) == {0.0, 1.0}
+    assert 0.40 <= a.mean() < 0.45
+    with pytest.raises(gpu.LithoError):
+        gpu.manhattan_mask(64, 1, wmin=0)

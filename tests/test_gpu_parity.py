@@ -1,0 +1,229 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 oracle on the same
+seeded inputs, at sizes the oracle finishes in seconds, plus the reference's
+own fixtures and size-independent properties at the full 1024^2 shape.
+
+Tolerances: TCC and eigenvalues are FP64 on the GPU (1e-12 relative at zero
+defocus; 1e-7 with defocus, where device cosf/sinf may differ from glibc by
+an ulp); images are FP32 on the GPU and must be within 1e-5 of the FP64
+oracle in max|a-b|/max|b| (validation/golden/run_verification.py L203-204).
+"""
+import numpy as np
+import pytest
+
+from conftest import (BASE_LAMBDA, BASE_NA, FIX_DEFOCUS, FIX_NA, golden, max_rel)
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-5
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1(orc, base_source):
+    """512^2 at 4 nm, Nx = 17, K = 10: oracle kernels via the Gram route."""
+    src, s = base_source
+    N = 512
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    A, S = orc.pupil_matrix(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx)
+    sc, K, split = orc.calc_kernels_gram(A, S, 10, Nx, Nx)
+    assert not split
+    return dict(N=N, Nx=Nx, src=src, s=s, A=A, S=S, sc=sc, K=K)
+
+
+@pytest.fixture(scope="module")
+def fixture_tcc(orc, fix_source):
+    src, s = fix_source
+    return orc.calc_tcc(src, s, BASE_LAMBDA, FIX_NA, FIX_DEFOCUS, 512, 512, 4, 4)
+
+
+# ------------------------------------------------------------ Module 4 -------
+def test_tcc_build_fixture_config(gpu, orc, fix_source, fixture_tcc):
+    case("tcc 101-source Lx=512 Nx=4 defocus 0.2")
+    src, s = fix_source
+    T = gpu.calcTCC(src, s, BASE_LAMBDA, FIX_NA, FIX_DEFOCUS, 512, 512, 4, 4)
+    assert max_rel(T, fixture_tcc) < 1e-7
+    tr = np.fromfile(golden("tcc_expected_r.bin"), np.float32).reshape(81, 81)
+    ti = np.fromfile(golden("tcc_expected_i.bin"), np.float32).reshape(81, 81)
+    assert max_rel(T, tr + 1j * ti) < 1e-6
+    assert np.array_equal(T, T.conj().T)
+
+
+def test_tcc_build_baseline_c1(gpu, orc, c1):
+    case("tcc 41-source N=512 Nx=17 defocus 0")
+    T = gpu.calcTCC(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
+    To = orc.calc_tcc(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
+    assert max_rel(T, To) < 1e-12
+    assert not T.imag.any()
+
+
+# ------------------------------------------------------------ Module 6 -------
+def test_kernels_topk_fixture(gpu, orc, fixture_tcc):
+    case("kernels_topk n=81 nk=12")
+    sc, K, fl = gpu.calcKernels(fixture_tcc, 12, 4, 4)
+    so, Ko, split = orc.calc_kernels(fixture_tcc, 12, 4, 4)
+    assert np.abs(sc - so).max() / so[0] < 1e-10
+    assert np.abs(K - Ko).max() < 1e-6  # canonical basis + phase rule, f32 output
+    assert bool(fl & gpu.LITHO_FLAG_CLUSTER_SPLIT) == split
+
+
+def test_kernels_topk_split_cluster_is_flagged(gpu, fixture_tcc):
+    case("kernels_topk n=81 nk=10 (splits a pair)")
+    _, _, fl = gpu.calcKernels(fixture_tcc, 10, 4, 4)
+    assert fl & gpu.LITHO_FLAG_CLUSTER_SPLIT
+
+
+@pytest.mark.parametrize("L,fname", [(256, "eig_resolution_256x256.txt"),
+                                     (1024, "eig_resolution_1024x1024.txt")])
+def test_eigenvalues_match_reference_runs(gpu, orc, fix_source, L, fname):
+    case(f"kernels_from_source 101-source L={L}")
+    src, s = fix_source
+    Nx = orc.compute_nx(FIX_NA, L, s, BASE_LAMBDA)
+    ref = np.loadtxt(golden(fname))
+    k = min(16, ref.size, (2 * Nx + 1) ** 2)
+    sc, _, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, FIX_NA, FIX_DEFOCUS, L, L, Nx, Nx, k)
+    big = ref[:k] > 1e-6 * ref[0]
+    assert np.all(np.abs(sc[big] - ref[:k][big]) / ref[:k][big] < 6e-6)
+
+
+def test_kernels_gram_route_c1(gpu, orc, c1):
+    case("kernels_from_source 41-source N=512 nk=10 (Gram)")
+    sc, K, fl = gpu.kernels_from_source(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048,
+                                        c1["Nx"], c1["Nx"], 10)
+    assert fl & gpu.LITHO_FLAG_GRAM_ROUTE
+    assert not fl & gpu.LITHO_FLAG_CLUSTER_SPLIT
+    assert np.abs(sc - c1["sc"]).max() / c1["sc"][0] < 1e-10
+    assert np.abs(K - c1["K"]).max() < 1e-6
+
+
+# ------------------------------------------------------------ Module 3 / 7 ---
+def test_mask_spectrum(gpu, orc):
+    case("mask spectrum N=512")
+    m = gpu.manhattan_mask(512, 3)
+    assert max_rel(gpu.FT_r2c_padded(m), orc.mask_spectrum(m)) < IMG_TOL
+
+
+def test_socs_tmpimg_hls_entry(gpu, orc, c1):
+    case("socs tmpimg C1")
+    m = gpu.manhattan_mask(c1["N"], 4)
+    mf = orc.mask_spectrum(m)
+    t = gpu.calc_socs_hls(mf, c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+    to = orc.socs_tmpimg(mf, c1["K"], c1["sc"].astype(np.float32), c1["Nx"], c1["Nx"])
+    assert t.shape == (128, 128)
+    assert max_rel(t, to) < IMG_TOL
+
+
+def test_fi(gpu, orc, c1):
+    case("FI 128 -> 512")
+    m = gpu.manhattan_mask(c1["N"], 5)
+    to = orc.socs_tmpimg(orc.mask_spectrum(m), c1["K"], c1["sc"].astype(np.float32), c1["Nx"],
+                         c1["Nx"])
+    assert max_rel(gpu.FI(to, 512, 2 * c1["Nx"]), orc.fi(to, 512, 512)) < IMG_TOL
+
+
+def test_calc_socs_from_spectrum(gpu, orc, c1):
+    case("calcSOCS from spectrum C1")
+    m = gpu.manhattan_mask(c1["N"], 6)
+    mf = orc.mask_spectrum(m)
+    img = gpu.calcSOCS(mf, c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+    ref = orc.calc_socs(mf, c1["K"], c1["sc"].astype(np.float32), c1["Nx"], c1["Nx"])
+    assert max_rel(img, ref) < IMG_TOL
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_socs_image_c1(gpu, orc, c1, seed):
+    case(f"socs_image C1 seed {seed}")
+    m = gpu.manhattan_mask(c1["N"], seed)
+    img = gpu.socs_image(m, c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+    ref = orc.calc_socs(orc.mask_spectrum(m), c1["K"], c1["sc"].astype(np.float32), c1["Nx"],
+                        c1["Nx"])
+    assert max_rel(img, ref) < IMG_TOL
+
+
+def test_socs_image_reference_fixture_full_rank(gpu, orc, fixture_tcc):
+    case("socs_image fixture mask, K = n = 81")
+    m = np.fromfile(golden("mask_256x256.bin"), np.float32).reshape(256, 256)
+    M = np.zeros((512, 512), np.float32)
+    M[128:384, 128:384] = m
+    sc, K, _ = gpu.calcKernels(fixture_tcc, 81, 4, 4)
+    img = gpu.socs_image(M, K, sc, 4, 4)
+    gold = np.fromfile(golden("image_expected_512x512.bin"), np.float32).reshape(512, 512)
+    # SOCS with every kernel equals the TCC-direct image the fixture holds
+    assert max_rel(img, gold) < IMG_TOL
+
+
+def test_socs_image_c2_full_size(gpu, orc, base_source):
+    case("socs_image C2 N=1024 Nx=34 K=24")
+    src, s = base_source
+    N = 1024
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    assert Nx == 34
+    sc, K, fl = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, 24)
+    assert not fl & gpu.LITHO_FLAG_CLUSTER_SPLIT
+    m = gpu.manhattan_mask(N, 11)
+    img = gpu.socs_image(m, K, sc, Nx, Nx)
+    ref = orc.calc_socs(orc.mask_spectrum(m), K.astype(np.complex128), sc.astype(np.float32), Nx,
+                        Nx, nthreads=8)
+    assert max_rel(img, ref) < IMG_TOL
+    # open frame: intensity of a clear mask is sum(lambda_k |K_k(0)|^2), uniform
+    clear = gpu.socs_image(np.ones((N, N), np.float32), K, sc, Nx, Nx)
+    c = (2 * Nx + 1) ** 2 // 2
+    expect = float(np.sum(sc.astype(np.float32) * np.abs(K[:, c]) ** 2))
+    assert np.abs(clear - expect).max() < 1e-5 * expect
+
+
+def test_batch_equals_single(gpu, orc, c1):
+    case("batch of 3 vs single")
+    ctx = gpu.Context()
+    try:
+        N = c1["N"]
+        masks = np.stack([gpu.manhattan_mask(N, 20 + i) for i in range(3)])
+        bank = ctx.bank(c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+        dm = ctx.alloc(masks.nbytes)
+        di = ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        ctx.socs_image_batch(bank, dm, 3, N, di)
+        out = ctx.d2h(np.zeros_like(masks), di)
+        for i in range(3):
+            single = gpu.socs_image(masks[i], c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+            assert np.array_equal(out[i], single)
+        ctx.free(dm)
+        ctx.free(di)
+        gpu.lib().litho_bank_destroy(bank)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("N", [256, 2048])
+def test_other_clip_sizes(gpu, orc, base_source, N):
+    case(f"socs_image N={N}")
+    src, s = base_source
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, 16)
+    m = gpu.manhattan_mask(N, 31)
+    img = gpu.socs_image(m, K, sc, Nx, Nx)
+    ref = orc.calc_socs(orc.mask_spectrum(m), K.astype(np.complex128), sc.astype(np.float32), Nx,
+                        Nx, nthreads=8)
+    assert max_rel(img, ref) < IMG_TOL
+
+
+def test_single_kernel_and_empty_mask(gpu, orc, c1):
+    case("K=1 and empty mask")
+    m = gpu.manhattan_mask(c1["N"], 9)
+    img = gpu.socs_image(m, c1["K"][:1], c1["sc"][:1], c1["Nx"], c1["Nx"])
+    ref = orc.calc_socs(orc.mask_spectrum(m), c1["K"][:1], c1["sc"][:1].astype(np.float32),
+                        c1["Nx"], c1["Nx"])
+    assert max_rel(img, ref) < IMG_TOL
+    z = gpu.socs_image(np.zeros((c1["N"], c1["N"]), np.float32), c1["K"], c1["sc"], c1["Nx"],
+                       c1["Nx"])
+    assert not z.any()
+
+
+def test_invalid_requests_raise(gpu, c1):
+    case("invalid shapes")
+    with pytest.raises(gpu.LithoError):
+        gpu.socs_image(np.zeros((100, 100), np.float32), c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+    with pytest.raises(gpu.LithoError):
+        gpu.socs_image(np.zeros((64, 64), np.float32), c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
