@@ -16,6 +16,7 @@
 // Every kernel takes its geometry as arguments and is launched with a grid
 // that the host checks against the buffers it was given.
 #include "fft_lds.hpp"
+#include "fft_reg.hpp"
 #include "socs_kernels.hpp"
 
 namespace litho {
@@ -200,6 +201,212 @@ __global__ __launch_bounds__(512) void socs_accum_kernel(const float2* __restric
         }
 }
 
+
+// ---------------------------------------------------------------------------
+// socs_accum2: same contract as socs_accum, with the conv-grid FFTs done as
+// two-stage register FFTs (G = P*Q, T = min(P,Q) threads per transform, one
+// LDS transpose).  The window is placed at indices [0, W) on both axes (a
+// frequency shift is a phase ramp along the other axis, invisible in |.|^2),
+// so stage-1 inputs with n1 >= NZ = ceil(W/Q) are compile-time zeros.
+// LDS: W_G table | SLOTS transpose buffers | Y rows [GR][W].
+// ---------------------------------------------------------------------------
+template <int P, int Q, int GR, int NZ, int NTH>
+__global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restrict__ M,
+                                                          const float2* __restrict__ bank,
+                                                          const float* __restrict__ scales,
+                                                          int nk, int Nx, float* __restrict__ out,
+                                                          int shift,
+                                                          const float2* __restrict__ tw) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int G = Geo::G;
+    constexpr int T = Geo::T;
+    constexpr int SLOTS = NTH / T;
+    constexpr int RB = GR / SLOTS;
+    constexpr int PER = Geo::PER;
+    static_assert(GR % SLOTS == 0 && RB >= 1, "rows per block must fill the slots");
+    extern __shared__ float2 lds[];
+    float2* twG = lds;
+    float2* xbuf = lds + G;
+    float2* Y = xbuf + SLOTS * Geo::XBUF;
+    const int W = 2 * Nx + 1;
+    const int tid = threadIdx.x;
+    const int slot = tid / T, t = tid - slot * T;
+    const int nblk = G / GR;
+    const long b = blockIdx.x / nblk;
+    const int g0 = (blockIdx.x - (int)b * nblk) * GR;
+    const float2* Mb = M + b * (long)W * W;
+    float2* xb = xbuf + slot * Geo::XBUF;
+    for (int i = tid; i < G; i += NTH) twG[i] = tw[i * (TW_MAX / G)];
+    float acc[RB * PER];
+#pragma unroll
+    for (int i = 0; i < RB * PER; i++) acc[i] = 0.f;
+    __syncthreads();
+
+    for (int k = 0; k < nk; k++) {
+        const float2* Kk = bank + (long)k * W * W;
+        const float lam = scales[k];
+        for (int c0 = 0; c0 < W; c0 += SLOTS) {
+            const int col = c0 + slot;
+            const bool act = col < W;
+            fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+                float2 v = make_float2(0.f, 0.f);
+                if (act && n < W) {
+                    const int j = n * W + col;
+                    v = cmul(Kk[j], Mb[j]);
+                }
+                return v;
+            });
+            __syncthreads();
+            fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) {
+                const int r = g - g0;
+                if (act && r >= 0 && r < GR) Y[r * W + col] = val;
+            });
+            __syncthreads();
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const float2* yr = Y + (rb * SLOTS + slot) * W;
+            fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+                return n < W ? yr[n] : make_float2(0.f, 0.f);
+            });
+            __syncthreads();
+            fft2_stage2<Geo, true>(t, xb, [&](int sidx, int, float2 val) {
+                acc[rb * PER + sidx] += lam * (val.x * val.x + val.y * val.y);
+            });
+            __syncthreads();
+        }
+    }
+    float* ob = out + b * (long)G * G;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int row = g0 + rb * SLOTS + slot;
+#pragma unroll
+        for (int s = 0; s < Geo::S2; s++)
+#pragma unroll
+            for (int k2 = 0; k2 < Q; k2++) {
+                const int h = t + T * s + P * k2;
+                const float v = acc[rb * PER + s * Q + k2];
+                if (shift) ob[(long)((row + G / 2) % G) * G + (h + G / 2) % G] = v;
+                else ob[(long)row * G + h] = v;
+            }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Register-FFT row kernels for N = P*Q (two real rows per complex transform,
+// T = min(P,Q) threads per transform, NTH/T transforms per block).
+//
+// rows_fwd2: same contract as rows_dft (window [kmin, kmin+count) of the
+// forward DFT of each real row, (-1)^k when ALT).
+// ---------------------------------------------------------------------------
+template <int P, int Q, int NTH, bool ALT>
+__global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict__ in,
+                                                        float2* __restrict__ out, int nrows,
+                                                        int kmin, int count,
+                                                        const float2* __restrict__ tw) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int L = Geo::G;
+    constexpr int T = Geo::T;
+    constexpr int FPB = NTH / T;
+    __shared__ float2 twG[L];
+    __shared__ float2 xbuf[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    for (int i = tid; i < L; i += NTH) twG[i] = tw[i * (TW_MAX / L)];
+    __syncthreads();
+    float2* xb = xbuf + f * Geo::XBUF;
+    const float* r0 = in + (2 * (act ? p : 0)) * (long)L;
+    fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
+        return act ? make_float2(r0[n], r0[L + n]) : make_float2(0.f, 0.f);
+    });
+    __syncthreads();
+    float2 res[Geo::PER];
+    fft2_stage2<Geo, false>(t, xb, [&](int sidx, int, float2 v) { res[sidx] = v; });
+    __syncthreads();
+    // natural-order spectrum into the (now free) transpose buffer
+#pragma unroll
+    for (int s = 0; s < Geo::S2; s++)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; k2++) xb[t + T * s + P * k2] = res[s * Q + k2];
+    __syncthreads();
+    if (!act) return;
+    for (int j = t; j < count; j += T) {
+        const int k = kmin + j;
+        const int kp = ((k % L) + L) % L;
+        const int km = ((-k % L) + L) % L;
+        const float2 z1 = xb[kp];
+        const float2 z2 = cconj(xb[km]);
+        float2 x1 = cscale(cadd(z1, z2), 0.5f);
+        const float2 d = csub(z1, z2);
+        float2 x2 = make_float2(0.5f * d.y, -0.5f * d.x);
+        if (ALT && (k & 1)) {
+            x1 = cscale(x1, -1.f);
+            x2 = cscale(x2, -1.f);
+        }
+        out[(2 * p) * (long)count + j] = x1;
+        out[(2 * p + 1) * (long)count + j] = x2;
+    }
+}
+
+// image_rows2: same contract as image_rows.  The Hermitian row spectrum
+// (|kx| <= band) is placed at indices kx + band in [0, 2 band] so stage-1
+// inputs past ceil((2band+1)/Q) rows are compile-time zeros; the shift is
+// undone on the output by e^{-2 pi i band x / N}.
+template <int P, int Q, int NZ, int NTH>
+__global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restrict__ U,
+                                                          float* __restrict__ img, int nrows,
+                                                          int band,
+                                                          const float2* __restrict__ tw) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int L = Geo::G;
+    constexpr int T = Geo::T;
+    constexpr int FPB = NTH / T;
+    __shared__ float2 twG[L];
+    __shared__ float2 xbuf[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const int nb = band + 1;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    for (int i = tid; i < L; i += NTH) twG[i] = tw[i * (TW_MAX / L)];
+    __syncthreads();
+    float2* xb = xbuf + f * Geo::XBUF;
+    const float2* u1 = U + (2 * (act ? p : 0)) * (long)nb;
+    const float2* u2 = u1 + nb;
+    fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+        float2 z = make_float2(0.f, 0.f);
+        const int kx = n - band;
+        if (act && kx >= -band && kx <= band) {
+            const int a = kx < 0 ? -kx : kx;
+            float2 a1 = u1[a], a2 = u2[a];
+            if (a == 0) {
+                z = make_float2(a1.x, a2.x);
+            } else if (kx > 0) {
+                z = make_float2(a1.x - a2.y, a1.y + a2.x);  // u1 + i u2
+            } else {
+                z = make_float2(a1.x + a2.y, a2.x - a1.y);  // conj(u1) + i conj(u2)
+            }
+        }
+        return z;
+    });
+    __syncthreads();
+    if (!act) return;
+    float* o1 = img + (2 * p) * (long)L;
+    float* o2 = o1 + L;
+    fft2_stage2<Geo, true>(t, xb, [&](int, int x, float2 v) {
+        // undo the index shift: multiply by e^{-2 pi i band x / L}
+        const float2 w = twG[(int)(((long)band * x) & (L - 1))];
+        const float2 r = cmul(v, w);
+        o1[x] = r.x;
+        o2[x] = r.y;
+    });
+}
+
 // ---------------------------------------------------------------------------
 // fi_cols: P [B][G][nb] float2 (row DFT of the unshifted conv-grid intensity,
 // kx = 0..band) -> U [B][N][nb] float2.  For column kx:
@@ -321,6 +528,22 @@ static int launch_rows_dft_L(const float* in, float2* out, long nrows, int kmin,
 int launch_rows_dft(int L, const float* in, float2* out, long nrows, int kmin, int count,
                     bool alt, const float2* tw, hipStream_t st) {
     if (nrows % 2 || count <= 0 || nrows > 0x7fffffffL) return LITHO_EARG;
+#define LITHO_RF2(p, q, nth)                                                                  \
+    if (L == p * q) {                                                                         \
+        constexpr int FPB = nth / Fft2Geom<p, q>::T;                                          \
+        unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                       \
+        if (alt)                                                                              \
+            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true>), dim3(grid), dim3(nth), 0, \
+                               st, in, out, (int)nrows, kmin, count, tw);                     \
+        else                                                                                  \
+            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, false>), dim3(grid), dim3(nth), 0,\
+                               st, in, out, (int)nrows, kmin, count, tw);                     \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
+    }
+    LITHO_RF2(16, 16, 256)
+    LITHO_RF2(16, 32, 256)
+    LITHO_RF2(32, 32, 256)
+#undef LITHO_RF2
     switch (L) {
         case 32: return launch_rows_dft_L<32>(in, out, nrows, kmin, count, alt, tw, st);
         case 64: return launch_rows_dft_L<64>(in, out, nrows, kmin, count, alt, tw, st);
@@ -399,9 +622,52 @@ int socs_rows_per_block(int G, int Nx) {
     return 0;
 }
 
+template <int P, int Q, int GR, int NZ, int NTH>
+static int launch_socs2(const float2* M, const float2* bank, const float* scales, int nk, int Nx,
+                        int B, float* out, int shift, const float2* tw, hipStream_t st) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int G = P * Q;
+    constexpr int SLOTS = NTH / Geo::T;
+    const int W = 2 * Nx + 1;
+    size_t lds = (size_t)(G + SLOTS * Geo::XBUF + GR * W) * sizeof(float2);
+    if (lds > 160 * 1024) return LITHO_EARG;
+    auto kern = socs_accum2_kernel<P, Q, GR, NZ, NTH>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024) != hipSuccess)
+            return LITHO_EHIP;
+        attr_set = true;
+    }
+    unsigned grid = (unsigned)((size_t)B * (G / GR));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NTH), lds, st, M, bank, scales, nk, Nx, out, shift,
+                       tw);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 int launch_socs_accum(int G, const float2* M, const float2* bank, const float* scales, int nk,
                       int Nx, int B, float* out, int shift, const float2* tw, hipStream_t st) {
     if (B <= 0 || nk < 0 || 4 * Nx + 1 > G) return LITHO_EARG;
+    const int W = 2 * Nx + 1;
+    // register-FFT variant: (P, Q, rows per block, nonzero stage-1 rows, threads)
+#define LITHO_S2(p, q, gr, nz, nth)                                                        \
+    if (G == p * q && W <= nz * q)                                                         \
+        return launch_socs2<p, q, gr, nz, nth>(M, bank, scales, nk, Nx, B, out, shift, tw, st);
+    LITHO_S2(4, 4, 16, 4, 64)
+    LITHO_S2(4, 8, 32, 4, 128)
+    LITHO_S2(8, 8, 64, 3, 512)
+    LITHO_S2(8, 8, 64, 8, 512)
+    LITHO_S2(8, 16, 128, 3, 512)
+    LITHO_S2(8, 16, 128, 4, 512)
+    LITHO_S2(8, 16, 128, 8, 512)
+    LITHO_S2(16, 16, 128, 5, 512)
+    LITHO_S2(16, 16, 128, 8, 512)
+    LITHO_S2(16, 16, 64, 16, 512)
+    LITHO_S2(16, 32, 32, 5, 256)
+    LITHO_S2(16, 32, 32, 16, 256)
+    LITHO_S2(32, 32, 16, 9, 256)
+    LITHO_S2(32, 32, 16, 32, 256)
+#undef LITHO_S2
     int GR = socs_rows_per_block(G, Nx);
     if (!GR) return LITHO_EARG;
 #define LITHO_SOCS_CASE(g, gr) \
@@ -475,6 +741,23 @@ static int launch_image_rows_N(const float2* U, float* img, long nrows, int band
 int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, const float2* tw,
                       hipStream_t st) {
     if (nrows % 2 || 2 * band + 1 > N || nrows > 0x7fffffffL) return LITHO_EARG;
+#define LITHO_IR2(p, q, nz, nth)                                                          \
+    if (N == p * q && 2 * band + 1 <= nz * q) {                                           \
+        constexpr int FPB = nth / Fft2Geom<p, q>::T;                                      \
+        unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                   \
+        hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth>), dim3(grid), dim3(nth), 0, \
+                           st, U, img, (int)nrows, band, tw);                             \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                   \
+    }
+    LITHO_IR2(16, 16, 5, 256)
+    LITHO_IR2(16, 16, 16, 256)
+    LITHO_IR2(16, 32, 3, 256)
+    LITHO_IR2(16, 32, 5, 256)
+    LITHO_IR2(16, 32, 16, 256)
+    LITHO_IR2(32, 32, 3, 256)
+    LITHO_IR2(32, 32, 9, 256)
+    LITHO_IR2(32, 32, 32, 256)
+#undef LITHO_IR2
     switch (N) {
         case 32: return launch_image_rows_N<32>(U, img, nrows, band, tw, st);
         case 64: return launch_image_rows_N<64>(U, img, nrows, band, tw, st);
