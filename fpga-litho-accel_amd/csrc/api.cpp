@@ -49,6 +49,9 @@ struct litho_ctx {
     DevBuf ws[8];
     hipEvent_t ev[NSTAGE + 1] = {};
     bool timed = false;
+    int nstage = 0;
+    const char* stage_names[NSTAGE] = {};
+    int path = LITHO_PATH_AUTO;
 };
 
 struct litho_bank {
@@ -685,21 +688,57 @@ int litho_bank_destroy(litho_bank* b) {
     return LITHO_OK;
 }
 
+int litho_ctx_set_path(litho_ctx* c, int path) {
+    if (!c || (path != LITHO_PATH_AUTO && path != LITHO_PATH_POW2)) return LITHO_EARG;
+    c->path = path;
+    return LITHO_OK;
+}
+
 int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
                            int N, float* d_image) {
     if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
     const int Nx = bank->Nx;
     const int W = 2 * Nx + 1;
-    const int G = next_pow2(4 * Nx + 1);
     const int band = 2 * Nx;
-    if (!is_pow2(N) || N < 32 || N > 4096 || G > N || G > 1024 || W > N) return LITHO_EARG;
+    if (!is_pow2(N) || N < 32 || N > 4096 || W > N) return LITHO_EARG;
     hipStream_t st = c->stream;
+    const int Gb = band_grid(Nx);
+    if (c->path == LITHO_PATH_AUTO && Gb && Gb <= N && N >= 256 && N <= 1024) {
+        // band pipeline: 5 launches
+        void *R, *M, *C, *U;
+        LITHO_TRY(ctx_ws(c, 0, (size_t)B * W * N * sizeof(float2), &R));
+        LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
+        LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
+        LITHO_TRY(ctx_ws(c, 4, (size_t)B * (band + 1) * N * sizeof(float2), &U));
+        static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
+                                      "image_rows"};
+        LITHO_HIP(hipEventRecord(c->ev[0], st));
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, (float2*)R, (long)B * N, -Nx, W, st));
+        LITHO_HIP(hipEventRecord(c->ev[1], st));
+        LITHO_TRY(launch_mask_cols_t(N, (float2*)R, (float2*)M, B, W, -Nx, W, st));
+        LITHO_HIP(hipEventRecord(c->ev[2], st));
+        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, bank->d_scales, bank->nk, Nx, B,
+                                   (float2*)C, st));
+        LITHO_HIP(hipEventRecord(c->ev[3], st));
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, (float2*)U, B, band, st));
+        LITHO_HIP(hipEventRecord(c->ev[4], st));
+        LITHO_TRY(launch_image_rows_t(N, (float2*)U, d_image, (long)B * N, band, st));
+        LITHO_HIP(hipEventRecord(c->ev[5], st));
+        c->nstage = 5;
+        for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
+        c->timed = true;
+        return LITHO_OK;
+    }
+    const int G = next_pow2(4 * Nx + 1);
+    if (G > N || G > 1024) return LITHO_EARG;
     void *R, *M, *T, *P, *U;
     LITHO_TRY(ctx_ws(c, 0, (size_t)B * N * W * sizeof(float2), &R));
     LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
     LITHO_TRY(ctx_ws(c, 2, (size_t)B * G * G * sizeof(float), &T));
     LITHO_TRY(ctx_ws(c, 3, (size_t)B * G * (band + 1) * sizeof(float2), &P));
     LITHO_TRY(ctx_ws(c, 4, (size_t)B * N * (band + 1) * sizeof(float2), &U));
+    static const char* names[] = {"mask_rows", "mask_cols", "socs_accum", "fi_rows", "fi_cols",
+                                  "image_rows"};
     LITHO_HIP(hipEventRecord(c->ev[0], st));
     LITHO_TRY(launch_rows_dft(N, d_mask, (float2*)R, (long)B * N, -Nx, W, true, c->tw, st));
     LITHO_HIP(hipEventRecord(c->ev[1], st));
@@ -714,17 +753,24 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     LITHO_HIP(hipEventRecord(c->ev[5], st));
     LITHO_TRY(launch_image_rows(N, (float2*)U, d_image, (long)B * N, band, c->tw, st));
     LITHO_HIP(hipEventRecord(c->ev[6], st));
+    c->nstage = 6;
+    for (int i = 0; i < 6; i++) c->stage_names[i] = names[i];
     c->timed = true;
     return LITHO_OK;
 }
 
 int litho_last_timing(litho_ctx* c, float* total_ms, float* stage_ms, int nstage) {
     if (!c || !c->timed) return LITHO_EARG;
-    LITHO_HIP(hipEventSynchronize(c->ev[NSTAGE]));
-    if (total_ms) LITHO_HIP(hipEventElapsedTime(total_ms, c->ev[0], c->ev[NSTAGE]));
-    for (int i = 0; i < nstage && i < NSTAGE; i++)
+    LITHO_HIP(hipEventSynchronize(c->ev[c->nstage]));
+    if (total_ms) LITHO_HIP(hipEventElapsedTime(total_ms, c->ev[0], c->ev[c->nstage]));
+    for (int i = 0; i < nstage && i < c->nstage; i++)
         LITHO_HIP(hipEventElapsedTime(&stage_ms[i], c->ev[i], c->ev[i + 1]));
-    return LITHO_OK;
+    return c->nstage;
+}
+
+const char* litho_stage_name(litho_ctx* c, int i) {
+    if (!c || i < 0 || i >= c->nstage || !c->stage_names[i]) return "";
+    return c->stage_names[i];
 }
 
 int litho_socs_image(const float* mask, int N, const float* krn_r, const float* krn_i,

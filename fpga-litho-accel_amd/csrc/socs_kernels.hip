@@ -236,7 +236,7 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     const int g0 = (blockIdx.x - (int)b * nblk) * GR;
     const float2* Mb = M + b * (long)W * W;
     float2* xb = xbuf + slot * Geo::XBUF;
-    for (int i = tid; i < G; i += NTH) twG[i] = tw[i * (TW_MAX / G)];
+    fill_twiddles<G>(twG, tid, NTH);
     float acc[RB * PER];
 #pragma unroll
     for (int i = 0; i < RB * PER; i++) acc[i] = 0.f;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
 // rows_fwd2: same contract as rows_dft (window [kmin, kmin+count) of the
 // forward DFT of each real row, (-1)^k when ALT).
 // ---------------------------------------------------------------------------
-template <int P, int Q, int NTH, bool ALT>
+template <int P, int Q, int NTH, bool ALT, bool OUT_T>
 __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict__ in,
                                                         float2* __restrict__ out, int nrows,
                                                         int kmin, int count,
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
     const long npairs = nrows / 2;
     const long p = (long)blockIdx.x * FPB + f;
     const bool act = p < npairs;
-    for (int i = tid; i < L; i += NTH) twG[i] = tw[i * (TW_MAX / L)];
+    fill_twiddles<L>(twG, tid, NTH);
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
     const float* r0 = in + (2 * (act ? p : 0)) * (long)L;
@@ -347,8 +347,16 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
             x1 = cscale(x1, -1.f);
             x2 = cscale(x2, -1.f);
         }
-        out[(2 * p) * (long)count + j] = x1;
-        out[(2 * p + 1) * (long)count + j] = x2;
+        if (OUT_T) {
+            // [img][j][row in img]: L rows per image (square clips)
+            const long r = 2 * p, img = r / L, y = r - img * L;
+            float2* o = out + (img * count + j) * (long)L + y;
+            o[0] = x1;
+            o[1] = x2;
+        } else {
+            out[(2 * p) * (long)count + j] = x1;
+            out[(2 * p + 1) * (long)count + j] = x2;
+        }
     }
 }
 
@@ -356,7 +364,7 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
 // (|kx| <= band) is placed at indices kx + band in [0, 2 band] so stage-1
 // inputs past ceil((2band+1)/Q) rows are compile-time zeros; the shift is
 // undone on the output by e^{-2 pi i band x / N}.
-template <int P, int Q, int NZ, int NTH>
+template <int P, int Q, int NZ, int NTH, bool IN_T>
 __global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restrict__ U,
                                                           float* __restrict__ img, int nrows,
                                                           int band,
@@ -373,17 +381,21 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restri
     const long npairs = nrows / 2;
     const long p = (long)blockIdx.x * FPB + f;
     const bool act = p < npairs;
-    for (int i = tid; i < L; i += NTH) twG[i] = tw[i * (TW_MAX / L)];
+    fill_twiddles<L>(twG, tid, NTH);
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
-    const float2* u1 = U + (2 * (act ? p : 0)) * (long)nb;
-    const float2* u2 = u1 + nb;
+    // IN_T: U is [img][kx][row in img] (L rows per image), else [row][kx]
+    const long r0 = 2 * (act ? p : 0);
+    const long img0 = r0 / L, y0 = r0 - img0 * L;
+    const float2* u1 = IN_T ? U + img0 * nb * (long)L + y0 : U + r0 * (long)nb;
+    const long ust = IN_T ? L : 1;       // stride between kx
+    const long u2o = IN_T ? 1 : nb;      // offset of the second row
     fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
         float2 z = make_float2(0.f, 0.f);
         const int kx = n - band;
         if (act && kx >= -band && kx <= band) {
             const int a = kx < 0 ? -kx : kx;
-            float2 a1 = u1[a], a2 = u2[a];
+            float2 a1 = u1[a * ust], a2 = u1[a * ust + u2o];
             if (a == 0) {
                 z = make_float2(a1.x, a2.x);
             } else if (kx > 0) {
@@ -404,6 +416,186 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restri
         const float2 r = cmul(v, w);
         o1[x] = r.x;
         o2[x] = r.y;
+    });
+}
+
+
+// ---------------------------------------------------------------------------
+// Band pipeline (image path): square conv grid G = P*P of any size with
+// factors 2 and 3 (e.g. 36, 81, 144), whole image per block.
+//
+// socs_band: SOCS accumulate as socs_accum2, then the forward R2C of the
+// accumulated intensity done in place: the row-pass outputs sit in exactly
+// the stage-1 input layout of a forward row FFT, so rows are transformed from
+// registers, kept for kx <= band, staged in LDS and transformed down the
+// columns; Cout[b][ky + band][kx] = (1/G^2) DFT(intensity)[ky][kx] for
+// |ky| <= band, 0 <= kx <= band (band = 2 Nx).  This is FI's R2C (L724-735)
+// restricted to the band the intensity occupies.
+// ---------------------------------------------------------------------------
+template <int P, int NZ, int SLOTS>
+__global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __restrict__ M,
+                                                              const float2* __restrict__ bank,
+                                                              const float* __restrict__ scales,
+                                                              int nk, int Nx,
+                                                              float2* __restrict__ Cout) {
+    using Geo = Fft2Geom<P, P>;
+    constexpr int G = Geo::G;
+    constexpr int NTH = P * SLOTS;
+    constexpr int RB = G / SLOTS;
+    static_assert(G % SLOTS == 0, "slots must divide the grid");
+    extern __shared__ float2 lds[];
+    float2* twG = lds;
+    float2* xbuf = lds + G;
+    float2* Y = xbuf + SLOTS * Geo::XBUF;  // [G][W]
+    const int W = 2 * Nx + 1;
+    const int band = 2 * Nx;
+    const int tid = threadIdx.x;
+    const int slot = tid / P, t = tid - slot * P;
+    const long b = blockIdx.x;
+    const float2* Mb = M + b * (long)W * W;
+    float2* xb = xbuf + slot * Geo::XBUF;
+    fill_twiddles<G>(twG, tid, NTH);
+    float acc[RB * P];
+#pragma unroll
+    for (int i = 0; i < RB * P; i++) acc[i] = 0.f;
+    __syncthreads();
+
+    for (int k = 0; k < nk; k++) {
+        const float2* Kk = bank + (long)k * W * W;
+        const float lam = scales[k];
+        for (int c0 = 0; c0 < W; c0 += SLOTS) {
+            const int col = c0 + slot;
+            const bool act = col < W;
+            fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+                float2 v = make_float2(0.f, 0.f);
+                if (act && n < W) {
+                    const int j = n * W + col;
+                    v = cmul(Kk[j], Mb[j]);
+                }
+                return v;
+            });
+            __syncthreads();
+            fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) {
+                if (act) Y[g * W + col] = val;
+            });
+            __syncthreads();
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const float2* yr = Y + (rb * SLOTS + slot) * W;
+            fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+                return n < W ? yr[n] : make_float2(0.f, 0.f);
+            });
+            __syncthreads();
+            fft2_stage2<Geo, true>(t, xb, [&](int sidx, int, float2 val) {
+                acc[rb * P + sidx] += lam * (val.x * val.x + val.y * val.y);
+            });
+            __syncthreads();
+        }
+    }
+    // forward R2C of the intensity rows straight from registers
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int r = rb * SLOTS + slot;
+        fft2_stage1_ij<Geo, P, false>(t, xb, twG, [&](int n1, int) {
+            return make_float2(acc[rb * P + n1], 0.f);
+        });
+        __syncthreads();
+        fft2_stage2<Geo, false>(t, xb, [&](int, int kx, float2 val) {
+            if (kx <= band) Y[r * W + kx] = val;
+        });
+        __syncthreads();
+    }
+    // columns kx = 0..band: forward FFT over the rows, keep |ky| <= band
+    const float inv = 1.0f / ((float)G * (float)G);
+    float2* Cb = Cout + b * (long)(2 * band + 1) * W;
+    for (int c0 = 0; c0 < W; c0 += SLOTS) {
+        const int col = c0 + slot;
+        const bool act = col < W;
+        fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
+            return act ? Y[n * W + col] : make_float2(0.f, 0.f);
+        });
+        __syncthreads();
+        fft2_stage2<Geo, false>(t, xb, [&](int, int kk, float2 val) {
+            const int ky = kk <= G / 2 ? kk : kk - G;
+            if (act && ky >= -band && ky <= band) Cb[(ky + band) * W + col] = cscale(val, inv);
+        });
+        __syncthreads();
+    }
+}
+
+// mask_cols2: R [B][cx][N] (row-DFT window, transposed) -> M [B][cy][cx],
+// M[b][j][c] = (1/N^2) (-1)^ky DFT_N(R[b][c][:])[ky mod N], ky = kymin + j.
+template <int P, int Q, int NTH>
+__global__ __launch_bounds__(NTH) void mask_cols2_kernel(const float2* __restrict__ R,
+                                                         float2* __restrict__ M, int cx,
+                                                         int kymin, int cy) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int N = Geo::G;
+    constexpr int T = Geo::T;
+    constexpr int FPB = NTH / T;
+    __shared__ float2 twG[N];
+    __shared__ float2 xbuf[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const long b = blockIdx.y;
+    const int col = blockIdx.x * FPB + f;
+    const bool act = col < cx;
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
+    float2* xb = xbuf + f * Geo::XBUF;
+    const float2* Rc = R + (b * cx + (act ? col : 0)) * (long)N;
+    fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
+        return act ? Rc[n] : make_float2(0.f, 0.f);
+    });
+    __syncthreads();
+    if (!act) return;
+    const float inv = 1.0f / ((float)N * (float)N);
+    float2* Mb = M + b * (long)cy * cx;
+    fft2_stage2<Geo, false>(t, xb, [&](int, int k, float2 val) {
+        const int ky = k < N / 2 ? k : k - N;
+        const int j = ky - kymin;
+        if (j >= 0 && j < cy) Mb[(long)j * cx + col] = cscale(val, (ky & 1) ? -inv : inv);
+    });
+}
+
+// fi_cols2: C [B][2band+1][nb] (band spectrum, kx = 0..band) -> U [B][nb][N]:
+// U[b][kx][y] = (-1)^kx sum_ky (-1)^ky C[ky][kx] e^{+2 pi i ky y / N}, the
+// band placed at indices ky + band and the shift undone on the output.
+template <int P, int Q, int NZ, int NTH>
+__global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict__ C,
+                                                       float2* __restrict__ U, int band) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int N = Geo::G;
+    constexpr int T = Geo::T;
+    constexpr int FPB = NTH / T;
+    __shared__ float2 twG[N];
+    __shared__ float2 xbuf[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const int nb = band + 1;
+    const long b = blockIdx.y;
+    const int col = blockIdx.x * FPB + f;
+    const bool act = col < nb;
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
+    float2* xb = xbuf + f * Geo::XBUF;
+    const float2* Cb = C + b * (long)(2 * band + 1) * nb;
+    fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+        float2 v = make_float2(0.f, 0.f);
+        if (act && n <= 2 * band) {
+            v = Cb[(long)n * nb + col];
+            if ((n - band) & 1) v = cscale(v, -1.f);
+        }
+        return v;
+    });
+    __syncthreads();
+    if (!act) return;
+    float2* Uc = U + (b * nb + col) * (long)N;
+    const float sx = (col & 1) ? -1.f : 1.f;
+    fft2_stage2<Geo, true>(t, xb, [&](int, int y, float2 val) {
+        const float2 w = twG[(int)(((long)band * y) % N)];  // e^{-2 pi i band y / N}
+        Uc[y] = cscale(cmul(val, w), sx);
     });
 }
 
@@ -533,10 +725,10 @@ int launch_rows_dft(int L, const float* in, float2* out, long nrows, int kmin, i
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                          \
         unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                       \
         if (alt)                                                                              \
-            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true>), dim3(grid), dim3(nth), 0, \
+            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, false>), dim3(grid), dim3(nth), 0, \
                                st, in, out, (int)nrows, kmin, count, tw);                     \
         else                                                                                  \
-            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, false>), dim3(grid), dim3(nth), 0,\
+            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, false, false>), dim3(grid), dim3(nth), 0,\
                                st, in, out, (int)nrows, kmin, count, tw);                     \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
     }
@@ -745,7 +937,7 @@ int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, 
     if (N == p * q && 2 * band + 1 <= nz * q) {                                           \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                      \
         unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                   \
-        hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth>), dim3(grid), dim3(nth), 0, \
+        hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, false>), dim3(grid), dim3(nth), 0, \
                            st, U, img, (int)nrows, band, tw);                             \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                   \
     }
@@ -769,6 +961,127 @@ int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, 
         case 4096: return launch_image_rows_N<4096>(U, img, nrows, band, tw, st);
         default: return LITHO_EARG;
     }
+}
+
+
+// ===========================================================================
+// band pipeline launchers
+// ===========================================================================
+int band_grid(int Nx) {
+    const int need = 4 * Nx + 1;
+    const int cands[] = {36, 81, 144};
+    for (int g : cands)
+        if (g >= need) return g;
+    return 0;
+}
+
+template <int P, int NZ, int SLOTS>
+static int launch_band(const float2* M, const float2* bank, const float* scales, int nk, int Nx,
+                       int B, float2* C, hipStream_t st) {
+    using Geo = Fft2Geom<P, P>;
+    const int W = 2 * Nx + 1;
+    size_t lds = (size_t)(Geo::G + SLOTS * Geo::XBUF + Geo::G * W) * sizeof(float2);
+    if (lds > 160 * 1024) return LITHO_EARG;
+    auto kern = socs_band_kernel<P, NZ, SLOTS>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024) != hipSuccess)
+            return LITHO_EHIP;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(B), dim3(P * SLOTS), lds, st, M, bank, scales, nk, Nx, C);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_socs_band(int G, const float2* M, const float2* bank, const float* scales, int nk,
+                     int Nx, int B, float2* C, hipStream_t st) {
+    const int W = 2 * Nx + 1;
+    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G) return LITHO_EARG;
+#define LITHO_BAND(p, nz, slots) \
+    if (G == p * p && W <= nz * p) return launch_band<p, nz, slots>(M, bank, scales, nk, Nx, B, C, st);
+    LITHO_BAND(6, 6, 36)
+    LITHO_BAND(9, 4, 27)
+    LITHO_BAND(9, 9, 27)
+    LITHO_BAND(12, 6, 48)
+    LITHO_BAND(12, 12, 48)
+#undef LITHO_BAND
+    return LITHO_EARG;
+}
+
+int launch_rows_dft_t(int L, const float* in, float2* out, long nrows, int kmin, int count,
+                      hipStream_t st) {
+    if (nrows % L || nrows % 2 || count <= 0) return LITHO_EARG;
+#define LITHO_RT(p, q, nth)                                                                   \
+    if (L == p * q) {                                                                         \
+        constexpr int FPB = nth / Fft2Geom<p, q>::T;                                          \
+        unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                       \
+        hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true>), dim3(grid), dim3(nth),  \
+                           0, st, in, out, (int)nrows, kmin, count, nullptr);                 \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
+    }
+    LITHO_RT(16, 16, 256)
+    LITHO_RT(16, 32, 256)
+    LITHO_RT(32, 32, 256)
+#undef LITHO_RT
+    return LITHO_EARG;
+}
+
+int launch_mask_cols_t(int N, const float2* R, float2* M, int B, int cx, int kymin, int cy,
+                       hipStream_t st) {
+    if (B <= 0 || B > 65535) return LITHO_EARG;
+#define LITHO_MC(p, q, nth)                                                                    \
+    if (N == p * q) {                                                                          \
+        constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
+        hipLaunchKernelGGL((mask_cols2_kernel<p, q, nth>), dim3(cdiv(cx, FPB), B), dim3(nth),  \
+                           0, st, R, M, cx, kymin, cy);                                        \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
+    }
+    LITHO_MC(16, 16, 256)
+    LITHO_MC(16, 32, 256)
+    LITHO_MC(32, 32, 256)
+#undef LITHO_MC
+    return LITHO_EARG;
+}
+
+int launch_fi_cols_t(int N, const float2* C, float2* U, int B, int band, hipStream_t st) {
+    if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
+#define LITHO_FC(p, q, nz, nth)                                                                 \
+    if (N == p * q && 2 * band + 1 <= nz * q) {                                                 \
+        constexpr int FPB = nth / Fft2Geom<p, q>::T;                                            \
+        hipLaunchKernelGGL((fi_cols2_kernel<p, q, nz, nth>), dim3(cdiv(band + 1, FPB), B),      \
+                           dim3(nth), 0, st, C, U, band);                                       \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                         \
+    }
+    LITHO_FC(16, 16, 3, 256)
+    LITHO_FC(16, 16, 16, 256)
+    LITHO_FC(16, 32, 3, 256)
+    LITHO_FC(16, 32, 16, 256)
+    LITHO_FC(32, 32, 5, 256)
+    LITHO_FC(32, 32, 32, 256)
+#undef LITHO_FC
+    return LITHO_EARG;
+}
+
+int launch_image_rows_t(int N, const float2* U, float* img, long nrows, int band,
+                        hipStream_t st) {
+    if (nrows % N || nrows % 2 || 2 * band + 1 > N) return LITHO_EARG;
+#define LITHO_IT(p, q, nz, nth)                                                                \
+    if (N == p * q && 2 * band + 1 <= nz * q) {                                                \
+        constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
+        unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                        \
+        hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true>), dim3(grid), dim3(nth),   \
+                           0, st, U, img, (int)nrows, band, nullptr);                          \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
+    }
+    LITHO_IT(16, 16, 3, 256)
+    LITHO_IT(16, 16, 16, 256)
+    LITHO_IT(16, 32, 3, 256)
+    LITHO_IT(16, 32, 16, 256)
+    LITHO_IT(32, 32, 5, 256)
+    LITHO_IT(32, 32, 32, 256)
+#undef LITHO_IT
+    return LITHO_EARG;
 }
 
 }  // namespace litho

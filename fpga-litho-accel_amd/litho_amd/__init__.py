@@ -31,8 +31,11 @@ EXPORTS = [
     "litho_source_dipole", "litho_source_point", "litho_outer_sigma", "litho_compute_nx",
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
-    "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_last_timing",
+    "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_ctx_set_path",
+    "litho_last_timing", "litho_stage_name",
 ]
+LITHO_PATH_AUTO = 0
+LITHO_PATH_POW2 = 1
 
 
 class LithoError(RuntimeError):
@@ -93,9 +96,12 @@ def lib():
         L.litho_bank_destroy.argtypes = [vp]
         L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
         L.litho_last_timing.argtypes = [vp, C.POINTER(f), C.POINTER(f), i]
+        L.litho_ctx_set_path.argtypes = [vp, i]
+        L.litho_stage_name.argtypes = [vp, i]
+        L.litho_stage_name.restype = C.c_char_p
         for name in EXPORTS:
             fn = getattr(L, name)
-            if name not in ("litho_status_string", "litho_outer_sigma"):
+            if name not in ("litho_status_string", "litho_outer_sigma", "litho_stage_name"):
                 fn.restype = C.c_int
         _LIB = L
     return _LIB
@@ -292,11 +298,18 @@ class Context:
         _check(lib().litho_socs_image_batch(self.h, bank, d_mask, B, N, d_image),
                "litho_socs_image_batch")
 
-    def last_timing(self, nstage=6):
+    def set_path(self, path):
+        _check(lib().litho_ctx_set_path(self.h, path), "litho_ctx_set_path")
+
+    def last_timing(self, nstage=8):
+        """(total ms, {stage name: ms}) of the last batch call."""
         tot = C.c_float()
         st = (C.c_float * nstage)()
-        _check(lib().litho_last_timing(self.h, C.byref(tot), st, nstage), "litho_last_timing")
-        return tot.value, list(st)
+        n = lib().litho_last_timing(self.h, C.byref(tot), st, nstage)
+        if n < 0:
+            _check(n, "litho_last_timing")
+        names = [lib().litho_stage_name(self.h, i).decode() for i in range(n)]
+        return tot.value, dict(zip(names, list(st)[:n]))
 
     def close(self):
         if self.h:

@@ -136,9 +136,19 @@ int litho_bank_destroy(litho_bank* bank);
  * context stream.  Workspaces are grown by the context as needed. */
 int litho_socs_image_batch(litho_ctx* ctx, const litho_bank* bank, const float* d_mask, int B,
                            int N, float* d_image);
-/* Timing on the context stream: records the stream time of the last
- * litho_socs_image_batch call's kernels (ms) and per-stage durations. */
+/* Conv-grid choice of the image path.  AUTO: smallest square grid of
+ * 2^a 3^b points (36, 81, 144) >= 4Nx+1 with the intensity's band spectrum
+ * taken inside the SOCS kernel, else the reference's nextPow2(4Nx+1) grid
+ * (litho_thread.cpp L833).  FI makes the image independent of the grid. */
+#define LITHO_PATH_AUTO 0
+#define LITHO_PATH_POW2 1
+int litho_ctx_set_path(litho_ctx* ctx, int path);
+/* Timing on the context stream of the last litho_socs_image_batch call:
+ * total and per-stage device time (ms).  Returns the number of stages (>= 0)
+ * or a negative status. */
 int litho_last_timing(litho_ctx* ctx, float* total_ms, float* stage_ms, int nstage);
+/* Name of stage i of the last batch call (static string, "" if none). */
+const char* litho_stage_name(litho_ctx* ctx, int i);
 
 #ifdef __cplusplus
 }

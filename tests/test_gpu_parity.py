@@ -196,6 +196,41 @@ def test_batch_equals_single(gpu, orc, c1):
         ctx.close()
 
 
+@pytest.mark.parametrize("N", [512, 1024])
+def test_band_and_pow2_grids_agree(gpu, orc, base_source, N):
+    """The image path's 81^2 / 144^2 band grid and the reference's nextPow2
+    grid give the same image (FI is exact for any grid >= 4Nx+1)."""
+    case(f"band vs pow2 grid N={N}")
+    src, s = base_source
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    nk = 10 if N == 512 else 24
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, nk)
+    masks = np.stack([gpu.manhattan_mask(N, 40 + i) for i in range(2)])
+    outs = []
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm = ctx.alloc(masks.nbytes)
+        di = ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        for path in (gpu.LITHO_PATH_AUTO, gpu.LITHO_PATH_POW2):
+            ctx.set_path(path)
+            ctx.socs_image_batch(bank, dm, 2, N, di)
+            outs.append(ctx.d2h(np.zeros_like(masks), di))
+            _, stages = ctx.last_timing()
+            assert ("socs_band" in stages) == (path == gpu.LITHO_PATH_AUTO)
+        ctx.free(dm)
+        ctx.free(di)
+        gpu.lib().litho_bank_destroy(bank)
+    finally:
+        ctx.close()
+    ref = orc.calc_socs(orc.mask_spectrum(masks[1]), K.astype(np.complex128),
+                        sc.astype(np.float32), Nx, Nx, nthreads=8)
+    assert max_rel(outs[0][1], ref) < IMG_TOL
+    assert max_rel(outs[1][1], ref) < IMG_TOL
+    assert max_rel(outs[0], outs[1]) < IMG_TOL
+
+
 @pytest.mark.parametrize("N", [256, 2048])
 def test_other_clip_sizes(gpu, orc, base_source, N):
     case(f"socs_image N={N}")
