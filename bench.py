@@ -92,7 +92,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--path", choices=["auto", "pow2"], default="auto",
                     help="conv grid: auto (band grid 144 at Nx=34) or the reference nextPow2 grid")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        ctx = L.Context()
+        a, b = ctx.alloc(args.copy_probe), ctx.alloc(args.copy_probe)
+        for _ in range(3):
+            ctx.d2d(b, a, args.copy_probe)
+        log(f"copied {args.copy_probe} bytes device-to-device 3 times")
+        ctx.close()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested). Run with --gpus 1.", file=sys.stderr)
@@ -155,6 +165,15 @@ def main():
     step_dev_ms = float(sum(stage_acc.values()))
     achieved = path_bytes * B / (step_dev_ms * 1e-3) / 1e9
 
+    # measured HBM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated against a
+    # device copy) for this pipeline, scaled to the batch
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "socs_pipeline_1024_k24_plain_traffic.json")
+    if os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        if set(tj["stages"]) == set(stage_acc) and tj["N"] == N and tj["Nx"] == Nx:
+            traffic = int(tj["total_bytes_per_step"] * B / tj["batch"])
+
     out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
 
     cpu = None
@@ -190,7 +209,7 @@ def main():
                    "form": "plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "scope": f"whole path ({len(stage_acc)} launches per step); algorithmic "
                               "bytes = 8 N^2 per image (mask read + image write)",
                      "dominant_kernel": dom},

@@ -163,6 +163,13 @@ int litho_memcpy_d2h(litho_ctx* c, void* dst, const void* src, size_t bytes) {
     return LITHO_OK;
 }
 
+int litho_memcpy_d2d(litho_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) return LITHO_EARG;
+    LITHO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    return LITHO_OK;
+}
+
 int litho_memset_dev(litho_ctx* c, void* dst, int value, size_t bytes) {
     if (!c || !dst) return LITHO_EARG;
     LITHO_HIP(hipMemsetAsync(dst, value, bytes, c->stream));

@@ -27,7 +27,7 @@ _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 EXPORTS = [
     "litho_status_string", "litho_version", "litho_ctx_create", "litho_ctx_destroy",
     "litho_ctx_sync", "litho_dev_alloc", "litho_dev_free", "litho_memcpy_h2d",
-    "litho_memcpy_d2h", "litho_memset_dev", "litho_source_annular", "litho_source_quadrupole",
+    "litho_memcpy_d2h", "litho_memset_dev", "litho_memcpy_d2d", "litho_source_annular", "litho_source_quadrupole",
     "litho_source_dipole", "litho_source_point", "litho_outer_sigma", "litho_compute_nx",
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
@@ -74,6 +74,7 @@ def lib():
         L.litho_memcpy_h2d.argtypes = [vp, vp, vp, sz]
         L.litho_memcpy_d2h.argtypes = [vp, vp, vp, sz]
         L.litho_memset_dev.argtypes = [vp, vp, i, sz]
+        L.litho_memcpy_d2d.argtypes = [vp, vp, vp, sz]
         L.litho_source_annular.argtypes = [_f32p, i, f, f]
         L.litho_source_quadrupole.argtypes = [_f32p, i, f, f]
         L.litho_source_dipole.argtypes = [_f32p, i, f, f, i]
@@ -282,6 +283,9 @@ class Context:
     def d2h(self, arr, src):
         _check(lib().litho_memcpy_d2h(self.h, arr.ctypes.data, src, arr.nbytes), "d2h")
         return arr
+
+    def d2d(self, dst, src, nbytes):
+        _check(lib().litho_memcpy_d2d(self.h, dst, src, nbytes), "d2d")
 
     def sync(self):
         _check(lib().litho_ctx_sync(self.h), "litho_ctx_sync")

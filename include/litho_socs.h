@@ -68,6 +68,8 @@ int litho_dev_free(litho_ctx* ctx, void* dptr);
 int litho_memcpy_h2d(litho_ctx* ctx, void* dst, const void* src, size_t bytes);
 int litho_memcpy_d2h(litho_ctx* ctx, void* dst, const void* src, size_t bytes);
 int litho_memset_dev(litho_ctx* ctx, void* dst, int value, size_t bytes);
+/* device-to-device copy on the context stream (synchronous) */
+int litho_memcpy_d2d(litho_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- Module 1 helpers (host) --------------------------------------------- */
 /* createSource (common/source.cpp L105-141): annular L5-23, cross quadrupole
