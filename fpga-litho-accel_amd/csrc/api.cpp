@@ -4,6 +4,7 @@
 // through a per-thread default context; the device-pointer batch entry point
 // is asynchronous on the caller's context stream.  No entry point falls back
 // to the CPU: without a usable GPU every compute call returns LITHO_ENODEV.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,12 +53,22 @@ struct litho_ctx {
     int nstage = 0;
     const char* stage_names[NSTAGE] = {};
     int path = LITHO_PATH_AUTO;
+    // pipelined form: two more streams, chunk events, cached graph
+    hipStream_t s1 = nullptr, s2 = nullptr;
+    std::vector<hipEvent_t> evA, evB;
+    hipEvent_t evStart = nullptr, evEnd1 = nullptr, evEnd2 = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<long> gkey;
 };
 
 struct litho_bank {
     litho_ctx* ctx = nullptr;
     float2* d_bank = nullptr;
     float* d_scales = nullptr;
+    unsigned* d_bank16 = nullptr;  // bf16 (re | im << 16), scaled per kernel
+    unsigned* d_bankh = nullptr;   // fp16 (re, im) pairs, scaled per kernel
+    float* d_kscale = nullptr;
     int nk = 0, Nx = 0, Ny = 0;
 };
 
@@ -125,6 +136,14 @@ int litho_ctx_destroy(litho_ctx* c) {
     if (c->tw) (void)hipFree(c->tw);
     for (int i = 0; i <= NSTAGE; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto e : c->evA) (void)hipEventDestroy(e);
+    for (auto e : c->evB) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->evStart, c->evEnd1, c->evEnd2})
+        if (e) (void)hipEventDestroy(e);
+    if (c->s1) (void)hipStreamSynchronize(c->s1), (void)hipStreamDestroy(c->s1);
+    if (c->s2) (void)hipStreamSynchronize(c->s2), (void)hipStreamDestroy(c->s2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LITHO_OK;
@@ -682,6 +701,37 @@ int litho_bank_create(litho_ctx* c, const float* krn_r, const float* krn_i, cons
                              c->stream));
     LITHO_HIP(hipMemcpyAsync(b->d_scales, scales, nk * sizeof(float), hipMemcpyHostToDevice,
                              c->stream));
+    // bf16 copy with one scale per kernel (max |re|, |im|), round to nearest even
+    std::vector<unsigned> h16(h.size());
+    std::vector<float> ks(nk, 1.0f);
+    auto bf16 = [](float f) -> unsigned {
+        unsigned u;
+        std::memcpy(&u, &f, 4);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return u >> 16;
+    };
+    for (int k = 0; k < nk; k++) {
+        float m = 0.f;
+        for (size_t j = 0; j < per; j++)
+            m = std::max(m, std::max(std::fabs(h[k * per + j].x), std::fabs(h[k * per + j].y)));
+        ks[k] = m > 0.f ? m : 1.0f;
+        for (size_t j = 0; j < per; j++)
+            h16[k * per + j] = bf16(h[k * per + j].x / ks[k]) | (bf16(h[k * per + j].y / ks[k]) << 16);
+    }
+    std::vector<__half2> hh(h.size());
+    for (int k = 0; k < nk; k++)
+        for (size_t j = 0; j < per; j++)
+            hh[k * per + j] = __floats2half2_rn(h[k * per + j].x / ks[k], h[k * per + j].y / ks[k]);
+    if (hipMalloc(&b->d_bank16, h16.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&b->d_bankh, hh.size() * sizeof(__half2)) != hipSuccess ||
+        hipMalloc(&b->d_kscale, nk * sizeof(float)) != hipSuccess)
+        return LITHO_ENOMEM;
+    LITHO_HIP(hipMemcpyAsync(b->d_bankh, hh.data(), hh.size() * sizeof(__half2),
+                             hipMemcpyHostToDevice, c->stream));
+    LITHO_HIP(hipMemcpyAsync(b->d_bank16, h16.data(), h16.size() * sizeof(unsigned),
+                             hipMemcpyHostToDevice, c->stream));
+    LITHO_HIP(hipMemcpyAsync(b->d_kscale, ks.data(), nk * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
     LITHO_HIP(hipStreamSynchronize(c->stream));
     *out = b.release();
     return LITHO_OK;
@@ -691,6 +741,9 @@ int litho_bank_destroy(litho_bank* b) {
     if (!b) return LITHO_EARG;
     if (b->d_bank) (void)hipFree(b->d_bank);
     if (b->d_scales) (void)hipFree(b->d_scales);
+    if (b->d_bank16) (void)hipFree(b->d_bank16);
+    if (b->d_bankh) (void)hipFree(b->d_bankh);
+    if (b->d_kscale) (void)hipFree(b->d_kscale);
     delete b;
     return LITHO_OK;
 }
@@ -720,16 +773,16 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
                                       "image_rows"};
         LITHO_HIP(hipEventRecord(c->ev[0], st));
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, (float2*)R, (long)B * N, -Nx, W, st));
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, -Nx, W, false, st));
         LITHO_HIP(hipEventRecord(c->ev[1], st));
-        LITHO_TRY(launch_mask_cols_t(N, (float2*)R, (float2*)M, B, W, -Nx, W, st));
+        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, W, -Nx, W, false, st));
         LITHO_HIP(hipEventRecord(c->ev[2], st));
-        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, bank->d_scales, bank->nk, Nx, B,
-                                   (float2*)C, st));
+        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, nullptr, 0, bank->d_scales,
+                                   bank->nk, Nx, B, (float2*)C, st));
         LITHO_HIP(hipEventRecord(c->ev[3], st));
-        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, (float2*)U, B, band, st));
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, st));
         LITHO_HIP(hipEventRecord(c->ev[4], st));
-        LITHO_TRY(launch_image_rows_t(N, (float2*)U, d_image, (long)B * N, band, st));
+        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, st));
         LITHO_HIP(hipEventRecord(c->ev[5], st));
         c->nstage = 5;
         for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
@@ -763,6 +816,112 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     c->nstage = 6;
     for (int i = 0; i < 6; i++) c->stage_names[i] = names[i];
     c->timed = true;
+    return LITHO_OK;
+}
+
+static int ensure_pipeline(litho_ctx* c, int chunks) {
+    if (!c->s1) LITHO_HIP(hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking));
+    if (!c->s2) LITHO_HIP(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&c->evStart, &c->evEnd1, &c->evEnd2})
+        if (!*e) LITHO_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    while ((int)c->evA.size() < chunks) {
+        hipEvent_t a, b;
+        LITHO_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+        LITHO_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+        c->evA.push_back(a);
+        c->evB.push_back(b);
+    }
+    return LITHO_OK;
+}
+
+// Enqueue one pipelined step on c->stream (+ s1, s2); captured or direct.
+static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
+                             int N, float* d_image, int prec, int chunks, void* R, void* U,
+                             float2* M, float2* C, int Gb) {
+    const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
+    const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
+    const int cb = (B + chunks - 1) / chunks;
+    const size_t mstr = (size_t)W * W, cstr = (size_t)(2 * band + 1) * (band + 1);
+    hipStream_t s0 = c->stream;
+    LITHO_HIP(hipEventRecord(c->evStart, s0));
+    LITHO_HIP(hipStreamWaitEvent(c->s1, c->evStart, 0));
+    LITHO_HIP(hipStreamWaitEvent(c->s2, c->evStart, 0));
+    const int bfmt = (prec & LITHO_PREC_BANK_BF16) ? 1 : ((prec & LITHO_PREC_BANK_FP16) ? 2 : 0);
+    const void* kb = bfmt == 1 ? (const void*)bank->d_bank16
+                               : (bfmt == 2 ? (const void*)bank->d_bankh : (const void*)bank->d_bank);
+    const float* ks = bfmt ? bank->d_kscale : nullptr;
+    for (int k = 0; k < chunks; k++) {
+        const int b0 = k * cb;
+        const int nb = std::min(cb, B - b0);
+        if (nb <= 0) break;
+        const float* mk = d_mask + (size_t)b0 * N * N;
+        float* ik = d_image + (size_t)b0 * N * N;
+        float2* Mk = M + (size_t)b0 * mstr;
+        float2* Ck = C + (size_t)b0 * cstr;
+        LITHO_TRY(launch_rows_dft_t(N, mk, R, (long)nb * N, -Nx, W, mixed, s0));
+        LITHO_TRY(launch_mask_cols_t(N, R, Mk, nb, W, -Nx, W, mixed, s0));
+        LITHO_HIP(hipEventRecord(c->evA[k], s0));
+        LITHO_HIP(hipStreamWaitEvent(c->s1, c->evA[k], 0));
+        LITHO_TRY(launch_socs_band(Gb, Mk, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, nb, Ck, c->s1));
+        LITHO_HIP(hipEventRecord(c->evB[k], c->s1));
+        LITHO_HIP(hipStreamWaitEvent(c->s2, c->evB[k], 0));
+        LITHO_TRY(launch_fi_cols_t(N, Ck, U, nb, band, mixed, c->s2));
+        LITHO_TRY(launch_image_rows_t(N, U, ik, (long)nb * N, band, mixed, c->s2));
+    }
+    LITHO_HIP(hipEventRecord(c->evEnd1, c->s1));
+    LITHO_HIP(hipEventRecord(c->evEnd2, c->s2));
+    LITHO_HIP(hipStreamWaitEvent(s0, c->evEnd1, 0));
+    LITHO_HIP(hipStreamWaitEvent(s0, c->evEnd2, 0));
+    return LITHO_OK;
+}
+
+int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const float* d_mask,
+                                     int B, int N, float* d_image, int precision, int chunks,
+                                     int use_graph) {
+    if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535 || chunks <= 0 ||
+        chunks > 64 || (precision & ~7) ||
+        ((precision & LITHO_PREC_BANK_BF16) && (precision & LITHO_PREC_BANK_FP16)))
+        return LITHO_EARG;
+    const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
+    const int Gb = band_grid(Nx);
+    if (!is_pow2(N) || N < 256 || N > 1024 || !Gb || Gb > N) return LITHO_EARG;
+    const bool mixed = (precision & LITHO_PREC_INTER_FP16) != 0;
+    chunks = std::min(chunks, B);
+    const int cb = (B + chunks - 1) / chunks;
+    const size_t es = mixed ? 4 : sizeof(float2);  // half2 or float2
+    void *R, *U, *M, *C;
+    LITHO_TRY(ctx_ws(c, 0, (size_t)cb * W * N * es, &R));
+    LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
+    LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
+    LITHO_TRY(ctx_ws(c, 4, (size_t)cb * (band + 1) * N * es, &U));
+    LITHO_TRY(ensure_pipeline(c, chunks));
+    c->nstage = 0;
+    c->timed = false;
+    if (!use_graph)
+        return enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, R, U,
+                                 (float2*)M, (float2*)C, Gb);
+    std::vector<long> key = {(long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
+                             (long)R, (long)U, (long)M, (long)C};
+    if (!c->gexec || key != c->gkey) {
+        if (c->gexec) LITHO_HIP(hipGraphExecDestroy(c->gexec));
+        if (c->graph) LITHO_HIP(hipGraphDestroy(c->graph));
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        LITHO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, R, U,
+                                   (float2*)M, (float2*)C, Gb);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (st != LITHO_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        LITHO_HIP(e);
+        c->graph = g;
+        LITHO_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->gkey = key;
+    }
+    LITHO_HIP(hipGraphLaunch(c->gexec, c->stream));
     return LITHO_OK;
 }
 

@@ -15,11 +15,64 @@
 //
 // Every kernel takes its geometry as arguments and is launched with a grid
 // that the host checks against the buffers it was given.
+#include <hip/hip_fp16.h>
+
 #include "fft_lds.hpp"
 #include "fft_reg.hpp"
 #include "socs_kernels.hpp"
 
 namespace litho {
+
+// ---------------------------------------------------------------------------
+// Storage of the two HBM intermediates (row-DFT window R, FI column output U):
+// float2, or half2 in the mixed-precision form (FP32 arithmetic either way).
+// Kernel banks: float2, or bf16 (re, im) pairs with one FP32 scale per kernel.
+// ---------------------------------------------------------------------------
+template <bool H>
+struct Store;
+template <>
+struct Store<false> {
+    using T = float2;
+    __device__ static __forceinline__ float2 ld(const T& v) { return v; }
+    __device__ static __forceinline__ T st(float2 v) { return v; }
+};
+template <>
+struct Store<true> {
+    using T = __half2;
+    __device__ static __forceinline__ float2 ld(const T& v) { return __half22float2(v); }
+    __device__ static __forceinline__ T st(float2 v) { return __floats2half2_rn(v.x, v.y); }
+};
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) {
+    return __uint_as_float(((unsigned)h) << 16);
+}
+
+// BK = 0: float2; 1: bf16 pairs x kscale[k]; 2: fp16 pairs (half2) x kscale[k]
+template <int BK>
+struct Bank;
+template <>
+struct Bank<0> {
+    __device__ static __forceinline__ float2 ld(const void* b, const float*, long idx, int) {
+        return ((const float2*)b)[idx];
+    }
+};
+template <>
+struct Bank<1> {
+    __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
+        const unsigned v = ((const unsigned*)b)[idx];
+        const float s = ks[k];
+        return make_float2(bf16_to_f32((unsigned short)(v & 0xffffu)) * s,
+                           bf16_to_f32((unsigned short)(v >> 16)) * s);
+    }
+};
+template <>
+struct Bank<2> {
+    __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
+        const float2 v = __half22float2(((const __half2*)b)[idx]);
+        const float s = ks[k];
+        return make_float2(v.x * s, v.y * s);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // rows_dft: in  [nrows][L] f32 (nrows even), out [nrows][count] float2
@@ -300,12 +353,14 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
 // rows_fwd2: same contract as rows_dft (window [kmin, kmin+count) of the
 // forward DFT of each real row, (-1)^k when ALT).
 // ---------------------------------------------------------------------------
-template <int P, int Q, int NTH, bool ALT, bool OUT_T>
+template <int P, int Q, int NTH, bool ALT, bool OUT_T, bool H16 = false>
 __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict__ in,
-                                                        float2* __restrict__ out, int nrows,
+                                                        void* __restrict__ out_, int nrows,
                                                         int kmin, int count,
                                                         const float2* __restrict__ tw) {
     using Geo = Fft2Geom<P, Q>;
+    using S = Store<H16>;
+    typename S::T* out = (typename S::T*)out_;
     constexpr int L = Geo::G;
     constexpr int T = Geo::T;
     constexpr int FPB = NTH / T;
@@ -350,12 +405,12 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
         if (OUT_T) {
             // [img][j][row in img]: L rows per image (square clips)
             const long r = 2 * p, img = r / L, y = r - img * L;
-            float2* o = out + (img * count + j) * (long)L + y;
-            o[0] = x1;
-            o[1] = x2;
+            typename S::T* o = out + (img * count + j) * (long)L + y;
+            o[0] = S::st(x1);
+            o[1] = S::st(x2);
         } else {
-            out[(2 * p) * (long)count + j] = x1;
-            out[(2 * p + 1) * (long)count + j] = x2;
+            out[(2 * p) * (long)count + j] = S::st(x1);
+            out[(2 * p + 1) * (long)count + j] = S::st(x2);
         }
     }
 }
@@ -364,8 +419,8 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
 // (|kx| <= band) is placed at indices kx + band in [0, 2 band] so stage-1
 // inputs past ceil((2band+1)/Q) rows are compile-time zeros; the shift is
 // undone on the output by e^{-2 pi i band x / N}.
-template <int P, int Q, int NZ, int NTH, bool IN_T>
-__global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restrict__ U,
+template <int P, int Q, int NZ, int NTH, bool IN_T, bool H16 = false>
+__global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict__ U_,
                                                           float* __restrict__ img, int nrows,
                                                           int band,
                                                           const float2* __restrict__ tw) {
@@ -385,9 +440,11 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restri
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
     // IN_T: U is [img][kx][row in img] (L rows per image), else [row][kx]
+    using S = Store<H16>;
+    const typename S::T* U = (const typename S::T*)U_;
     const long r0 = 2 * (act ? p : 0);
     const long img0 = r0 / L, y0 = r0 - img0 * L;
-    const float2* u1 = IN_T ? U + img0 * nb * (long)L + y0 : U + r0 * (long)nb;
+    const typename S::T* u1 = IN_T ? U + img0 * nb * (long)L + y0 : U + r0 * (long)nb;
     const long ust = IN_T ? L : 1;       // stride between kx
     const long u2o = IN_T ? 1 : nb;      // offset of the second row
     fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
@@ -395,7 +452,7 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restri
         const int kx = n - band;
         if (act && kx >= -band && kx <= band) {
             const int a = kx < 0 ? -kx : kx;
-            float2 a1 = u1[a * ust], a2 = u1[a * ust + u2o];
+            float2 a1 = S::ld(u1[a * ust]), a2 = S::ld(u1[a * ust + u2o]);
             if (a == 0) {
                 z = make_float2(a1.x, a2.x);
             } else if (kx > 0) {
@@ -432,9 +489,10 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const float2* __restri
 // |ky| <= band, 0 <= kx <= band (band = 2 Nx).  This is FI's R2C (L724-735)
 // restricted to the band the intensity occupies.
 // ---------------------------------------------------------------------------
-template <int P, int NZ, int SLOTS>
+template <int P, int NZ, int SLOTS, int BK = 0>
 __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __restrict__ M,
-                                                              const float2* __restrict__ bank,
+                                                              const void* __restrict__ bank,
+                                                              const float* __restrict__ kscale,
                                                               const float* __restrict__ scales,
                                                               int nk, int Nx,
                                                               float2* __restrict__ Cout) {
@@ -461,7 +519,7 @@ __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __re
     __syncthreads();
 
     for (int k = 0; k < nk; k++) {
-        const float2* Kk = bank + (long)k * W * W;
+        const long kbase = (long)k * W * W;
         const float lam = scales[k];
         for (int c0 = 0; c0 < W; c0 += SLOTS) {
             const int col = c0 + slot;
@@ -470,7 +528,7 @@ __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __re
                 float2 v = make_float2(0.f, 0.f);
                 if (act && n < W) {
                     const int j = n * W + col;
-                    v = cmul(Kk[j], Mb[j]);
+                    v = cmul(Bank<BK>::ld(bank, kscale, kbase + j, k), Mb[j]);
                 }
                 return v;
             });
@@ -526,8 +584,8 @@ __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __re
 
 // mask_cols2: R [B][cx][N] (row-DFT window, transposed) -> M [B][cy][cx],
 // M[b][j][c] = (1/N^2) (-1)^ky DFT_N(R[b][c][:])[ky mod N], ky = kymin + j.
-template <int P, int Q, int NTH>
-__global__ __launch_bounds__(NTH) void mask_cols2_kernel(const float2* __restrict__ R,
+template <int P, int Q, int NTH, bool H16 = false>
+__global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict__ R_,
                                                          float2* __restrict__ M, int cx,
                                                          int kymin, int cy) {
     using Geo = Fft2Geom<P, Q>;
@@ -544,9 +602,10 @@ __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const float2* __restric
     fill_twiddles<N>(twG, tid, NTH);
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
-    const float2* Rc = R + (b * cx + (act ? col : 0)) * (long)N;
+    using S = Store<H16>;
+    const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N;
     fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
-        return act ? Rc[n] : make_float2(0.f, 0.f);
+        return act ? S::ld(Rc[n]) : make_float2(0.f, 0.f);
     });
     __syncthreads();
     if (!act) return;
@@ -562,9 +621,9 @@ __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const float2* __restric
 // fi_cols2: C [B][2band+1][nb] (band spectrum, kx = 0..band) -> U [B][nb][N]:
 // U[b][kx][y] = (-1)^kx sum_ky (-1)^ky C[ky][kx] e^{+2 pi i ky y / N}, the
 // band placed at indices ky + band and the shift undone on the output.
-template <int P, int Q, int NZ, int NTH>
+template <int P, int Q, int NZ, int NTH, bool H16 = false>
 __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict__ C,
-                                                       float2* __restrict__ U, int band) {
+                                                       void* __restrict__ U_, int band) {
     using Geo = Fft2Geom<P, Q>;
     constexpr int N = Geo::G;
     constexpr int T = Geo::T;
@@ -591,11 +650,12 @@ __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict_
     });
     __syncthreads();
     if (!act) return;
-    float2* Uc = U + (b * nb + col) * (long)N;
+    using S = Store<H16>;
+    typename S::T* Uc = (typename S::T*)U_ + (b * nb + col) * (long)N;
     const float sx = (col & 1) ? -1.f : 1.f;
     fft2_stage2<Geo, true>(t, xb, [&](int, int y, float2 val) {
         const float2 w = twG[(int)(((long)band * y) % N)];  // e^{-2 pi i band y / N}
-        Uc[y] = cscale(cmul(val, w), sx);
+        Uc[y] = S::st(cscale(cmul(val, w), sx));
     });
 }
 
@@ -975,14 +1035,14 @@ int band_grid(int Nx) {
     return 0;
 }
 
-template <int P, int NZ, int SLOTS>
-static int launch_band(const float2* M, const float2* bank, const float* scales, int nk, int Nx,
-                       int B, float2* C, hipStream_t st) {
+template <int P, int NZ, int SLOTS, int BK>
+static int launch_band(const float2* M, const void* bank, const float* kscale, const float* scales,
+                       int nk, int Nx, int B, float2* C, hipStream_t st) {
     using Geo = Fft2Geom<P, P>;
     const int W = 2 * Nx + 1;
     size_t lds = (size_t)(Geo::G + SLOTS * Geo::XBUF + Geo::G * W) * sizeof(float2);
     if (lds > 160 * 1024) return LITHO_EARG;
-    auto kern = socs_band_kernel<P, NZ, SLOTS>;
+    auto kern = socs_band_kernel<P, NZ, SLOTS, BK>;
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -990,16 +1050,22 @@ static int launch_band(const float2* M, const float2* bank, const float* scales,
             return LITHO_EHIP;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, dim3(B), dim3(P * SLOTS), lds, st, M, bank, scales, nk, Nx, C);
+    hipLaunchKernelGGL(kern, dim3(B), dim3(P * SLOTS), lds, st, M, bank, kscale, scales, nk, Nx, C);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
-int launch_socs_band(int G, const float2* M, const float2* bank, const float* scales, int nk,
-                     int Nx, int B, float2* C, hipStream_t st) {
+int launch_socs_band(int G, const float2* M, const void* bank, const float* kscale, int bank_fmt,
+                     const float* scales, int nk, int Nx, int B, float2* C, hipStream_t st) {
     const int W = 2 * Nx + 1;
-    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G) return LITHO_EARG;
-#define LITHO_BAND(p, nz, slots) \
-    if (G == p * p && W <= nz * p) return launch_band<p, nz, slots>(M, bank, scales, nk, Nx, B, C, st);
+    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 2) return LITHO_EARG;
+#define LITHO_BAND(p, nz, slots)                                                                  \
+    if (G == p * p && W <= nz * p) {                                                              \
+        if (bank_fmt == 1)                                                                        \
+            return launch_band<p, nz, slots, 1>(M, bank, kscale, scales, nk, Nx, B, C, st);       \
+        if (bank_fmt == 2)                                                                        \
+            return launch_band<p, nz, slots, 2>(M, bank, kscale, scales, nk, Nx, B, C, st);       \
+        return launch_band<p, nz, slots, 0>(M, bank, kscale, scales, nk, Nx, B, C, st);           \
+    }
     LITHO_BAND(6, 6, 36)
     LITHO_BAND(9, 4, 27)
     LITHO_BAND(9, 9, 27)
@@ -1009,15 +1075,19 @@ int launch_socs_band(int G, const float2* M, const float2* bank, const float* sc
     return LITHO_EARG;
 }
 
-int launch_rows_dft_t(int L, const float* in, float2* out, long nrows, int kmin, int count,
+int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, int count, bool h16,
                       hipStream_t st) {
     if (nrows % L || nrows % 2 || count <= 0) return LITHO_EARG;
 #define LITHO_RT(p, q, nth)                                                                   \
     if (L == p * q) {                                                                         \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                          \
         unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                       \
-        hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true>), dim3(grid), dim3(nth),  \
-                           0, st, in, out, (int)nrows, kmin, count, nullptr);                 \
+        if (h16)                                                                              \
+            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true, true>), dim3(grid),   \
+                               dim3(nth), 0, st, in, out, (int)nrows, kmin, count, nullptr);  \
+        else                                                                                  \
+            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true, false>), dim3(grid),  \
+                               dim3(nth), 0, st, in, out, (int)nrows, kmin, count, nullptr);  \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
     }
     LITHO_RT(16, 16, 256)
@@ -1027,14 +1097,18 @@ int launch_rows_dft_t(int L, const float* in, float2* out, long nrows, int kmin,
     return LITHO_EARG;
 }
 
-int launch_mask_cols_t(int N, const float2* R, float2* M, int B, int cx, int kymin, int cy,
+int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
                        hipStream_t st) {
     if (B <= 0 || B > 65535) return LITHO_EARG;
 #define LITHO_MC(p, q, nth)                                                                    \
     if (N == p * q) {                                                                          \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
-        hipLaunchKernelGGL((mask_cols2_kernel<p, q, nth>), dim3(cdiv(cx, FPB), B), dim3(nth),  \
-                           0, st, R, M, cx, kymin, cy);                                        \
+        if (h16)                                                                               \
+            hipLaunchKernelGGL((mask_cols2_kernel<p, q, nth, true>), dim3(cdiv(cx, FPB), B),   \
+                               dim3(nth), 0, st, R, M, cx, kymin, cy);                         \
+        else                                                                                   \
+            hipLaunchKernelGGL((mask_cols2_kernel<p, q, nth, false>), dim3(cdiv(cx, FPB), B),  \
+                               dim3(nth), 0, st, R, M, cx, kymin, cy);                         \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     LITHO_MC(16, 16, 256)
@@ -1044,13 +1118,17 @@ int launch_mask_cols_t(int N, const float2* R, float2* M, int B, int cx, int kym
     return LITHO_EARG;
 }
 
-int launch_fi_cols_t(int N, const float2* C, float2* U, int B, int band, hipStream_t st) {
+int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, hipStream_t st) {
     if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
 #define LITHO_FC(p, q, nz, nth)                                                                 \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                                 \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                            \
-        hipLaunchKernelGGL((fi_cols2_kernel<p, q, nz, nth>), dim3(cdiv(band + 1, FPB), B),      \
-                           dim3(nth), 0, st, C, U, band);                                       \
+        if (h16)                                                                                \
+            hipLaunchKernelGGL((fi_cols2_kernel<p, q, nz, nth, true>),                          \
+                               dim3(cdiv(band + 1, FPB), B), dim3(nth), 0, st, C, U, band);     \
+        else                                                                                    \
+            hipLaunchKernelGGL((fi_cols2_kernel<p, q, nz, nth, false>),                         \
+                               dim3(cdiv(band + 1, FPB), B), dim3(nth), 0, st, C, U, band);     \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                         \
     }
     LITHO_FC(16, 16, 3, 256)
@@ -1063,15 +1141,19 @@ int launch_fi_cols_t(int N, const float2* C, float2* U, int B, int band, hipStre
     return LITHO_EARG;
 }
 
-int launch_image_rows_t(int N, const float2* U, float* img, long nrows, int band,
+int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, bool h16,
                         hipStream_t st) {
     if (nrows % N || nrows % 2 || 2 * band + 1 > N) return LITHO_EARG;
 #define LITHO_IT(p, q, nz, nth)                                                                \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                                \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
         unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                        \
-        hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true>), dim3(grid), dim3(nth),   \
-                           0, st, U, img, (int)nrows, band, nullptr);                          \
+        if (h16)                                                                               \
+            hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true, true>), dim3(grid),    \
+                               dim3(nth), 0, st, U, img, (int)nrows, band, nullptr);           \
+        else                                                                                   \
+            hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true, false>), dim3(grid),   \
+                               dim3(nth), 0, st, U, img, (int)nrows, band, nullptr);           \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     LITHO_IT(16, 16, 3, 256)

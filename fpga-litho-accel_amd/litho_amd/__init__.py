@@ -32,8 +32,13 @@ EXPORTS = [
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
     "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_ctx_set_path",
-    "litho_last_timing", "litho_stage_name",
+    "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
 ]
+LITHO_PREC_FP32 = 0
+LITHO_PREC_INTER_FP16 = 1
+LITHO_PREC_BANK_BF16 = 2
+LITHO_PREC_BANK_FP16 = 4
+LITHO_PREC_MIXED = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16
 LITHO_PATH_AUTO = 0
 LITHO_PATH_POW2 = 1
 
@@ -98,6 +103,7 @@ def lib():
         L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
         L.litho_last_timing.argtypes = [vp, C.POINTER(f), C.POINTER(f), i]
         L.litho_ctx_set_path.argtypes = [vp, i]
+        L.litho_socs_image_batch_pipelined.argtypes = [vp, vp, vp, i, i, vp, i, i, i]
         L.litho_stage_name.argtypes = [vp, i]
         L.litho_stage_name.restype = C.c_char_p
         for name in EXPORTS:
@@ -301,6 +307,12 @@ class Context:
     def socs_image_batch(self, bank, d_mask, B, N, d_image):
         _check(lib().litho_socs_image_batch(self.h, bank, d_mask, B, N, d_image),
                "litho_socs_image_batch")
+
+    def socs_image_batch_pipelined(self, bank, d_mask, B, N, d_image, precision=5, chunks=4,
+                                   graph=True):
+        _check(lib().litho_socs_image_batch_pipelined(self.h, bank, d_mask, B, N, d_image,
+                                                      precision, chunks, int(graph)),
+               "litho_socs_image_batch_pipelined")
 
     def set_path(self, path):
         _check(lib().litho_ctx_set_path(self.h, path), "litho_ctx_set_path")

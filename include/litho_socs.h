@@ -152,6 +152,25 @@ int litho_last_timing(litho_ctx* ctx, float* total_ms, float* stage_ms, int nsta
 /* Name of stage i of the last batch call (static string, "" if none). */
 const char* litho_stage_name(litho_ctx* ctx, int i);
 
+/* Batched form (no counterpart in the reference): the batch is split into
+ * `chunks` sub-batches that are software-pipelined over three streams (mask
+ * spectrum | SOCS accumulate | Fourier interpolation), and the whole step is
+ * captured once as a HIP graph (per shape and buffers) and replayed.
+ * `precision` is a set of storage flags; arithmetic stays FP32 in all of them:
+ * INTER_FP16 stores the two HBM intermediates (row-DFT window, FI column
+ * output) as fp16; BANK_BF16 / BANK_FP16 read the kernel bank as 16-bit pairs
+ * with one FP32 scale per kernel.  LITHO_PREC_MIXED is the combination that
+ * meets 1e-3 max relative error against FP64.  Uses the band conv grid
+ * (N in {256, 512, 1024}, 4Nx+1 <= 144). */
+#define LITHO_PREC_FP32 0
+#define LITHO_PREC_INTER_FP16 1
+#define LITHO_PREC_BANK_BF16 2
+#define LITHO_PREC_BANK_FP16 4
+#define LITHO_PREC_MIXED (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16)
+int litho_socs_image_batch_pipelined(litho_ctx* ctx, const litho_bank* bank, const float* d_mask,
+                                     int B, int N, float* d_image, int precision, int chunks,
+                                     int use_graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -262,3 +262,48 @@ def test_invalid_requests_raise(gpu, c1):
         gpu.socs_image(np.zeros((100, 100), np.float32), c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
     with pytest.raises(gpu.LithoError):
         gpu.socs_image(np.zeros((64, 64), np.float32), c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+
+
+@pytest.mark.parametrize("precision,graph", [(0, False), (0, True), (5, True)])
+def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
+    """The pipelined / graph-captured batch equals the plain path (FP32) or
+    stays within 1e-3 of FP64 (mixed storage), including a replayed graph and
+    a batch that does not divide into the sub-batches."""
+    case(f"pipelined batch precision={precision} graph={graph}")
+    src, s = base_source
+    N = 1024
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, 24)
+    B = 5
+    masks = np.stack([gpu.manhattan_mask(N, 60 + i) for i in range(B)])
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm = ctx.alloc(masks.nbytes)
+        d1 = ctx.alloc(masks.nbytes)
+        d2 = ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        ctx.socs_image_batch(bank, dm, B, N, d1)
+        plain = ctx.d2h(np.zeros_like(masks), d1)
+        for _ in range(3):  # capture once, replay
+            ctx.socs_image_batch_pipelined(bank, dm, B, N, d2, precision, 2, graph)
+            ctx.sync()
+            got = ctx.d2h(np.zeros_like(masks), d2)
+            if precision == 0:
+                assert np.array_equal(got, plain)
+        ref = orc.calc_socs(orc.mask_spectrum(masks[4]), K.astype(np.complex128),
+                            sc.astype(np.float32), Nx, Nx, nthreads=8)
+        tol = IMG_TOL if precision == 0 else 1e-3
+        assert max_rel(got[4], ref) < tol
+        if precision:
+            # report the error of every storage flag on its own
+            for flag in (1, 2, 4):
+                ctx.socs_image_batch_pipelined(bank, dm, B, N, d2, flag, 2, False)
+                ctx.sync()
+                e = max_rel(ctx.d2h(np.zeros_like(masks), d2)[4], ref)
+                print(f"[precision flag {flag}] max rel err vs FP64 {e:.3e}", flush=True)
+        for p_ in (dm, d1, d2):
+            ctx.free(p_)
+        gpu.lib().litho_bank_destroy(bank)
+    finally:
+        ctx.close()
