@@ -996,6 +996,39 @@ int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn
     return LITHO_OK;
 }
 
+int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double* tcc_im, int Nx,
+                    int Ny, float* image) {
+    if (!mask || !tcc_re || !tcc_im || !image || Nx < 0 || Ny != Nx || !is_pow2(N) || N < 256 ||
+        N > 1024 || 4 * Nx + 1 > N)
+        return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    const int W = 2 * Nx + 1, n = W * W, band = 2 * Nx;
+    const size_t NN = (size_t)N * N;
+    std::vector<double2> ht((size_t)n * n);
+    for (size_t i = 0; i < ht.size(); i++) ht[i] = make_double2(tcc_re[i], tcc_im[i]);
+    Dev dm, dR, dM, dT, dC, dU, dI;
+    LITHO_TRY(dm.alloc(NN * sizeof(float)));
+    LITHO_TRY(dR.alloc((size_t)W * N * sizeof(float2)));
+    LITHO_TRY(dM.alloc((size_t)n * sizeof(float2)));
+    LITHO_TRY(dT.alloc(ht.size() * sizeof(double2)));
+    LITHO_TRY(dC.alloc((size_t)(2 * band + 1) * W * sizeof(float2)));
+    LITHO_TRY(dU.alloc((size_t)W * N * sizeof(float2)));
+    LITHO_TRY(dI.alloc(NN * sizeof(float)));
+    LITHO_HIP(hipMemcpyAsync(dm.p, mask, NN * sizeof(float), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(dT.p, ht.data(), ht.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, -Nx, W, false, st));
+    LITHO_TRY(launch_mask_cols_t(N, dR.p, dM.as<float2>(), 1, W, -Nx, W, false, st));
+    LITHO_TRY(launch_tcc_image_spectrum(dM.as<float2>(), dT.as<double2>(), Nx, Ny,
+                                        dC.as<float2>(), st));
+    LITHO_TRY(launch_fi_cols_t(N, dC.as<float2>(), dU.p, 1, band, false, st));
+    LITHO_TRY(launch_image_rows_t(N, dU.p, dI.as<float>(), N, band, false, st));
+    LITHO_HIP(hipMemcpyAsync(image, dI.p, NN * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    return LITHO_OK;
+}
+
 int litho_fi(const float* tmpImgp, int G, int N, int band, float* image) {
     if (!tmpImgp || !image || !is_pow2(G) || !is_pow2(N) || G > N || G > 1024 || N > 4096 ||
         band < 0 || 2 * band + 1 > G || G < 32)

@@ -207,4 +207,53 @@ int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
+
+// ---------------------------------------------------------------------------
+// TCC-direct image spectrum (calcImage, litho_thread.cpp L539-583), FP64:
+//   C[ny2 + 2Ny][nx2] = sum_{n1} M(n2+n1) conj(M(n1)) TCC[n1][n2+n1]
+// for ny2 in [-2Ny, 2Ny], nx2 in [0, 2Nx], n1 and n2+n1 inside the window.
+// M is the (2Ny+1) x (2Nx+1) centred mask spectrum window (float2); one
+// block per output, the n1 sum reduced across the block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void tcc_image_spectrum_kernel(const float2* __restrict__ M,
+                                                                 const double2* __restrict__ T,
+                                                                 int Nx, int Ny,
+                                                                 float2* __restrict__ C) {
+    __shared__ double sr[256], si[256];
+    const int W = 2 * Nx + 1, H = 2 * Ny + 1, n = W * H;
+    const int nx2 = blockIdx.x, ny2 = (int)blockIdx.y - 2 * Ny;
+    double ar = 0.0, ai = 0.0;
+    for (int id1 = threadIdx.x; id1 < n; id1 += 256) {
+        const int nx1 = id1 % W - Nx, ny1 = id1 / W - Ny;
+        const int mx = nx2 + nx1, my = ny2 + ny1;
+        if (mx < -Nx || mx > Nx || my < -Ny || my > Ny) continue;
+        const int id2 = (my + Ny) * W + (mx + Nx);
+        const float2 a = M[id2], b = M[id1];
+        // (a * conj(b)) * T[id1][id2]
+        const double pr = (double)a.x * b.x + (double)a.y * b.y;
+        const double pi = (double)a.y * b.x - (double)a.x * b.y;
+        const double2 t = T[(long)id1 * n + id2];
+        ar += pr * t.x - pi * t.y;
+        ai += pr * t.y + pi * t.x;
+    }
+    sr[threadIdx.x] = ar;
+    si[threadIdx.x] = ai;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            sr[threadIdx.x] += sr[threadIdx.x + st];
+            si[threadIdx.x] += si[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) C[(long)blockIdx.y * W + nx2] = make_float2((float)sr[0], (float)si[0]);
+}
+
+int launch_tcc_image_spectrum(const float2* M, const double2* T, int Nx, int Ny, float2* C,
+                              hipStream_t st) {
+    dim3 grid(2 * Nx + 1, 4 * Ny + 1);
+    hipLaunchKernelGGL(tcc_image_spectrum_kernel, grid, dim3(256), 0, st, M, T, Nx, Ny, C);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 }  // namespace litho

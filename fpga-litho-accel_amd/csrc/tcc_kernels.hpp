@@ -24,4 +24,9 @@ int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m,
 int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
                      int has_imag, double2* C, hipStream_t st);
 
+// TCC-direct image band spectrum C [(4Ny+1)][(2Nx+1)] from the mask window M
+// [(2Ny+1)][(2Nx+1)] and the n x n TCC (calcImage)
+int launch_tcc_image_spectrum(const float2* M, const double2* T, int Nx, int Ny, float2* C,
+                              hipStream_t st);
+
 }  // namespace litho

@@ -31,6 +31,7 @@ EXPORTS = [
     "litho_source_dipole", "litho_source_point", "litho_outer_sigma", "litho_compute_nx",
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
+    "litho_image_tcc",
     "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_ctx_set_path",
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
 ]
@@ -98,6 +99,7 @@ def lib():
                                         i, i, i, i, i]
         L.litho_socs_image.argtypes = [_f32p, i, _f32p, _f32p, _f32p, i, i, i, _f32p]
         L.litho_fi.argtypes = [_f32p, i, i, i, _f32p]
+        L.litho_image_tcc.argtypes = [_f32p, i, _f64p, _f64p, i, i, _f32p]
         L.litho_bank_create.argtypes = [vp, _f32p, _f32p, _f32p, i, i, i, C.POINTER(vp)]
         L.litho_bank_destroy.argtypes = [vp]
         L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
@@ -252,6 +254,18 @@ def calcSOCS(mskf, krns, scales, Nx, Ny):
     mskf = np.asarray(mskf)
     t = calc_socs_hls(mskf, krns, scales, Nx, Ny)
     return FI(t, mskf.shape[0], 2 * Nx)
+
+
+def calcImage(mask, tcc, Nx, Ny):
+    """TCC-direct image (calcImage + FT_c2r_padded, L539-583 / L371-425) on the GPU."""
+    mask = np.ascontiguousarray(mask, np.float32)
+    N = mask.shape[0]
+    tcc = np.asarray(tcc, np.complex128)
+    out = np.zeros(N * N, np.float32)
+    _check(lib().litho_image_tcc(mask.ravel(), N, np.ascontiguousarray(tcc.real).ravel(),
+                                 np.ascontiguousarray(tcc.imag).ravel(), Nx, Ny, out),
+           "litho_image_tcc")
+    return out.reshape(N, N)
 
 
 def socs_image(mask, krns, scales, Nx, Ny):

@@ -129,6 +129,13 @@ int litho_socs_image(const float* mask, int N, const float* krn_r, const float* 
  * tmpImgp) -> N x N band-limited interpolation. */
 int litho_fi(const float* tmpImgp, int G, int N, int band, float* image);
 
+/* ---- Module 5 (verification): calcImage + FT_c2r_padded ------------------- */
+/* TCC-direct aerial image (litho_thread.cpp L539-583, L371-425): the full-rank
+ * reference image SOCS converges to as nk -> n.  Square N (256, 512, 1024),
+ * Nx == Ny, tcc_* n x n (both triangles read). */
+int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double* tcc_im, int Nx,
+                    int Ny, float* image);
+
 /* ---- device-resident batched path (the bench path) ------------------------- */
 /* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin). */
 int litho_bank_create(litho_ctx* ctx, const float* krn_r, const float* krn_i,

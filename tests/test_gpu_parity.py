@@ -307,3 +307,57 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
         gpu.lib().litho_bank_destroy(bank)
     finally:
         ctx.close()
+
+
+def test_tcc_direct_image_reference_fixture(gpu, orc, fixture_tcc):
+    """GPU calcImage reproduces the reference's image_expected_512x512.bin."""
+    case("tcc-direct image fixture")
+    m = np.fromfile(golden("mask_256x256.bin"), np.float32).reshape(256, 256)
+    M = np.zeros((512, 512), np.float32)
+    M[128:384, 128:384] = m
+    img = gpu.calcImage(M, fixture_tcc, 4, 4)
+    gold = np.fromfile(golden("image_expected_512x512.bin"), np.float32).reshape(512, 512)
+    assert max_rel(img, gold) < IMG_TOL
+
+
+def test_tcc_direct_image_c1_and_full_rank_socs(gpu, orc, c1):
+    """TCC-direct image on the 512^2 BASELINE shape vs the oracle, and the
+    SOCS image with every kernel (K = n via the explicit route) converging to it."""
+    case("tcc-direct image C1")
+    T = orc.calc_tcc(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
+    m = gpu.manhattan_mask(c1["N"], 77)
+    img = gpu.calcImage(m, T, c1["Nx"], c1["Nx"])
+    ref = orc.calc_image_tcc(orc.mask_spectrum(m), T, c1["Nx"], c1["Nx"])
+    assert max_rel(img, ref) < IMG_TOL
+    # rank of the TCC is 572 (< n = 1225): the top 572 kernels span it
+    sc, K, _ = gpu.kernels_from_source(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048,
+                                       c1["Nx"], c1["Nx"], 572)
+    socs = gpu.socs_image(m, K, sc, c1["Nx"], c1["Nx"])
+    assert max_rel(socs, img) < 1e-4
+
+
+def test_large_clip_implicit_tcc(gpu, orc, base_source):
+    """4096^2 clip at 4 nm (period 16384 nm): Nx = 137, a 75625-dim TCC that is
+    never formed; the source-space Gram route gives K = 64 kernels (checked
+    against LAPACK on the Gram and by their residual through B B^H), and the
+    image matches the FP64 oracle fed the same kernels."""
+    case("large clip N=4096 Nx=137 K=64")
+    src, s = base_source
+    N = 4096
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    assert Nx == 137
+    sc, K, fl = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, 64)
+    assert fl & gpu.LITHO_FLAG_GRAM_ROUTE
+    A, S = orc.pupil_matrix(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx)
+    B = A * np.sqrt(S)[None, :]
+    w = np.linalg.eigvalsh(B.conj().T @ B)[::-1]
+    assert np.abs(sc - w[:64]).max() / w[0] < 1e-10
+    V = K[:, ::-1].T.astype(np.complex128)  # v = flip(K)
+    R = B @ (B.conj().T @ V) - V * sc
+    assert np.abs(R).max() / sc[0] < 1e-5  # f32 kernel storage
+    del A, B
+    m = gpu.manhattan_mask(N, 99)
+    img = gpu.socs_image(m, K, sc, Nx, Nx)
+    ref = orc.calc_socs(orc.mask_spectrum(m), K.astype(np.complex128), sc.astype(np.float32), Nx,
+                        Nx, nthreads=16)
+    assert max_rel(img, ref) < IMG_TOL
