@@ -53,10 +53,10 @@ struct litho_ctx {
     int nstage = 0;
     const char* stage_names[NSTAGE] = {};
     int path = LITHO_PATH_AUTO;
-    // pipelined form: two more streams, chunk events, cached graph
-    hipStream_t s1 = nullptr, s2 = nullptr;
-    std::vector<hipEvent_t> evA, evB;
-    hipEvent_t evStart = nullptr, evEnd1 = nullptr, evEnd2 = nullptr;
+    // batched form: extra streams and their join events, cached graph
+    std::vector<hipStream_t> xs;
+    std::vector<hipEvent_t> xe;
+    hipEvent_t evStart = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<long> gkey;
@@ -138,12 +138,9 @@ int litho_ctx_destroy(litho_ctx* c) {
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    for (auto e : c->evA) (void)hipEventDestroy(e);
-    for (auto e : c->evB) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->evStart, c->evEnd1, c->evEnd2})
-        if (e) (void)hipEventDestroy(e);
-    if (c->s1) (void)hipStreamSynchronize(c->s1), (void)hipStreamDestroy(c->s1);
-    if (c->s2) (void)hipStreamSynchronize(c->s2), (void)hipStreamDestroy(c->s2);
+    for (auto e : c->xe) (void)hipEventDestroy(e);
+    if (c->evStart) (void)hipEventDestroy(c->evStart);
+    for (auto x : c->xs) (void)hipStreamSynchronize(x), (void)hipStreamDestroy(x);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LITHO_OK;
@@ -773,16 +770,16 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
                                       "image_rows"};
         LITHO_HIP(hipEventRecord(c->ev[0], st));
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, -Nx, W, false, st));
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, -Nx, W, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[1], st));
-        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, W, -Nx, W, false, st));
+        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, W, -Nx, W, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[2], st));
         LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, nullptr, 0, bank->d_scales,
-                                   bank->nk, Nx, B, (float2*)C, st));
+                                   bank->nk, Nx, B, (float2*)C, false, st));
         LITHO_HIP(hipEventRecord(c->ev[3], st));
-        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, st));
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[4], st));
-        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, st));
+        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[5], st));
         c->nstage = 5;
         for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
@@ -819,59 +816,59 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     return LITHO_OK;
 }
 
-static int ensure_pipeline(litho_ctx* c, int chunks) {
-    if (!c->s1) LITHO_HIP(hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking));
-    if (!c->s2) LITHO_HIP(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&c->evStart, &c->evEnd1, &c->evEnd2})
-        if (!*e) LITHO_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-    while ((int)c->evA.size() < chunks) {
-        hipEvent_t a, b;
-        LITHO_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-        LITHO_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
-        c->evA.push_back(a);
-        c->evB.push_back(b);
+static int ensure_pipeline(litho_ctx* c, int nstreams) {
+    while ((int)c->xs.size() < nstreams - 1) {
+        hipStream_t s;
+        LITHO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        c->xs.push_back(s);
+        hipEvent_t e;
+        LITHO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->xe.push_back(e);
     }
+    if (!c->evStart) LITHO_HIP(hipEventCreateWithFlags(&c->evStart, hipEventDisableTiming));
     return LITHO_OK;
 }
 
-// Enqueue one pipelined step on c->stream (+ s1, s2); captured or direct.
+// Enqueue one batched step: sub-batch k runs its five launches in order on
+// stream k % ns; different sub-batches overlap across streams.  Captured or
+// direct.
 static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
-                             int N, float* d_image, int prec, int chunks, void* R, void* U,
-                             float2* M, float2* C, int Gb) {
+                             int N, float* d_image, int prec, int chunks, int ns, char* R,
+                             size_t rstr, char* U, size_t ustr, float2* M, float2* C, int Gb) {
     const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
+    const bool lean = chunks > 1;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
     const int cb = (B + chunks - 1) / chunks;
     const size_t mstr = (size_t)W * W, cstr = (size_t)(2 * band + 1) * (band + 1);
-    hipStream_t s0 = c->stream;
-    LITHO_HIP(hipEventRecord(c->evStart, s0));
-    LITHO_HIP(hipStreamWaitEvent(c->s1, c->evStart, 0));
-    LITHO_HIP(hipStreamWaitEvent(c->s2, c->evStart, 0));
     const int bfmt = (prec & LITHO_PREC_BANK_BF16) ? 1 : ((prec & LITHO_PREC_BANK_FP16) ? 2 : 0);
     const void* kb = bfmt == 1 ? (const void*)bank->d_bank16
                                : (bfmt == 2 ? (const void*)bank->d_bankh : (const void*)bank->d_bank);
     const float* ks = bfmt ? bank->d_kscale : nullptr;
+    hipStream_t s0 = c->stream;
+    LITHO_HIP(hipEventRecord(c->evStart, s0));
+    for (int i = 1; i < ns; i++) LITHO_HIP(hipStreamWaitEvent(c->xs[i - 1], c->evStart, 0));
     for (int k = 0; k < chunks; k++) {
         const int b0 = k * cb;
         const int nb = std::min(cb, B - b0);
         if (nb <= 0) break;
+        hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
+        void* Rk = R + (k % ns) * rstr;
+        void* Uk = U + (k % ns) * ustr;
         const float* mk = d_mask + (size_t)b0 * N * N;
         float* ik = d_image + (size_t)b0 * N * N;
         float2* Mk = M + (size_t)b0 * mstr;
         float2* Ck = C + (size_t)b0 * cstr;
-        LITHO_TRY(launch_rows_dft_t(N, mk, R, (long)nb * N, -Nx, W, mixed, s0));
-        LITHO_TRY(launch_mask_cols_t(N, R, Mk, nb, W, -Nx, W, mixed, s0));
-        LITHO_HIP(hipEventRecord(c->evA[k], s0));
-        LITHO_HIP(hipStreamWaitEvent(c->s1, c->evA[k], 0));
-        LITHO_TRY(launch_socs_band(Gb, Mk, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, nb, Ck, c->s1));
-        LITHO_HIP(hipEventRecord(c->evB[k], c->s1));
-        LITHO_HIP(hipStreamWaitEvent(c->s2, c->evB[k], 0));
-        LITHO_TRY(launch_fi_cols_t(N, Ck, U, nb, band, mixed, c->s2));
-        LITHO_TRY(launch_image_rows_t(N, U, ik, (long)nb * N, band, mixed, c->s2));
+        LITHO_TRY(launch_rows_dft_t(N, mk, Rk, (long)nb * N, -Nx, W, mixed, lean, st));
+        LITHO_TRY(launch_mask_cols_t(N, Rk, Mk, nb, W, -Nx, W, mixed, lean, st));
+        LITHO_TRY(launch_socs_band(Gb, Mk, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, nb, Ck, lean,
+                                   st));
+        LITHO_TRY(launch_fi_cols_t(N, Ck, Uk, nb, band, mixed, lean, st));
+        LITHO_TRY(launch_image_rows_t(N, Uk, ik, (long)nb * N, band, mixed, lean, st));
     }
-    LITHO_HIP(hipEventRecord(c->evEnd1, c->s1));
-    LITHO_HIP(hipEventRecord(c->evEnd2, c->s2));
-    LITHO_HIP(hipStreamWaitEvent(s0, c->evEnd1, 0));
-    LITHO_HIP(hipStreamWaitEvent(s0, c->evEnd2, 0));
+    for (int i = 1; i < ns; i++) {
+        LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
+        LITHO_HIP(hipStreamWaitEvent(s0, c->xe[i - 1], 0));
+    }
     return LITHO_OK;
 }
 
@@ -887,19 +884,21 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
     if (!is_pow2(N) || N < 256 || N > 1024 || !Gb || Gb > N) return LITHO_EARG;
     const bool mixed = (precision & LITHO_PREC_INTER_FP16) != 0;
     chunks = std::min(chunks, B);
+    const int ns = std::min(chunks, 4);  // GPU_MAX_HW_QUEUES is 4 by default
     const int cb = (B + chunks - 1) / chunks;
     const size_t es = mixed ? 4 : sizeof(float2);  // half2 or float2
+    const size_t rstr = (size_t)cb * W * N * es, ustr = (size_t)cb * (band + 1) * N * es;
     void *R, *U, *M, *C;
-    LITHO_TRY(ctx_ws(c, 0, (size_t)cb * W * N * es, &R));
+    LITHO_TRY(ctx_ws(c, 0, ns * rstr, &R));
     LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
     LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
-    LITHO_TRY(ctx_ws(c, 4, (size_t)cb * (band + 1) * N * es, &U));
-    LITHO_TRY(ensure_pipeline(c, chunks));
+    LITHO_TRY(ctx_ws(c, 4, ns * ustr, &U));
+    LITHO_TRY(ensure_pipeline(c, ns));
     c->nstage = 0;
     c->timed = false;
     if (!use_graph)
-        return enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, R, U,
-                                 (float2*)M, (float2*)C, Gb);
+        return enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, ns, (char*)R,
+                                 rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb);
     std::vector<long> key = {(long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
                              (long)R, (long)U, (long)M, (long)C};
     if (!c->gexec || key != c->gkey) {
@@ -908,8 +907,8 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         c->gexec = nullptr;
         c->graph = nullptr;
         LITHO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, R, U,
-                                   (float2*)M, (float2*)C, Gb);
+        int st = enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, ns,
+                                   (char*)R, rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st != LITHO_OK) {
@@ -1018,12 +1017,12 @@ int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double
     LITHO_TRY(dI.alloc(NN * sizeof(float)));
     LITHO_HIP(hipMemcpyAsync(dm.p, mask, NN * sizeof(float), hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(dT.p, ht.data(), ht.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, -Nx, W, false, st));
-    LITHO_TRY(launch_mask_cols_t(N, dR.p, dM.as<float2>(), 1, W, -Nx, W, false, st));
+    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, -Nx, W, false, false, st));
+    LITHO_TRY(launch_mask_cols_t(N, dR.p, dM.as<float2>(), 1, W, -Nx, W, false, false, st));
     LITHO_TRY(launch_tcc_image_spectrum(dM.as<float2>(), dT.as<double2>(), Nx, Ny,
                                         dC.as<float2>(), st));
-    LITHO_TRY(launch_fi_cols_t(N, dC.as<float2>(), dU.p, 1, band, false, st));
-    LITHO_TRY(launch_image_rows_t(N, dU.p, dI.as<float>(), N, band, false, st));
+    LITHO_TRY(launch_fi_cols_t(N, dC.as<float2>(), dU.p, 1, band, false, false, st));
+    LITHO_TRY(launch_image_rows_t(N, dU.p, dI.as<float>(), N, band, false, false, st));
     LITHO_HIP(hipMemcpyAsync(image, dI.p, NN * sizeof(float), hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
     return LITHO_OK;

@@ -200,14 +200,15 @@ struct Fft2Geom {
 // n1 < NZ, i.e. n < NZ*Q; larger inputs are zero), multiplies by the
 // inter-stage twiddles (twG[m] = W_G^m forward, from LDS) and writes the
 // transpose buffer xb[k1 * LDQ + n2].
-// W_G^m = e^{-2 pi i m / G} for m < G, from double sincospi rounded once
-// (G need not be a power of two).
+// W_G^m = e^{-2 pi i m / G} for m < G (G need not be a power of two), from
+// single-precision sincospi of the reduced argument 2m/G (the pi-scaled
+// argument keeps the reduction exact; results within ~1 ulp).
 template <int G>
 __device__ __forceinline__ void fill_twiddles(float2* __restrict__ twG, int tid, int nth) {
     for (int i = tid; i < G; i += nth) {
-        double sn, cs;
-        sincospi(2.0 * (double)i / (double)G, &sn, &cs);
-        twG[i] = make_float2((float)cs, (float)-sn);
+        float sn, cs;
+        sincospif(2.0f * (float)i / (float)G, &sn, &cs);
+        twG[i] = make_float2(cs, -sn);
     }
 }
 

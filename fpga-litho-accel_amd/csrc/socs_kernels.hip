@@ -1055,9 +1055,11 @@ static int launch_band(const float2* M, const void* bank, const float* kscale, c
 }
 
 int launch_socs_band(int G, const float2* M, const void* bank, const float* kscale, int bank_fmt,
-                     const float* scales, int nk, int Nx, int B, float2* C, hipStream_t st) {
+                     const float* scales, int nk, int Nx, int B, float2* C, bool lean,
+                     hipStream_t st) {
     const int W = 2 * Nx + 1;
     if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 2) return LITHO_EARG;
+
 #define LITHO_BAND(p, nz, slots)                                                                  \
     if (G == p * p && W <= nz * p) {                                                              \
         if (bank_fmt == 1)                                                                        \
@@ -1065,6 +1067,9 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float* ksca
         if (bank_fmt == 2)                                                                        \
             return launch_band<p, nz, slots, 2>(M, bank, kscale, scales, nk, Nx, B, C, st);       \
         return launch_band<p, nz, slots, 0>(M, bank, kscale, scales, nk, Nx, B, C, st);           \
+    }
+    if (lean) {
+        LITHO_BAND(12, 6, 24)
     }
     LITHO_BAND(6, 6, 36)
     LITHO_BAND(9, 4, 27)
@@ -1076,7 +1081,7 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float* ksca
 }
 
 int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, int count, bool h16,
-                      hipStream_t st) {
+                      bool lean, hipStream_t st) {
     if (nrows % L || nrows % 2 || count <= 0) return LITHO_EARG;
 #define LITHO_RT(p, q, nth)                                                                   \
     if (L == p * q) {                                                                         \
@@ -1092,13 +1097,16 @@ int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, i
     }
     LITHO_RT(16, 16, 256)
     LITHO_RT(16, 32, 256)
+    if (lean) {
+        LITHO_RT(32, 32, 128)
+    }
     LITHO_RT(32, 32, 256)
 #undef LITHO_RT
     return LITHO_EARG;
 }
 
 int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
-                       hipStream_t st) {
+                       bool lean, hipStream_t st) {
     if (B <= 0 || B > 65535) return LITHO_EARG;
 #define LITHO_MC(p, q, nth)                                                                    \
     if (N == p * q) {                                                                          \
@@ -1113,12 +1121,16 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
     }
     LITHO_MC(16, 16, 256)
     LITHO_MC(16, 32, 256)
+    if (lean) {
+        LITHO_MC(32, 32, 128)
+    }
     LITHO_MC(32, 32, 256)
 #undef LITHO_MC
     return LITHO_EARG;
 }
 
-int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, hipStream_t st) {
+int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
+                     hipStream_t st) {
     if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
 #define LITHO_FC(p, q, nz, nth)                                                                 \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                                 \
@@ -1135,6 +1147,9 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
     LITHO_FC(16, 16, 16, 256)
     LITHO_FC(16, 32, 3, 256)
     LITHO_FC(16, 32, 16, 256)
+    if (lean) {
+        LITHO_FC(32, 32, 5, 128)
+    }
     LITHO_FC(32, 32, 5, 256)
     LITHO_FC(32, 32, 32, 256)
 #undef LITHO_FC
@@ -1142,7 +1157,7 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
 }
 
 int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, bool h16,
-                        hipStream_t st) {
+                        bool lean, hipStream_t st) {
     if (nrows % N || nrows % 2 || 2 * band + 1 > N) return LITHO_EARG;
 #define LITHO_IT(p, q, nz, nth)                                                                \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                                \
@@ -1160,6 +1175,9 @@ int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, 
     LITHO_IT(16, 16, 16, 256)
     LITHO_IT(16, 32, 3, 256)
     LITHO_IT(16, 32, 16, 256)
+    if (lean) {
+        LITHO_IT(32, 32, 5, 128)
+    }
     LITHO_IT(32, 32, 5, 256)
     LITHO_IT(32, 32, 32, 256)
 #undef LITHO_IT

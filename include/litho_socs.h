@@ -160,9 +160,11 @@ int litho_last_timing(litho_ctx* ctx, float* total_ms, float* stage_ms, int nsta
 const char* litho_stage_name(litho_ctx* ctx, int i);
 
 /* Batched form (no counterpart in the reference): the batch is split into
- * `chunks` sub-batches that are software-pipelined over three streams (mask
- * spectrum | SOCS accumulate | Fourier interpolation), and the whole step is
- * captured once as a HIP graph (per shape and buffers) and replayed.
+ * `chunks` sub-batches spread over up to four streams (sub-batch k on stream
+ * k mod 4, its five launches in order) so that the compute-bound SOCS of one
+ * sub-batch shares CUs with the memory-bound FFTs of another (smaller blocks
+ * are used when chunks > 1), and the whole step is captured once as a HIP
+ * graph (per shape and buffers) and replayed.
  * `precision` is a set of storage flags; arithmetic stays FP32 in all of them:
  * INTER_FP16 stores the two HBM intermediates (row-DFT window, FI column
  * output) as fp16; BANK_BF16 / BANK_FP16 read the kernel bank as 16-bit pairs
