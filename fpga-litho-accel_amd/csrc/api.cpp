@@ -11,10 +11,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
+#include "dft_mfma.hpp"
 #include "eigen_kernels.hpp"
 #include "fft_lds.hpp"
 #include "socs_kernels.hpp"
@@ -60,6 +63,8 @@ struct litho_ctx {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<long> gkey;
+    // matrix-core DFT tables, keyed by (kind, N, a, b)
+    std::map<std::tuple<int, int, int, int>, DevBuf> tables;
 };
 
 struct litho_bank {
@@ -133,6 +138,8 @@ int litho_ctx_destroy(litho_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& b : c->ws)
         if (b.p) (void)hipFree(b.p);
+    for (auto& kv : c->tables)
+        if (kv.second.p) (void)hipFree(kv.second.p);
     if (c->tw) (void)hipFree(c->tw);
     for (int i = 0; i <= NSTAGE; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -816,6 +823,28 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     return LITHO_OK;
 }
 
+// fp16 DFT table of the matrix-core kernels: kind 0 mask rows (N, kmin, count),
+// kind 1 image rows (N, nb).  Built and uploaded on first use, outside any
+// stream capture.
+static int dft_table(litho_ctx* c, int kind, int N, int a, int b, const void** out) {
+    const auto key = std::make_tuple(kind, N, a, b);
+    auto it = c->tables.find(key);
+    if (it == c->tables.end()) {
+        const std::vector<unsigned short> t =
+            kind == 0 ? mask_rows_table(N, a, b) : image_rows_table(N, a);
+        DevBuf d;
+        d.bytes = t.size() * sizeof(unsigned short);
+        if (hipMalloc(&d.p, d.bytes) != hipSuccess) return LITHO_ENOMEM;
+        if (hipMemcpy(d.p, t.data(), d.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(d.p);
+            return LITHO_EHIP;
+        }
+        it = c->tables.emplace(key, d).first;
+    }
+    *out = it->second.p;
+    return LITHO_OK;
+}
+
 static int ensure_pipeline(litho_ctx* c, int nstreams) {
     while ((int)c->xs.size() < nstreams - 1) {
         hipStream_t s;
@@ -832,14 +861,20 @@ static int ensure_pipeline(litho_ctx* c, int nstreams) {
 // Enqueue one batched step: sub-batch k runs its five launches in order on
 // stream k % ns; different sub-batches overlap across streams.  Captured or
 // direct.
-static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
-                             int N, float* d_image, int prec, int chunks, int ns, char* R,
-                             size_t rstr, char* U, size_t ustr, float2* M, float2* C, int Gb) {
+static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const void* d_mask, int B,
+                             int N, void* d_image, int prec, int chunks, int ns, char* R,
+                             size_t rstr, char* U, size_t ustr, float2* M, float2* C, int Gb,
+                             const void* tmask, const void* timg) {
     const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
+    const bool mfma = (prec & LITHO_PREC_DFT_MFMA) != 0;
+    const bool u8 = (prec & LITHO_PREC_MASK_U8) != 0;
+    const bool out16 = (prec & LITHO_PREC_IMAGE_FP16) != 0;
     const bool lean = chunks > 1;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
     const int cb = (B + chunks - 1) / chunks;
     const size_t mstr = (size_t)W * W, cstr = (size_t)(2 * band + 1) * (band + 1);
+    const size_t NN = (size_t)N * N;
+    const size_t mbytes = u8 ? 1 : sizeof(float), ibytes = out16 ? 2 : sizeof(float);
     const int bfmt = (prec & LITHO_PREC_BANK_BF16) ? 1 : ((prec & LITHO_PREC_BANK_FP16) ? 2 : 0);
     const void* kb = bfmt == 1 ? (const void*)bank->d_bank16
                                : (bfmt == 2 ? (const void*)bank->d_bankh : (const void*)bank->d_bank);
@@ -854,16 +889,27 @@ static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const float* 
         hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
         void* Rk = R + (k % ns) * rstr;
         void* Uk = U + (k % ns) * ustr;
-        const float* mk = d_mask + (size_t)b0 * N * N;
-        float* ik = d_image + (size_t)b0 * N * N;
+        const void* mk = (const char*)d_mask + (size_t)b0 * NN * mbytes;
+        void* ik = (char*)d_image + (size_t)b0 * NN * ibytes;
         float2* Mk = M + (size_t)b0 * mstr;
         float2* Ck = C + (size_t)b0 * cstr;
-        LITHO_TRY(launch_rows_dft_t(N, mk, Rk, (long)nb * N, -Nx, W, mixed, lean, st));
+        if (mfma)
+            LITHO_TRY(launch_mask_rows_mfma(N, mk, u8, tmask, Rk, nb, W, mixed, st));
+        else
+            LITHO_TRY(launch_rows_dft_t(N, (const float*)mk, Rk, (long)nb * N, -Nx, W, mixed,
+                                        lean, st));
         LITHO_TRY(launch_mask_cols_t(N, Rk, Mk, nb, W, -Nx, W, mixed, lean, st));
         LITHO_TRY(launch_socs_band(Gb, Mk, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, nb, Ck, lean,
                                    st));
-        LITHO_TRY(launch_fi_cols_t(N, Ck, Uk, nb, band, mixed, lean, st));
-        LITHO_TRY(launch_image_rows_t(N, Uk, ik, (long)nb * N, band, mixed, lean, st));
+        if (mfma) {
+            // FP32 U: the image-side product splits it into hi + lo parts
+            LITHO_TRY(launch_fi_cols_t(N, Ck, Uk, nb, band, false, lean, st));
+            LITHO_TRY(launch_image_rows_mfma(N, (const float2*)Uk, band + 1, timg, ik, nb, out16,
+                                             st));
+        } else {
+            LITHO_TRY(launch_fi_cols_t(N, Ck, Uk, nb, band, mixed, lean, st));
+            LITHO_TRY(launch_image_rows_t(N, Uk, (float*)ik, (long)nb * N, band, mixed, lean, st));
+        }
     }
     for (int i = 1; i < ns; i++) {
         LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
@@ -872,12 +918,16 @@ static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const float* 
     return LITHO_OK;
 }
 
-int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const float* d_mask,
-                                     int B, int N, float* d_image, int precision, int chunks,
+int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const void* d_mask,
+                                     int B, int N, void* d_image, int precision, int chunks,
                                      int use_graph) {
+    const int known = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 |
+                      LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA;
+    const bool mfma = (precision & LITHO_PREC_DFT_MFMA) != 0;
     if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535 || chunks <= 0 ||
-        chunks > 64 || (precision & ~7) ||
-        ((precision & LITHO_PREC_BANK_BF16) && (precision & LITHO_PREC_BANK_FP16)))
+        chunks > 64 || (precision & ~known) ||
+        ((precision & LITHO_PREC_BANK_BF16) && (precision & LITHO_PREC_BANK_FP16)) ||
+        (!mfma && (precision & (LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16))))
         return LITHO_EARG;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
     const int Gb = band_grid(Nx);
@@ -887,8 +937,14 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
     const int ns = std::min(chunks, 4);  // GPU_MAX_HW_QUEUES is 4 by default
     const int cb = (B + chunks - 1) / chunks;
     const size_t es = mixed ? 4 : sizeof(float2);  // half2 or float2
-    const size_t rstr = (size_t)cb * W * N * es, ustr = (size_t)cb * (band + 1) * N * es;
+    const size_t ues = (mixed && !mfma) ? 4 : sizeof(float2);
+    const size_t rstr = (size_t)cb * W * N * es, ustr = (size_t)cb * (band + 1) * N * ues;
     void *R, *U, *M, *C;
+    const void *tmask = nullptr, *timg = nullptr;
+    if (mfma) {
+        LITHO_TRY(dft_table(c, 0, N, -Nx, W, &tmask));
+        LITHO_TRY(dft_table(c, 1, N, band + 1, 0, &timg));
+    }
     LITHO_TRY(ctx_ws(c, 0, ns * rstr, &R));
     LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
     LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
@@ -898,7 +954,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
     c->timed = false;
     if (!use_graph)
         return enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, ns, (char*)R,
-                                 rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb);
+                                 rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb, tmask, timg);
     std::vector<long> key = {(long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
                              (long)R, (long)U, (long)M, (long)C};
     if (!c->gexec || key != c->gkey) {
@@ -908,7 +964,8 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         c->graph = nullptr;
         LITHO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int st = enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, ns,
-                                   (char*)R, rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb);
+                                   (char*)R, rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb,
+                                   tmask, timg);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st != LITHO_OK) {

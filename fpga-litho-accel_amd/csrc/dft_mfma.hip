@@ -1,0 +1,402 @@
+// Band DFTs of the mask rows and of the image rows as fp16 matrix products on
+// the gfx950 matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulate).
+//
+// Only a narrow band of the N-point row spectrum is live on both ends of the
+// SOCS path (the 2Nx+1 window on the mask side, the 2·(2Nx)+1 intensity band on
+// the image side), so each of these transforms is a dense N x (band) product:
+//
+//   mask side   R[b][j][y] = sum_x mask[b][y][x] * s_k e^{-2 pi i k x / N},  k = kmin + j
+//               (s_k = (-1)^k folds the centring shift of FT_r2c_padded,
+//                litho_thread.cpp L255-331, into the table)
+//   image side  img[b][y][x] = sum_{kx=0..nb-1} w_kx Re(U[b][kx][y] e^{+2 pi i kx x / N})
+//               (w_0 = 1, w_kx = 2: the c2r of FI, litho_thread.cpp L692-772)
+//
+// Numerics.  The mask is exact in fp16 (u8 levels 0..255, or 0/1 floats); the
+// mask-side table is one fp16 rounding of cos/sin.  On the image side U is
+// split on the fly into hi + lo fp16 parts (lo scaled by 2^12 so it stays in
+// fp16's normal range); the table rows for the lo parts carry the 2^-12.
+//
+// Tiling.  A block is 4 waves; each wave owns 64 rows (4 row tiles of 16) so
+// that one 16x32 B fragment read from LDS feeds four MFMAs.  The B operand
+// (the cos/sin table) is pre-arranged on the host in fragment order: lane l
+// reads 16 contiguous bytes, so LDS reads are conflict-free and the global ->
+// LDS copy is a straight uint4 copy.  Within one k-step, lane l of the MFMA
+// holds the 8 k-values 8*(l/16) .. +7 of its A row (l%16) and B column (l%16);
+// the D fragment holds rows 4*(l/16) + i, column l%16.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "dft_mfma.hpp"
+
+namespace litho {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// two bytes of d (byte 2*hi, 2*hi+1) as exact fp16 values packed in a dword:
+// 0x6400 | b is the fp16 value 1024 + b, minus 1024 is exact.
+__device__ __forceinline__ unsigned u8pair_to_h2(unsigned d, int hi) {
+    const unsigned s = hi ? d >> 16 : d;
+    const unsigned bits = (s & 0xffu) | ((s & 0xff00u) << 8) | 0x64006400u;
+    h2v v = __builtin_bit_cast(h2v, bits);
+    v = v - (h2v){(_Float16)1024.f, (_Float16)1024.f};
+    return __builtin_bit_cast(unsigned, v);
+}
+
+__device__ __forceinline__ unsigned f2_to_h2(float a, float b) {
+    h2v v = {(_Float16)a, (_Float16)b};
+    return __builtin_bit_cast(unsigned, v);
+}
+
+constexpr int ROWS_PER_BLOCK = 256;  // 4 waves x 64 rows
+constexpr int RT = 4;                // row tiles per wave (image side)
+constexpr int KCH = 128;             // mask-side K (pixel) chunk staged in LDS
+constexpr int XCH = 64;              // image-side output columns per LDS chunk
+
+// ---------------------------------------------------------------------------
+// Mask rows -> transposed row-DFT window R [B][count][N] (same contract as the
+// FFT row kernel's transposed output).  NG = ceil(count / 16) frequency groups,
+// each as a real-part and an imaginary-part column tile.
+// ---------------------------------------------------------------------------
+template <int NG, int RTW, bool U8, bool H16>
+__global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_kernel(
+    const void* __restrict__ mask_, const u4v* __restrict__ Fpk, void* __restrict__ R_, int N,
+    int count, float oscale) {
+    constexpr int NT = 2 * NG;
+    constexpr int NTH = ROWS_PER_BLOCK * 4 / RTW;  // 64 threads per 16 rows x RTW
+    constexpr int RT = RTW;
+    constexpr int CH = 4 * NT * 64;  // uint4 per K chunk (4 k-steps)
+    constexpr int PER = CH / NTH;
+    static_assert(CH % NTH == 0, "chunk copy");
+    // double-buffered table chunk: chunk c+1 is fetched while chunk c is used
+    __shared__ u4v fs[2][CH];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int b = blockIdx.y;
+    const int ybase = blockIdx.x * ROWS_PER_BLOCK + w * 16 * RT;  // row within the image
+    const long rbase = (long)b * N + ybase;
+    f4 acc[RT][NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[rt][t] = (f4){0.f, 0.f, 0.f, 0.f};
+    // A fragments: 32 consecutive pixels of row (16 rt + r16) per lane and
+    // chunk; u8 pixels stay packed until their k-step and are prefetched.
+    auto load_u8 = [&](int c, u4v (&raw)[RT][2]) {
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+            const long row = rbase + 16 * rt + r16;
+            const u4v* p = (const u4v*)((const unsigned char*)mask_ + row * N + c * KCH + 32 * g);
+            raw[rt][0] = p[0];
+            raw[rt][1] = p[1];
+        }
+    };
+    u4v raw[RT][2], rawn[RT][2];
+    u4v pre[PER];
+    if (U8) load_u8(0, raw);
+#pragma unroll
+    for (int i = 0; i < PER; i++) fs[0][tid + NTH * i] = Fpk[tid + NTH * i];
+    __syncthreads();
+    const int nch = N / KCH;
+    for (int c = 0; c < nch; c++) {
+        const bool more = c + 1 < nch;
+        if (more) {
+            if (U8) load_u8(c + 1, rawn);
+#pragma unroll
+            for (int i = 0; i < PER; i++) pre[i] = Fpk[(long)(c + 1) * CH + tid + NTH * i];
+        }
+        u4v af[RT][4];
+        if (!U8) {
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                const long row = rbase + 16 * rt + r16;
+                const float4* p = (const float4*)((const float*)mask_ + row * N + c * KCH + 32 * g);
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const float4 x0 = p[2 * s], x1 = p[2 * s + 1];
+                    af[rt][s] = (u4v){f2_to_h2(x0.x, x0.y), f2_to_h2(x0.z, x0.w),
+                                      f2_to_h2(x1.x, x1.y), f2_to_h2(x1.z, x1.w)};
+                }
+            }
+        }
+        const u4v* fb = fs[c & 1];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            u4v as[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++)
+                as[rt] = U8 ? (u4v){u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1)], 0),
+                                    u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1)], 1),
+                                    u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1) + 1], 0),
+                                    u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1) + 1], 1)}
+                            : af[rt][s];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const h8 bf = __builtin_bit_cast(h8, fb[(s * NT + t) * 64 + lane]);
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++)
+                    acc[rt][t] = mfma16(__builtin_bit_cast(h8, as[rt]), bf, acc[rt][t]);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) fs[(c + 1) & 1][tid + NTH * i] = pre[i];
+            if (U8) {
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    raw[rt][0] = rawn[rt][0];
+                    raw[rt][1] = rawn[rt][1];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D fragment: rows 4g + i of the tile, column r16 = frequency 16 fg + r16
+#pragma unroll
+    for (int fg = 0; fg < NG; fg++) {
+        const int j = 16 * fg + r16;
+        if (j >= count) continue;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+            const int y = ybase + 16 * rt + 4 * g;
+            const f4 re = acc[rt][2 * fg] * oscale, im = acc[rt][2 * fg + 1] * oscale;
+            const long o = ((long)b * count + j) * N + y;
+            if (H16) {
+                uint4 v;
+                v.x = f2_to_h2(re[0], im[0]);
+                v.y = f2_to_h2(re[1], im[1]);
+                v.z = f2_to_h2(re[2], im[2]);
+                v.w = f2_to_h2(re[3], im[3]);
+                *(uint4*)((unsigned*)R_ + o) = v;
+            } else {
+                float4* q = (float4*)((float2*)R_ + o);
+                q[0] = make_float4(re[0], im[0], re[1], im[1]);
+                q[1] = make_float4(re[2], im[2], re[3], im[3]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// U [B][nb][N] (float2, FP32) -> image [B][N][N] (f32 or fp16).  NKS k-steps of
+// 32 cover 8 kx values each: (re, im) x (hi, lo) for two kx per lane group.
+// The product is formed transposed (table as the A operand, U as B) so that a
+// lane's D fragment is 4 consecutive pixels of one image row: one vector store.
+// ---------------------------------------------------------------------------
+template <int NKS, bool OUT16>
+__global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* __restrict__ U,
+                                                                 const u4v* __restrict__ Hpk,
+                                                                 void* __restrict__ img_, int N,
+                                                                 int nb) {
+    constexpr int CH = 4 * NKS * 64;  // uint4 per XCH-column chunk (4 column tiles)
+    constexpr int PER = CH / 256;
+    static_assert(CH % 256 == 0, "chunk copy");
+    __shared__ u4v fs[2][CH];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int b = blockIdx.y;
+    const int ybase = blockIdx.x * ROWS_PER_BLOCK + w * 64;
+    const float2* Ub = U + (long)b * nb * N;
+    u4v pre[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) pre[i] = Hpk[tid + 256 * i];
+    // U fragments for the whole K extent stay in registers
+    u4v a[RT][NKS];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+        const int y = ybase + 16 * rt + r16;
+#pragma unroll
+        for (int s = 0; s < NKS; s++) {
+            unsigned q[4];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int kx = 8 * s + 2 * g + e;
+                const float2 v = kx < nb ? Ub[(long)kx * N + y] : make_float2(0.f, 0.f);
+                const _Float16 hr = (_Float16)v.x, hi = (_Float16)v.y;
+                const float lr = (v.x - (float)hr) * 4096.f, li = (v.y - (float)hi) * 4096.f;
+                q[2 * e] = __builtin_bit_cast(unsigned, (h2v){hr, hi});
+                q[2 * e + 1] = f2_to_h2(lr, li);
+            }
+            a[rt][s] = (u4v){q[0], q[1], q[2], q[3]};
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) fs[0][tid + 256 * i] = pre[i];
+    __syncthreads();
+    const long ib = (long)b * N * N;
+    const int nch = N / XCH;
+    for (int c = 0; c < nch; c++) {
+        const bool more = c + 1 < nch;
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) pre[i] = Hpk[(long)(c + 1) * CH + tid + 256 * i];
+        }
+        const u4v* fb = fs[c & 1];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            f4 acc[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) acc[rt] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const h8 tf = __builtin_bit_cast(h8, fb[(t * NKS + s) * 64 + lane]);
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++)
+                    acc[rt] = mfma16(tf, __builtin_bit_cast(h8, a[rt][s]), acc[rt]);
+            }
+            // D[x][y]: x = 4g + i of the column tile, y = r16 of the row tile
+            const int x = c * XCH + 16 * t + 4 * g;
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                const long o = ib + (long)(ybase + 16 * rt + r16) * N + x;
+                if (OUT16) {
+                    uint2 v;
+                    v.x = f2_to_h2(acc[rt][0], acc[rt][1]);
+                    v.y = f2_to_h2(acc[rt][2], acc[rt][3]);
+                    *(uint2*)((_Float16*)img_ + o) = v;
+                } else {
+                    *(float4*)((float*)img_ + o) =
+                        make_float4(acc[rt][0], acc[rt][1], acc[rt][2], acc[rt][3]);
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) fs[(c + 1) & 1][tid + 256 * i] = pre[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host: tables in fragment order, launchers
+// ---------------------------------------------------------------------------
+static inline unsigned short f16_bits(double v) {
+    return __builtin_bit_cast(unsigned short, (_Float16)(float)v);
+}
+
+std::vector<unsigned short> mask_rows_table(int N, int kmin, int count) {
+    const int NG = (count + 15) / 16, NT = 2 * NG;
+    const int nch = N / KCH;
+    std::vector<unsigned short> t((size_t)nch * 4 * NT * 64 * 8);
+    for (int c = 0; c < nch; c++)
+        for (int s = 0; s < 4; s++)
+            for (int tt = 0; tt < NT; tt++)
+                for (int l = 0; l < 64; l++)
+                    for (int i = 0; i < 8; i++) {
+                        const int x = c * KCH + 32 * (l >> 4) + 8 * s + i;
+                        const int j = 16 * (tt >> 1) + (l & 15);
+                        double v = 0.0;
+                        if (j < count) {
+                            const long k = kmin + j;
+                            const long m = (((k * x) % N) + N) % N;  // exact phase index
+                            const double th = 2.0 * M_PI * (double)m / N;
+                            const double sg = (k & 1) ? -1.0 : 1.0;
+                            v = (tt & 1) ? -sg * std::sin(th) : sg * std::cos(th);
+                        }
+                        t[((((size_t)c * 4 + s) * NT + tt) * 64 + l) * 8 + i] = f16_bits(v);
+                    }
+    return t;
+}
+
+std::vector<unsigned short> image_rows_table(int N, int nb) {
+    const int NKS = (nb + 7) / 8;
+    const int nch = N / XCH;
+    std::vector<unsigned short> t((size_t)nch * 4 * NKS * 64 * 8);
+    for (int c = 0; c < nch; c++)
+        for (int tt = 0; tt < 4; tt++)
+            for (int s = 0; s < NKS; s++)
+                for (int l = 0; l < 64; l++)
+                    for (int i = 0; i < 8; i++) {
+                        const int x = c * XCH + 16 * tt + (l & 15);
+                        const int kx = 8 * s + 2 * (l >> 4) + (i >> 2);
+                        const int part = i & 3;
+                        double v = 0.0;
+                        if (kx < nb) {
+                            const double wk = kx ? 2.0 : 1.0;
+                            const long m = ((long)kx * x) % N;
+                            const double th = 2.0 * M_PI * (double)m / N;
+                            v = (part & 1) ? -wk * std::sin(th) : wk * std::cos(th);
+                            if (kx == 0 && (part & 1)) v = 0.0;
+                            if (part & 2) v /= 4096.0;
+                        }
+                        t[((((size_t)c * 4 + tt) * NKS + s) * 64 + l) * 8 + i] = f16_bits(v);
+                    }
+    return t;
+}
+
+int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, void* R, int B,
+                          int count, bool h16, hipStream_t st) {
+    if (N % ROWS_PER_BLOCK || N % KCH || B <= 0 || B > 65535 || count <= 0 || count > 80)
+        return LITHO_EARG;
+    const int NG = (count + 15) / 16;
+    const dim3 grid(N / ROWS_PER_BLOCK, B);
+    const float osc = u8 ? 1.f / 255.f : 1.f;
+    const u4v* F = (const u4v*)table;
+#define LITHO_MR(ng, rtw)                                                                      \
+    if (NG == ng) {                                                                            \
+        constexpr int nth = ROWS_PER_BLOCK * 4 / rtw;                                          \
+        if (u8 && h16)                                                                         \
+            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, true, true>), grid, dim3(nth), 0, \
+                               st, mask, F, R, N, count, osc);                                 \
+        else if (u8)                                                                           \
+            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, true, false>), grid, dim3(nth), 0,\
+                               st, mask, F, R, N, count, osc);                                 \
+        else if (h16)                                                                          \
+            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, false, true>), grid, dim3(nth), 0,\
+                               st, mask, F, R, N, count, osc);                                 \
+        else                                                                                   \
+            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, false, false>), grid, dim3(nth),  \
+                               0, st, mask, F, R, N, count, osc);                              \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
+    }
+    LITHO_MR(1, 4)
+    LITHO_MR(2, 4)
+    LITHO_MR(3, 4)
+    LITHO_MR(4, 2)
+    LITHO_MR(5, 2)
+#undef LITHO_MR
+    return LITHO_EARG;
+}
+
+int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, void* img, int B,
+                           bool out16, hipStream_t st) {
+    if (N % ROWS_PER_BLOCK || N % XCH || B <= 0 || B > 65535 || nb <= 0 || nb > 72)
+        return LITHO_EARG;
+    const int NKS = (nb + 7) / 8;
+    const dim3 grid(N / ROWS_PER_BLOCK, B);
+    const u4v* H = (const u4v*)table;
+#define LITHO_IR(nks)                                                                        \
+    if (NKS == nks) {                                                                        \
+        if (out16)                                                                           \
+            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, true>), grid, dim3(256), 0, st, U, \
+                               H, img, N, nb);                                               \
+        else                                                                                 \
+            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, false>), grid, dim3(256), 0, st,   \
+                               U, H, img, N, nb);                                            \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                      \
+    }
+    LITHO_IR(1)
+    LITHO_IR(2)
+    LITHO_IR(3)
+    LITHO_IR(4)
+    LITHO_IR(5)
+    LITHO_IR(6)
+    LITHO_IR(7)
+    LITHO_IR(8)
+    LITHO_IR(9)
+#undef LITHO_IR
+    return LITHO_EARG;
+}
+
+}  // namespace litho

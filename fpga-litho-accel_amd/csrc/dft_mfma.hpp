@@ -1,0 +1,25 @@
+// Band DFTs of mask rows and image rows on the fp16 matrix cores
+// (dft_mfma.hip).  Launchers return a LITHO_* status.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace litho {
+
+// fp16 cos/sin tables in MFMA fragment order (uploaded once per shape)
+std::vector<unsigned short> mask_rows_table(int N, int kmin, int count);
+std::vector<unsigned short> image_rows_table(int N, int nb);
+
+// mask [B][N][N] (u8 levels / 255, or f32) -> R [B][count][N] (half2 if h16, else
+// float2): window kx in [kmin, kmin + count) of each row's DFT, (-1)^kx applied.
+// N a multiple of 256, count <= 80.
+int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, void* R, int B,
+                          int count, bool h16, hipStream_t st);
+// U [B][nb][N] float2 (kx = 0..nb-1 of each image row) -> image [B][N][N]
+// (fp16 if out16, else f32), the c2r sum with the DC imaginary part dropped.
+int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, void* img, int B,
+                           bool out16, hipStream_t st);
+
+}  // namespace litho

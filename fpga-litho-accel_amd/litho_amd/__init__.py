@@ -39,7 +39,12 @@ LITHO_PREC_FP32 = 0
 LITHO_PREC_INTER_FP16 = 1
 LITHO_PREC_BANK_BF16 = 2
 LITHO_PREC_BANK_FP16 = 4
+LITHO_PREC_MASK_U8 = 8
+LITHO_PREC_IMAGE_FP16 = 16
+LITHO_PREC_DFT_MFMA = 32
 LITHO_PREC_MIXED = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16
+LITHO_PREC_COMPACT = (LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8 | LITHO_PREC_DFT_MFMA
+                      | LITHO_PREC_IMAGE_FP16)
 LITHO_PATH_AUTO = 0
 LITHO_PATH_POW2 = 1
 
@@ -322,8 +327,11 @@ class Context:
         _check(lib().litho_socs_image_batch(self.h, bank, d_mask, B, N, d_image),
                "litho_socs_image_batch")
 
-    def socs_image_batch_pipelined(self, bank, d_mask, B, N, d_image, precision=5, chunks=4,
+    def socs_image_batch_pipelined(self, bank, d_mask, B, N, d_image, precision=5, chunks=1,
                                    graph=True):
+        """Batched form (flags LITHO_PREC_*): d_mask holds f32 masks, or uint8
+        levels with LITHO_PREC_MASK_U8; d_image receives f32, or fp16 with
+        LITHO_PREC_IMAGE_FP16."""
         _check(lib().litho_socs_image_batch_pipelined(self.h, bank, d_mask, B, N, d_image,
                                                       precision, chunks, int(graph)),
                "litho_socs_image_batch_pipelined")

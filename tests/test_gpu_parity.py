@@ -309,6 +309,74 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
         ctx.close()
 
 
+@pytest.mark.parametrize("N,prec", [(1024, 32), (1024, 8 | 32 | 1 | 4), (1024, 8 | 32 | 4 | 16),
+                                    (512, 8 | 32 | 1 | 4), (256, 8 | 32)])
+def test_matrix_core_forms(gpu, orc, base_source, N, prec):
+    """Band DFTs on the fp16 matrix cores (u8 / f32 masks, f32 / fp16 images)
+    stay within 1e-3 of FP64 on binary and 8-bit grey masks; graph replays
+    reproduce the eager result bit for bit."""
+    case(f"matrix-core form N={N} precision={prec}")
+    src, s = base_source
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    K_n = 24 if N == 1024 else 16
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx,
+                                       K_n)
+    B = 3
+    u8 = bool(prec & gpu.LITHO_PREC_MASK_U8)
+    out16 = bool(prec & gpu.LITHO_PREC_IMAGE_FP16)
+    rng = np.random.default_rng(N + prec)
+    masks = np.stack([gpu.manhattan_mask(N, 70 + i) for i in range(B)])
+    if u8:
+        levels = (masks * 255).astype(np.uint8)
+        # last clip: random grey levels 0..255 (mask = level / 255)
+        levels[2] = rng.integers(0, 256, (N, N), dtype=np.uint8)
+        masks = levels.astype(np.float32) / np.float32(255.0)
+        host_in = levels
+    else:
+        host_in = masks
+    odt = np.float16 if out16 else np.float32
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm = ctx.alloc(host_in.nbytes)
+        di = ctx.alloc(B * N * N * 4)
+        ctx.h2d(dm, host_in)
+        ctx.socs_image_batch_pipelined(bank, dm, B, N, di, prec, 1, False)
+        ctx.sync()
+        eager = ctx.d2h(np.zeros((B, N, N), odt), di)
+        for _ in range(2):
+            ctx.socs_image_batch_pipelined(bank, dm, B, N, di, prec, 2, True)
+            ctx.sync()
+            assert np.array_equal(ctx.d2h(np.zeros((B, N, N), odt), di), eager)
+        for i in (0, 2):
+            ref = orc.calc_socs(orc.mask_spectrum(masks[i]), K.astype(np.complex128),
+                                sc.astype(np.float32), Nx, Nx, nthreads=8)
+            e = max_rel(eager[i].astype(np.float32), ref)
+            print(f"[matrix-core N={N} prec={prec} clip {i}] max rel err vs FP64 {e:.3e}",
+                  flush=True)
+            assert e < 1e-3
+        ctx.free(dm)
+        ctx.free(di)
+        gpu.lib().litho_bank_destroy(bank)
+    finally:
+        ctx.close()
+
+
+def test_matrix_core_flag_rules(gpu, c1):
+    """u8 masks and fp16 images need the matrix-core DFTs; unknown flags fail."""
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+        d = ctx.alloc(4 * 512 * 512)
+        for bad in (8, 16, 8 | 16, 64, 2 | 4 | 32):
+            with pytest.raises(gpu.LithoError):
+                ctx.socs_image_batch_pipelined(bank, d, 1, 512, d, bad, 1, False)
+        ctx.free(d)
+        gpu.lib().litho_bank_destroy(bank)
+    finally:
+        ctx.close()
+
+
 def test_tcc_direct_image_reference_fixture(gpu, orc, fixture_tcc):
     """GPU calcImage reproduces the reference's image_expected_512x512.bin."""
     case("tcc-direct image fixture")
