@@ -125,10 +125,20 @@ def main():
     sc, K, flags = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx, Nx, NK)
     setup_s = time.perf_counter() - t0
     log(f"kernels (TCC Gram route + FP64 Jacobi on GPU) {setup_s:.3f}s flags={flags}")
+    # precision form of the setup: split-BF16 Gram contraction, FP64 eigensolve
+    t0 = time.perf_counter()
+    sc16, K16, flags16 = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx,
+                                               Nx, NK, tcc_prec=L.LITHO_TCC_SPLIT_BF16)
+    setup16_s = time.perf_counter() - t0
+    eig_err = float(np.abs(sc16 - sc).max() / sc[0])
+    krn_err = float(max(np.abs(K16[k] - K[k]).max() / np.abs(K[k]).max() for k in range(NK)))
+    log(f"kernels (split-BF16 Gram contraction + FP64 Jacobi) {setup16_s:.3f}s; vs FP64 "
+        f"eigenvalues {eig_err:.2e}, kernels {krn_err:.2e}")
 
     ctx = L.Context()
     ctx.set_path(L.LITHO_PATH_POW2 if args.path == "pow2" else L.LITHO_PATH_AUTO)
     bank = ctx.bank(K, sc, Nx, Nx)
+    bank16 = ctx.bank(K16, sc16, Nx, Nx)  # chosen form: kernels of the split-BF16 route
     masks = np.stack([L.manhattan_mask(N, 1000 + i, WMIN, WMAX, FILL) for i in range(B)])
     d_mask = ctx.alloc(masks.nbytes)
     d_img = ctx.alloc(masks.nbytes)
@@ -170,13 +180,13 @@ def main():
 
     # ---- chosen form: 8-bit masks, fp16 band DFTs on the matrix cores, fp16
     # kernel bank, fp16 images; the step captured as one hipGraph -------------
-    def timed_form(precision, dm, graph=True):
+    def timed_form(precision, dm, bk=bank, graph=True):
         for _ in range(max(2, args.warmup)):
-            ctx.socs_image_batch_pipelined(bank, dm, B, N, d_img2, precision, args.chunks, graph)
+            ctx.socs_image_batch_pipelined(bk, dm, B, N, d_img2, precision, args.chunks, graph)
         ctx.sync()
         t = time.perf_counter()
         for _ in range(args.steps):
-            ctx.socs_image_batch_pipelined(bank, dm, B, N, d_img2, precision, args.chunks, graph)
+            ctx.socs_image_batch_pipelined(bk, dm, B, N, d_img2, precision, args.chunks, graph)
         ctx.sync()
         return B * args.steps / (time.perf_counter() - t)
 
@@ -187,10 +197,13 @@ def main():
     pipe_fp32 = timed_form(L.LITHO_PREC_FP32, d_mask)
     mixed = timed_form(L.LITHO_PREC_MIXED, d_mask)
     mixed_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img2)
-    chosen = timed_form(L.LITHO_PREC_COMPACT, d_mask8)
-    log(f"graph FP32 {pipe_fp32:.1f} images/s; fp16 storage {mixed:.1f}; chosen form (u8 masks, "
-        f"matrix-core DFTs, fp16 bank and images) {chosen:.1f} images/s = "
-        f"{chosen / images_per_s:.2f}x plain")
+    bf16_bank_prec = L.LITHO_PREC_COMPACT & ~L.LITHO_PREC_BANK_FP16 | L.LITHO_PREC_BANK_BF16
+    bf16_bank = timed_form(bf16_bank_prec, d_mask8, bank16)
+    bf16_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
+    chosen = timed_form(L.LITHO_PREC_COMPACT, d_mask8, bank16)
+    log(f"graph FP32 {pipe_fp32:.1f} images/s; fp16 storage {mixed:.1f}; bf16 bank {bf16_bank:.1f}; "
+        f"chosen form (split-BF16 kernels, u8 masks, matrix-core DFTs, fp16 bank and images) "
+        f"{chosen:.1f} images/s = {chosen / images_per_s:.2f}x plain")
     chosen_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
 
     # measured HBM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated against a
@@ -208,11 +221,12 @@ def main():
     max_err = None
     chosen_err = None
     mixed_err = None
+    bf16_err = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rate, n_done, dt, errs = cpu_baseline(masks[:8], K, sc, Nx, threads, args.cpu_budget,
-                                              [out_imgs, chosen_imgs, mixed_imgs])
-        max_err, chosen_err, mixed_err = errs
+                                              [out_imgs, chosen_imgs, mixed_imgs, bf16_imgs])
+        max_err, chosen_err, mixed_err, bf16_err = errs
         cpu = {"value": round(rate, 4), "unit": "images/s", "cores": threads, "kind": "port",
                "sample": f"{n_done} clip passes over 8 of the same 1024^2 clips, Module 3 + "
                          f"Module 7 (mask FFT, "
@@ -247,10 +261,11 @@ def main():
         "kernels": kernels,
         "max_rel_err_vs_fp64": max_err,
         "chosen_form": {
-            "form": "batched mixed precision: masks as 8-bit levels, mask-row and image-row band "
-                    "DFTs as fp16 matrix products with FP32 accumulate (image side split hi+lo), "
-                    "fp16 kernel bank with per-kernel FP32 scale, FP32 column FFTs and SOCS, "
-                    "fp16 images; the step captured as one hipGraph",
+            "form": "kernels from the split-BF16 Gram contraction (FP32 accumulate) and FP64 "
+                    "eigensolve; batched mixed precision: masks as 8-bit levels, mask-row and "
+                    "image-row band DFTs as fp16 matrix products with FP32 accumulate (image side "
+                    "split hi+lo), fp16 kernel bank with per-kernel FP32 scale, FP32 column FFTs "
+                    "and SOCS, fp16 images; the step captured as one hipGraph",
             "precision_flags": L.LITHO_PREC_COMPACT,
             "value": round(chosen, 2), "unit": "images/s",
             "ratio_vs_plain": round(chosen / images_per_s, 3),
@@ -260,12 +275,18 @@ def main():
                               "fp16 image write)",
             "graph_fp32_value": round(pipe_fp32, 2),
             "fp16_storage_value": round(mixed, 2),
-            "fp16_storage_max_rel_err_vs_fp64": mixed_err},
+            "fp16_storage_max_rel_err_vs_fp64": mixed_err,
+            "tcc_split_bf16": {"setup_s": round(setup16_s, 3), "fp64_setup_s": round(setup_s, 3),
+                               "eigenvalue_max_rel_err": eig_err,
+                               "kernel_max_rel_err": krn_err},
+            "bf16_bank_value": round(bf16_bank, 2),
+            "bf16_bank_max_rel_err_vs_fp64": bf16_err},
         "setup_s": round(setup_s, 3),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     L.lib().litho_bank_destroy(bank)
+    L.lib().litho_bank_destroy(bank16)
     ctx.free(d_mask)
     ctx.free(d_img)
     ctx.free(d_img2)

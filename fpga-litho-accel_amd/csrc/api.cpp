@@ -507,9 +507,11 @@ int jacobi_solve(litho_ctx* c, double2* H, double2* V, int n) {
 
 // From descending eigenvalues: how many vectors to canonicalise and the
 // clusters among them (same rule as the oracle: consecutive gaps <= rtol*|w0|).
+// rtol is 1e-10 after an FP64 contraction; after the split-BF16 contraction
+// the FP32 K-slice sums split symmetry-degenerate pairs by ~1e-7 relative, so
+// the rule widens to 1e-6 (still far below any non-symmetric gap met here).
 void plan_clusters(const std::vector<double>& wd, int nk, int* mtop, int* split,
-                   std::vector<int2>& clusters) {
-    const double rtol = 1e-10;
+                   std::vector<int2>& clusters, double rtol = 1e-10) {
     const int m = (int)wd.size();
     const double scale = std::fabs(wd[0]);
     *split = (nk < m && std::fabs(wd[nk - 1] - wd[nk]) <= rtol * scale) ? 1 : 0;
@@ -554,7 +556,8 @@ int finish_kernels(litho_ctx* c, double2* Vt, int n, int mtop, int nk, int Nx, i
 
 // Eigen of device H (n x n, upper triangle valid) -> top kernels.
 int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int nk,
-                            double* scales, float* kr, float* ki, int* flags) {
+                            double* scales, float* kr, float* ki, int* flags,
+                            double rtol = 1e-10) {
     hipStream_t st = c->stream;
     Dev V, w, order, Vt, wd;
     LITHO_TRY(V.alloc((size_t)n * n * sizeof(double2)));
@@ -571,7 +574,7 @@ int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int
     for (int r = 0; r < n; r++) wdesc[r] = hw[ho[r]];
     int mtop, split;
     std::vector<int2> clusters;
-    plan_clusters(wdesc, nk, &mtop, &split, clusters);
+    plan_clusters(wdesc, nk, &mtop, &split, clusters, rtol);
     LITHO_TRY(Vt.alloc((size_t)mtop * n * sizeof(double2)));
     LITHO_TRY(wd.alloc(mtop * sizeof(double)));
     LITHO_TRY(launch_gather_top(V.as<double2>(), w.as<double>(), order.as<int>(), n, mtop,
@@ -582,12 +585,22 @@ int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int
     return LITHO_OK;
 }
 
-}  // namespace
+// C = (X diag(sx)) Y^H in FP64 MFMA (prec LITHO_TCC_FP64) or split BF16 on the
+// matrix cores with FP32 accumulation (LITHO_TCC_SPLIT_BF16).
+static int contract_herm(litho_ctx* c, const double2* X, const double* sx, const double2* Y, int n,
+                         int m, int imag, int prec, double2* C) {
+    hipStream_t st = c->stream;
+    if (prec == LITHO_TCC_FP64) return launch_herm_gemm(X, sx, Y, n, m, imag, C, st);
+    const int nks = herm_gemm_bf16s_slices(n, m);
+    Dev part;
+    LITHO_TRY(part.alloc((size_t)nks * n * n * sizeof(float2)));
+    LITHO_TRY(launch_herm_gemm_bf16s(X, sx, Y, n, m, imag, part.as<float2>(), nks, C, st));
+    LITHO_HIP(hipStreamSynchronize(st));  // before the scratch is freed
+    return LITHO_OK;
+}
 
-extern "C" {
-
-int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize, double* tcc_re,
-                    double* tcc_im) {
+static int tcc_build_impl(const LithoOptics* op, const float* src, int srcSize, int prec,
+                          double* tcc_re, double* tcc_im) {
     if (!op || !src || !tcc_re || !tcc_im) return LITHO_EARG;
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
@@ -597,8 +610,8 @@ int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize, double
     const size_t nn = (size_t)P.n * P.n;
     Dev C;
     LITHO_TRY(C.alloc(nn * sizeof(double2)));
-    LITHO_TRY(launch_herm_gemm(P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), P.n, P.m,
-                               P.imag, C.as<double2>(), st));
+    LITHO_TRY(contract_herm(c, P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), P.n, P.m,
+                            P.imag, prec, C.as<double2>()));
     std::vector<double2> h(nn);
     LITHO_HIP(hipMemcpyAsync(h.data(), C.p, nn * sizeof(double2), hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
@@ -607,6 +620,38 @@ int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize, double
         tcc_im[i] = h[i].y;
     }
     return LITHO_OK;
+}
+
+static int kernels_from_source_impl(const LithoOptics* op, const float* src, int srcSize, int nk,
+                                    int prec, double* scales, float* krn_r, float* krn_i,
+                                    int* flags);
+
+}  // namespace
+
+extern "C" {
+
+int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize, double* tcc_re,
+                    double* tcc_im) {
+    return tcc_build_impl(op, src, srcSize, LITHO_TCC_FP64, tcc_re, tcc_im);
+}
+
+int litho_tcc_build_prec(const LithoOptics* op, const float* src, int srcSize, int tcc_prec,
+                         double* tcc_re, double* tcc_im) {
+    if (tcc_prec != LITHO_TCC_FP64 && tcc_prec != LITHO_TCC_SPLIT_BF16) return LITHO_EARG;
+    return tcc_build_impl(op, src, srcSize, tcc_prec, tcc_re, tcc_im);
+}
+
+int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSize, int nk,
+                              double* scales, float* krn_r, float* krn_i, int* flags) {
+    return kernels_from_source_impl(op, src, srcSize, nk, LITHO_TCC_FP64, scales, krn_r, krn_i,
+                                    flags);
+}
+
+int litho_kernels_from_source_prec(const LithoOptics* op, const float* src, int srcSize, int nk,
+                                   int tcc_prec, double* scales, float* krn_r, float* krn_i,
+                                   int* flags) {
+    if (tcc_prec != LITHO_TCC_FP64 && tcc_prec != LITHO_TCC_SPLIT_BF16) return LITHO_EARG;
+    return kernels_from_source_impl(op, src, srcSize, nk, tcc_prec, scales, krn_r, krn_i, flags);
 }
 
 int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx, int Ny, int nk,
@@ -626,8 +671,13 @@ int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx
     return kernels_from_device_tcc(c, H.as<double2>(), n, Nx, Ny, nk, scales, krn_r, krn_i, flags);
 }
 
-int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSize, int nk,
-                              double* scales, float* krn_r, float* krn_i, int* flags) {
+}  // extern "C"
+
+namespace {
+static int kernels_from_source_impl(const LithoOptics* op, const float* src, int srcSize, int nk,
+                                    int prec, double* scales, float* krn_r, float* krn_i,
+                                    int* flags) {
+    const double rtol = prec == LITHO_TCC_FP64 ? 1e-10 : 1e-6;
     if (!op || !src || !scales || !krn_r || !krn_i || nk <= 0) return LITHO_EARG;
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
@@ -639,10 +689,10 @@ int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSi
     if (n <= m) {
         Dev H;
         LITHO_TRY(H.alloc((size_t)n * n * sizeof(double2)));
-        LITHO_TRY(launch_herm_gemm(P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), n, m,
-                                   P.imag, H.as<double2>(), st));
+        LITHO_TRY(contract_herm(c, P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), n, m,
+                                P.imag, prec, H.as<double2>()));
         return kernels_from_device_tcc(c, H.as<double2>(), n, op->Nx, op->Ny, nk, scales, krn_r,
-                                       krn_i, flags);
+                                       krn_i, flags, rtol);
     }
     // Gram route: G = B^H B (m x m), B = A diag(sqrt S)
     Dev Bt, G, U, w, order, Vt, wd;
@@ -651,8 +701,8 @@ int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSi
     LITHO_TRY(U.alloc((size_t)m * m * sizeof(double2)));
     LITHO_TRY(launch_conj_transpose_scale(P.A.as<double2>(), P.S.as<double>(), n, m,
                                           Bt.as<double2>(), st));
-    LITHO_TRY(launch_herm_gemm(Bt.as<double2>(), nullptr, Bt.as<double2>(), m, n, P.imag,
-                               G.as<double2>(), st));
+    LITHO_TRY(contract_herm(c, Bt.as<double2>(), nullptr, Bt.as<double2>(), m, n, P.imag, prec,
+                            G.as<double2>()));
     LITHO_TRY(jacobi_solve(c, G.as<double2>(), U.as<double2>(), m));
     LITHO_TRY(w.alloc(m * sizeof(double)));
     LITHO_TRY(order.alloc(m * sizeof(int)));
@@ -666,7 +716,7 @@ int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSi
     for (int r = 0; r < m; r++) wdesc[r] = hw[ho[r]];
     int mtop, split;
     std::vector<int2> clusters;
-    plan_clusters(wdesc, nk, &mtop, &split, clusters);
+    plan_clusters(wdesc, nk, &mtop, &split, clusters, rtol);
     LITHO_TRY(Vt.alloc((size_t)mtop * n * sizeof(double2)));
     LITHO_TRY(wd.alloc(mtop * sizeof(double)));
     LITHO_TRY(launch_lift(P.A.as<double2>(), P.S.as<double>(), U.as<double2>(), w.as<double>(),
@@ -677,6 +727,9 @@ int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSi
     if (flags) *flags = (split ? LITHO_FLAG_CLUSTER_SPLIT : 0) | LITHO_FLAG_GRAM_ROUTE;
     return LITHO_OK;
 }
+}  // namespace
+
+extern "C" {
 
 // ---------------------------------------------------------------------------
 // Module 7

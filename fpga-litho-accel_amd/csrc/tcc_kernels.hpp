@@ -23,6 +23,11 @@ int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m,
 // C = (X diag(sx)) Y^H (sx may be null), Hermitian output, n x n
 int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
                      int has_imag, double2* C, hipStream_t st);
+// Same contraction in split BF16 on the matrix cores (tcc_bf16.hip): part is
+// an FP32 scratch of nks * n * n float2 (nks from herm_gemm_bf16s_slices).
+int herm_gemm_bf16s_slices(int n, int m);
+int launch_herm_gemm_bf16s(const double2* X, const double* sx, const double2* Y, int n, int m,
+                           int has_imag, float2* part, int nks, double2* C, hipStream_t st);
 
 // TCC-direct image band spectrum C [(4Ny+1)][(2Nx+1)] from the mask window M
 // [(2Ny+1)][(2Nx+1)] and the n x n TCC (calcImage)

@@ -34,7 +34,10 @@ EXPORTS = [
     "litho_image_tcc",
     "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_ctx_set_path",
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
+    "litho_tcc_build_prec", "litho_kernels_from_source_prec",
 ]
+LITHO_TCC_FP64 = 0
+LITHO_TCC_SPLIT_BF16 = 1
 LITHO_PREC_FP32 = 0
 LITHO_PREC_INTER_FP16 = 1
 LITHO_PREC_BANK_BF16 = 2
@@ -100,6 +103,9 @@ def lib():
                                          C.POINTER(i)]
         L.litho_kernels_from_source.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p,
                                                 _f32p, _f32p, C.POINTER(i)]
+        L.litho_tcc_build_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p, _f64p]
+        L.litho_kernels_from_source_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, i,
+                                                     _f64p, _f32p, _f32p, C.POINTER(i)]
         L.litho_socs_tmpimg.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                         i, i, i, i, i]
         L.litho_socs_image.argtypes = [_f32p, i, _f32p, _f32p, _f32p, i, i, i, _f32p]
@@ -186,15 +192,16 @@ def FT_r2c_padded(mask):
 
 
 # ---- Module 4 ---------------------------------------------------------------
-def calcTCC(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny):
-    """TCC = A diag(S) A^H (L441-532), complex128 n x n, built on the GPU."""
+def calcTCC(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny, tcc_prec=LITHO_TCC_FP64):
+    """TCC = A diag(S) A^H (L441-532), complex128 n x n, built on the GPU
+    (FP64 MFMA, or split BF16 with tcc_prec=LITHO_TCC_SPLIT_BF16)."""
     src = np.ascontiguousarray(src, np.float32)
     n = (2 * Nx + 1) * (2 * Ny + 1)
     re = np.zeros(n * n)
     im = np.zeros(n * n)
     op = optics(wavelength, NA, defocus, outerSigma, Lx, Ly, Nx, Ny)
-    _check(lib().litho_tcc_build(C.byref(op), src.ravel(), src.shape[0], re, im),
-           "litho_tcc_build")
+    _check(lib().litho_tcc_build_prec(C.byref(op), src.ravel(), src.shape[0], tcc_prec, re, im),
+           "litho_tcc_build_prec")
     return (re + 1j * im).reshape(n, n)
 
 
@@ -213,8 +220,10 @@ def calcKernels(tcc, nk, Nx, Ny):
     return sc, (kr + 1j * ki).astype(np.complex64).reshape(nk, n), fl.value
 
 
-def kernels_from_source(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny, nk):
-    """Source -> top-nk kernels on the GPU (explicit or Gram route)."""
+def kernels_from_source(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny, nk,
+                        tcc_prec=LITHO_TCC_FP64):
+    """Source -> top-nk kernels on the GPU (explicit or Gram route); the
+    contraction in FP64 or split BF16 (tcc_prec), the eigensolve in FP64."""
     src = np.ascontiguousarray(src, np.float32)
     n = (2 * Nx + 1) * (2 * Ny + 1)
     sc = np.zeros(nk)
@@ -222,8 +231,9 @@ def kernels_from_source(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny
     ki = np.zeros(nk * n, np.float32)
     fl = C.c_int(0)
     op = optics(wavelength, NA, defocus, outerSigma, Lx, Ly, Nx, Ny)
-    _check(lib().litho_kernels_from_source(C.byref(op), src.ravel(), src.shape[0], nk, sc, kr,
-                                           ki, C.byref(fl)), "litho_kernels_from_source")
+    _check(lib().litho_kernels_from_source_prec(C.byref(op), src.ravel(), src.shape[0], nk,
+                                                tcc_prec, sc, kr, ki, C.byref(fl)),
+           "litho_kernels_from_source_prec")
     return sc, (kr + 1j * ki).astype(np.complex64).reshape(nk, n), fl.value
 
 

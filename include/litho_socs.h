@@ -113,6 +113,19 @@ int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx
 int litho_kernels_from_source(const LithoOptics* op, const float* src, int srcSize, int nk,
                               double* scales, float* krn_r, float* krn_i, int* flags);
 
+/* Precision form of the two calls above (no counterpart in the reference):
+ * the TCC / Gram contraction in split BF16 on the matrix cores (every operand
+ * as bf16 hi + lo, three products per real product, FP32 accumulate, FP64
+ * sum over K slices); the eigensolve, the Gram eigenproblem and the lift stay
+ * FP64.  tcc_prec LITHO_TCC_FP64 gives the plain results. */
+#define LITHO_TCC_FP64 0
+#define LITHO_TCC_SPLIT_BF16 1
+int litho_tcc_build_prec(const LithoOptics* op, const float* src, int srcSize, int tcc_prec,
+                         double* tcc_re, double* tcc_im);
+int litho_kernels_from_source_prec(const LithoOptics* op, const float* src, int srcSize, int nk,
+                                   int tcc_prec, double* scales, float* krn_r, float* krn_i,
+                                   int* flags);
+
 /* ---- Module 7: calcSOCS, litho_thread.cpp L813-1040 ------------------------ */
 /* HLS-v18-shaped entry (socs_2048_stream_refactored_v18_hls_fft.cpp L287-388):
  * full Lx x Ly spectrum planes, kernels nk x (2ny+1)(2nx+1) concatenated,

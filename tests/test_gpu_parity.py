@@ -377,6 +377,49 @@ def test_matrix_core_flag_rules(gpu, c1):
         ctx.close()
 
 
+@pytest.mark.parametrize("defocus", [0.0, 0.2])
+def test_split_bf16_tcc(gpu, base_source, defocus):
+    """Split-BF16 matrix-core TCC (explicit route, n <= n_src) against the FP64
+    MFMA TCC: real pupil at zero defocus, complex pupil with defocus."""
+    case(f"split-bf16 TCC defocus={defocus}")
+    src, s = base_source
+    N = 256
+    Nx = gpu.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    args = (src, s, BASE_LAMBDA, BASE_NA, defocus, 4 * N, 4 * N, Nx, Nx)
+    t64 = gpu.calcTCC(*args)
+    t16 = gpu.calcTCC(*args, tcc_prec=gpu.LITHO_TCC_SPLIT_BF16)
+    e = np.abs(t16 - t64).max() / np.abs(t64).max()
+    print(f"[split-bf16 TCC n={t64.shape[0]} defocus={defocus}] max err / max|TCC| {e:.3e}",
+          flush=True)
+    assert np.allclose(t16, t16.conj().T)
+    assert e < 5e-5
+
+
+def test_split_bf16_kernels_and_image(gpu, orc, base_source):
+    """C2 precision form: split-BF16 Gram contraction + FP64 eigensolve + FP64
+    lift; eigenvalues, kernels and the FP32 SOCS image against FP64."""
+    case("split-bf16 Gram route C2")
+    src, s = base_source
+    N, nk = 1024, 24
+    Nx = gpu.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    args = (src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, nk)
+    sc64, K64, f64 = gpu.kernels_from_source(*args)
+    sc16, K16, f16 = gpu.kernels_from_source(*args, tcc_prec=gpu.LITHO_TCC_SPLIT_BF16)
+    assert f64 == f16
+    eig = np.abs(sc16 - sc64).max() / sc64[0]
+    kerr = max(np.abs(K16[k] - K64[k]).max() / np.abs(K64[k]).max() for k in range(nk))
+    m = gpu.manhattan_mask(N, 77)
+    img = gpu.socs_image(m, K16, sc16, Nx, Nx)
+    ref = orc.calc_socs(orc.mask_spectrum(m), K64.astype(np.complex128), sc64.astype(np.float32),
+                        Nx, Nx, nthreads=8)
+    ierr = max_rel(img, ref)
+    print(f"[split-bf16 C2] eigenvalues {eig:.3e} kernels {kerr:.3e} image {ierr:.3e} (vs FP64)",
+          flush=True)
+    assert eig < 1e-4
+    assert kerr < 1e-3  # canonical basis in the mirror-degenerate pairs as well
+    assert ierr < 1e-4
+
+
 def test_tcc_direct_image_reference_fixture(gpu, orc, fixture_tcc):
     """GPU calcImage reproduces the reference's image_expected_512x512.bin."""
     case("tcc-direct image fixture")
