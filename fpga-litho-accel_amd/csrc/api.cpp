@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,6 +66,13 @@ struct litho_ctx {
     std::vector<long> gkey;
     // matrix-core DFT tables, keyed by (kind, N, a, b)
     std::map<std::tuple<int, int, int, int>, DevBuf> tables;
+    // band-pipeline plans chosen by litho_autotune, keyed by (N, Nx, nk, precision)
+    std::map<std::tuple<int, int, int, int>, std::array<int, 3>> plans;
+    // mixed-size batches: four workspaces per bucket, per-bucket end events
+    std::vector<DevBuf> bws;
+    std::vector<hipEvent_t> bev;
+    hipEvent_t bev0 = nullptr;
+    int nbev = 0;
 };
 
 struct litho_bank {
@@ -140,6 +148,10 @@ int litho_ctx_destroy(litho_ctx* c) {
         if (b.p) (void)hipFree(b.p);
     for (auto& kv : c->tables)
         if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& b : c->bws)
+        if (b.p) (void)hipFree(b.p);
+    for (auto e : c->bev) (void)hipEventDestroy(e);
+    if (c->bev0) (void)hipEventDestroy(c->bev0);
     if (c->tw) (void)hipFree(c->tw);
     for (int i = 0; i <= NSTAGE; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -817,7 +829,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     const int Nx = bank->Nx;
     const int W = 2 * Nx + 1;
     const int band = 2 * Nx;
-    if (!is_pow2(N) || N < 32 || N > 4096 || W > N) return LITHO_EARG;
+    if ((!is_pow2(N) && N != 768) || N < 32 || N > 4096 || W > N) return LITHO_EARG;
     hipStream_t st = c->stream;
     const int Gb = band_grid(Nx);
     if (c->path == LITHO_PATH_AUTO && Gb && Gb <= N && N >= 256 && N <= 1024) {
@@ -847,7 +859,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         return LITHO_OK;
     }
     const int G = next_pow2(4 * Nx + 1);
-    if (G > N || G > 1024) return LITHO_EARG;
+    if (!is_pow2(N) || G > N || G > 1024) return LITHO_EARG;
     void *R, *M, *T, *P, *U;
     LITHO_TRY(ctx_ws(c, 0, (size_t)B * N * W * sizeof(float2), &R));
     LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
@@ -911,114 +923,139 @@ static int ensure_pipeline(litho_ctx* c, int nstreams) {
     return LITHO_OK;
 }
 
-// Enqueue one batched step: sub-batch k runs its five launches in order on
-// stream k % ns; different sub-batches overlap across streams.  Captured or
-// direct.
-static int enqueue_pipelined(litho_ctx* c, const litho_bank* bank, const void* d_mask, int B,
-                             int N, void* d_image, int prec, int chunks, int ns, char* R,
-                             size_t rstr, char* U, size_t ustr, float2* M, float2* C, int Gb,
-                             const void* tmask, const void* timg) {
+// ---------------------------------------------------------------------------
+// Band pipeline of one batch of same-size clips under a plan: the conv grid
+// G (36 = 6^2, 81 = 9^2 or 144 = 12^2: the radix of the SOCS FFTs), the SOCS
+// block shape (lean: 24 kernel slots per block instead of 48) and the block
+// shape of the FFT row / column kernels (lean: 128 threads).
+// ---------------------------------------------------------------------------
+struct BandPlan {
+    int G = 0;
+    bool lean_socs = false;
+    bool lean_mem = false;
+};
+
+static BandPlan plan_of(litho_ctx* c, int N, int Nx, int nk, int prec) {
+    BandPlan p;
+    auto it = c->plans.find(std::make_tuple(N, Nx, nk, prec));
+    if (it != c->plans.end()) {
+        p.G = it->second[0];
+        p.lean_socs = it->second[1] != 0;
+        p.lean_mem = it->second[2] != 0;
+    } else {
+        p.G = band_grid(Nx);
+    }
+    return p;
+}
+
+static bool prec_ok(int prec) {
+    const int known = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 |
+                      LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA;
+    if (prec & ~known) return false;
+    if ((prec & LITHO_PREC_BANK_BF16) && (prec & LITHO_PREC_BANK_FP16)) return false;
+    if (!(prec & LITHO_PREC_DFT_MFMA) && (prec & (LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16)))
+        return false;
+    return true;
+}
+
+static bool band_shape_ok(int N, int Nx) {
+    const int Gb = band_grid(Nx);
+    return (N == 256 || N == 512 || N == 768 || N == 1024) && Gb && Gb <= N;
+}
+
+static int grow(DevBuf& b, size_t bytes, void** out) {
+    if (b.bytes < bytes) {
+        if (b.p) LITHO_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+        if (hipMalloc(&b.p, bytes ? bytes : 1) != hipSuccess) {
+            b.p = nullptr;
+            return LITHO_ENOMEM;
+        }
+        b.bytes = bytes;
+    }
+    *out = b.p;
+    return LITHO_OK;
+}
+
+struct BandWs {
+    void *R = nullptr, *U = nullptr, *M = nullptr, *C = nullptr;
+};
+
+// four workspaces (row window R, mask window M, intensity band C, FI column
+// output U) for B clips in the given slots
+static int band_ws(DevBuf* slots, int B, int N, int Nx, int prec, BandWs* w) {
+    const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
+    const bool mfma = (prec & LITHO_PREC_DFT_MFMA) != 0;
+    const size_t W = 2 * Nx + 1, band = 2 * Nx;
+    const size_t es = mixed ? 4 : sizeof(float2), ues = (mixed && !mfma) ? 4 : sizeof(float2);
+    LITHO_TRY(grow(slots[0], (size_t)B * W * N * es, &w->R));
+    LITHO_TRY(grow(slots[1], (size_t)B * W * W * sizeof(float2), &w->M));
+    LITHO_TRY(grow(slots[2], (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &w->C));
+    LITHO_TRY(grow(slots[3], (size_t)B * (band + 1) * N * ues, &w->U));
+    return LITHO_OK;
+}
+
+// matrix-core tables of one clip shape (uploaded outside any capture)
+static int mfma_tables(litho_ctx* c, int N, int Nx, const void** tm, const void** ti) {
+    LITHO_TRY(dft_table(c, 0, N, -Nx, 2 * Nx + 1, tm));
+    return dft_table(c, 1, N, 2 * Nx + 1, 0, ti);
+}
+
+// the five launches of one same-size batch on stream st
+static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask, int B, int N,
+                        void* d_image, int prec, const BandPlan& pl, const BandWs& w,
+                        hipStream_t st) {
     const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
     const bool mfma = (prec & LITHO_PREC_DFT_MFMA) != 0;
     const bool u8 = (prec & LITHO_PREC_MASK_U8) != 0;
     const bool out16 = (prec & LITHO_PREC_IMAGE_FP16) != 0;
-    const bool lean = chunks > 1;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
-    const int cb = (B + chunks - 1) / chunks;
-    const size_t mstr = (size_t)W * W, cstr = (size_t)(2 * band + 1) * (band + 1);
-    const size_t NN = (size_t)N * N;
-    const size_t mbytes = u8 ? 1 : sizeof(float), ibytes = out16 ? 2 : sizeof(float);
     const int bfmt = (prec & LITHO_PREC_BANK_BF16) ? 1 : ((prec & LITHO_PREC_BANK_FP16) ? 2 : 0);
     const void* kb = bfmt == 1 ? (const void*)bank->d_bank16
                                : (bfmt == 2 ? (const void*)bank->d_bankh : (const void*)bank->d_bank);
     const float* ks = bfmt ? bank->d_kscale : nullptr;
-    hipStream_t s0 = c->stream;
-    LITHO_HIP(hipEventRecord(c->evStart, s0));
-    for (int i = 1; i < ns; i++) LITHO_HIP(hipStreamWaitEvent(c->xs[i - 1], c->evStart, 0));
-    for (int k = 0; k < chunks; k++) {
-        const int b0 = k * cb;
-        const int nb = std::min(cb, B - b0);
-        if (nb <= 0) break;
-        hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
-        void* Rk = R + (k % ns) * rstr;
-        void* Uk = U + (k % ns) * ustr;
-        const void* mk = (const char*)d_mask + (size_t)b0 * NN * mbytes;
-        void* ik = (char*)d_image + (size_t)b0 * NN * ibytes;
-        float2* Mk = M + (size_t)b0 * mstr;
-        float2* Ck = C + (size_t)b0 * cstr;
-        if (mfma)
-            LITHO_TRY(launch_mask_rows_mfma(N, mk, u8, tmask, Rk, nb, W, mixed, st));
-        else
-            LITHO_TRY(launch_rows_dft_t(N, (const float*)mk, Rk, (long)nb * N, -Nx, W, mixed,
-                                        lean, st));
-        LITHO_TRY(launch_mask_cols_t(N, Rk, Mk, nb, W, -Nx, W, mixed, lean, st));
-        LITHO_TRY(launch_socs_band(Gb, Mk, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, nb, Ck, lean,
-                                   st));
-        if (mfma) {
-            // FP32 U: the image-side product splits it into hi + lo parts
-            LITHO_TRY(launch_fi_cols_t(N, Ck, Uk, nb, band, false, lean, st));
-            LITHO_TRY(launch_image_rows_mfma(N, (const float2*)Uk, band + 1, timg, ik, nb, out16,
-                                             st));
-        } else {
-            LITHO_TRY(launch_fi_cols_t(N, Ck, Uk, nb, band, mixed, lean, st));
-            LITHO_TRY(launch_image_rows_t(N, Uk, (float*)ik, (long)nb * N, band, mixed, lean, st));
-        }
+    const void *tm = nullptr, *ti = nullptr;
+    if (mfma) {
+        auto a = c->tables.find(std::make_tuple(0, N, -Nx, W));
+        auto b = c->tables.find(std::make_tuple(1, N, band + 1, 0));
+        if (a == c->tables.end() || b == c->tables.end()) return LITHO_EARG;
+        tm = a->second.p;
+        ti = b->second.p;
     }
-    for (int i = 1; i < ns; i++) {
-        LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
-        LITHO_HIP(hipStreamWaitEvent(s0, c->xe[i - 1], 0));
+    if (mfma)
+        LITHO_TRY(launch_mask_rows_mfma(N, d_mask, u8, tm, w.R, B, W, mixed, st));
+    else
+        LITHO_TRY(launch_rows_dft_t(N, (const float*)d_mask, w.R, (long)B * N, -Nx, W, mixed,
+                                    pl.lean_mem, st));
+    LITHO_TRY(launch_mask_cols_t(N, w.R, (float2*)w.M, B, W, -Nx, W, mixed, pl.lean_mem, st));
+    LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, B,
+                               (float2*)w.C, pl.lean_socs, st));
+    if (mfma) {
+        // FP32 U: the image-side product splits it into hi + lo parts
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st));
+        LITHO_TRY(launch_image_rows_mfma(N, (const float2*)w.U, band + 1, ti, d_image, B, out16,
+                                         st));
+    } else {
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st));
+        LITHO_TRY(launch_image_rows_t(N, w.U, (float*)d_image, (long)B * N, band, mixed,
+                                      pl.lean_mem, st));
     }
     return LITHO_OK;
 }
 
-int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const void* d_mask,
-                                     int B, int N, void* d_image, int precision, int chunks,
-                                     int use_graph) {
-    const int known = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 |
-                      LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA;
-    const bool mfma = (precision & LITHO_PREC_DFT_MFMA) != 0;
-    if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535 || chunks <= 0 ||
-        chunks > 64 || (precision & ~known) ||
-        ((precision & LITHO_PREC_BANK_BF16) && (precision & LITHO_PREC_BANK_FP16)) ||
-        (!mfma && (precision & (LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16))))
-        return LITHO_EARG;
-    const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
-    const int Gb = band_grid(Nx);
-    if (!is_pow2(N) || N < 256 || N > 1024 || !Gb || Gb > N) return LITHO_EARG;
-    const bool mixed = (precision & LITHO_PREC_INTER_FP16) != 0;
-    chunks = std::min(chunks, B);
-    const int ns = std::min(chunks, 4);  // GPU_MAX_HW_QUEUES is 4 by default
-    const int cb = (B + chunks - 1) / chunks;
-    const size_t es = mixed ? 4 : sizeof(float2);  // half2 or float2
-    const size_t ues = (mixed && !mfma) ? 4 : sizeof(float2);
-    const size_t rstr = (size_t)cb * W * N * es, ustr = (size_t)cb * (band + 1) * N * ues;
-    void *R, *U, *M, *C;
-    const void *tmask = nullptr, *timg = nullptr;
-    if (mfma) {
-        LITHO_TRY(dft_table(c, 0, N, -Nx, W, &tmask));
-        LITHO_TRY(dft_table(c, 1, N, band + 1, 0, &timg));
-    }
-    LITHO_TRY(ctx_ws(c, 0, ns * rstr, &R));
-    LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
-    LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
-    LITHO_TRY(ctx_ws(c, 4, ns * ustr, &U));
-    LITHO_TRY(ensure_pipeline(c, ns));
-    c->nstage = 0;
-    c->timed = false;
-    if (!use_graph)
-        return enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, ns, (char*)R,
-                                 rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb, tmask, timg);
-    std::vector<long> key = {(long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
-                             (long)R, (long)U, (long)M, (long)C};
+extern "C++" {
+// Capture `enqueue` on the context stream once per key and replay it.
+template <class F>
+static int run_graph(litho_ctx* c, const std::vector<long>& key, F&& enqueue) {
     if (!c->gexec || key != c->gkey) {
         if (c->gexec) LITHO_HIP(hipGraphExecDestroy(c->gexec));
         if (c->graph) LITHO_HIP(hipGraphDestroy(c->graph));
         c->gexec = nullptr;
         c->graph = nullptr;
+        c->gkey.clear();
         LITHO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_pipelined(c, bank, d_mask, B, N, d_image, precision, chunks, ns,
-                                   (char*)R, rstr, (char*)U, ustr, (float2*)M, (float2*)C, Gb,
-                                   tmask, timg);
+        int st = enqueue();
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st != LITHO_OK) {
@@ -1032,6 +1069,251 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
     }
     LITHO_HIP(hipGraphLaunch(c->gexec, c->stream));
     return LITHO_OK;
+}
+}  // extern "C++"
+
+int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const void* d_mask,
+                                     int B, int N, void* d_image, int precision, int chunks,
+                                     int use_graph) {
+    if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535 || chunks <= 0 ||
+        chunks > 64 || !prec_ok(precision))
+        return LITHO_EARG;
+    const int Nx = bank->Nx;
+    if (!band_shape_ok(N, Nx)) return LITHO_EARG;
+    const bool mfma = (precision & LITHO_PREC_DFT_MFMA) != 0;
+    chunks = std::min(chunks, B);
+    const int ns = std::min(chunks, 4);  // GPU_MAX_HW_QUEUES is 4 by default
+    const int cb = (B + chunks - 1) / chunks;
+    if (mfma) {
+        const void *tm, *ti;
+        LITHO_TRY(mfma_tables(c, N, Nx, &tm, &ti));
+    }
+    // sub-batch k uses the R / U workspaces of stream k % ns; M and C hold
+    // the whole batch
+    BandWs w[4];
+    {
+        const bool mixed = (precision & LITHO_PREC_INTER_FP16) != 0;
+        const size_t W = 2 * Nx + 1, band = 2 * Nx;
+        const size_t es = mixed ? 4 : sizeof(float2);
+        const size_t ues = (mixed && !mfma) ? 4 : sizeof(float2);
+        const size_t rstr = (size_t)cb * W * N * es, ustr = (size_t)cb * (band + 1) * N * ues;
+        void *R, *M, *C, *U;
+        LITHO_TRY(ctx_ws(c, 0, ns * rstr, &R));
+        LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
+        LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
+        LITHO_TRY(ctx_ws(c, 4, ns * ustr, &U));
+        for (int k = 0; k < ns; k++) {
+            w[k].R = (char*)R + k * rstr;
+            w[k].U = (char*)U + k * ustr;
+            w[k].M = M;
+            w[k].C = C;
+        }
+    }
+    LITHO_TRY(ensure_pipeline(c, ns));
+    c->nstage = 0;
+    c->timed = false;
+    const BandPlan tuned = plan_of(c, N, Nx, bank->nk, precision);
+    auto enqueue = [&]() -> int {
+        const size_t NN = (size_t)N * N;
+        const size_t mb = (precision & LITHO_PREC_MASK_U8) ? 1 : sizeof(float);
+        const size_t ib = (precision & LITHO_PREC_IMAGE_FP16) ? 2 : sizeof(float);
+        const size_t W = 2 * Nx + 1, band = 2 * Nx;
+        hipStream_t s0 = c->stream;
+        LITHO_HIP(hipEventRecord(c->evStart, s0));
+        for (int i = 1; i < ns; i++) LITHO_HIP(hipStreamWaitEvent(c->xs[i - 1], c->evStart, 0));
+        for (int k = 0; k < chunks; k++) {
+            const int b0 = k * cb;
+            const int nb = std::min(cb, B - b0);
+            if (nb <= 0) break;
+            hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
+            BandWs wk = w[k % ns];
+            wk.M = (float2*)w[0].M + (size_t)b0 * W * W;
+            wk.C = (float2*)w[0].C + (size_t)b0 * (2 * band + 1) * (band + 1);
+            BandPlan pl = tuned;
+            if (chunks > 1) pl.lean_socs = pl.lean_mem = true;
+            LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN * mb, nb, N,
+                                   (char*)d_image + (size_t)b0 * NN * ib, precision, pl, wk, st));
+        }
+        for (int i = 1; i < ns; i++) {
+            LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
+            LITHO_HIP(hipStreamWaitEvent(s0, c->xe[i - 1], 0));
+        }
+        return LITHO_OK;
+    };
+    if (!use_graph) return enqueue();
+    std::vector<long> key = {1, (long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
+                             (long)w[0].R, (long)w[0].U, (long)w[0].M, (long)w[0].C, tuned.G,
+                             tuned.lean_socs, tuned.lean_mem};
+    return run_graph(c, key, enqueue);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed clip sizes: one bucket per size (its own kernel bank), every bucket's
+// five launches on one of up to four streams, the whole set as one graph.
+// ---------------------------------------------------------------------------
+static int check_buckets(const LithoBucket* bk, int nb, int prec, int* active) {
+    if (!bk || nb <= 0 || nb > 64 || !prec_ok(prec)) return LITHO_EARG;
+    *active = 0;
+    for (int i = 0; i < nb; i++) {
+        const LithoBucket& b = bk[i];
+        if (b.count < 0 || b.count > 65535) return LITHO_EARG;
+        if (b.count == 0) continue;
+        if (!b.bank || !b.d_masks || !b.d_images || !band_shape_ok(b.N, b.bank->Nx))
+            return LITHO_EARG;
+        (*active)++;
+    }
+    return LITHO_OK;
+}
+
+static int prepare_buckets(litho_ctx* c, const LithoBucket* bk, int nb, int prec, int sample,
+                           std::vector<BandWs>& w) {
+    if ((int)c->bws.size() < 4 * nb) c->bws.resize(4 * nb);
+    w.assign(nb, BandWs());
+    for (int i = 0; i < nb; i++) {
+        const LithoBucket& b = bk[i];
+        if (b.count == 0) continue;
+        const int B = sample > 0 ? std::min(sample, b.count) : b.count;
+        LITHO_TRY(band_ws(&c->bws[4 * i], B, b.N, b.bank->Nx, prec, &w[i]));
+        if (prec & LITHO_PREC_DFT_MFMA) {
+            const void *tm, *ti;
+            LITHO_TRY(mfma_tables(c, b.N, b.bank->Nx, &tm, &ti));
+        }
+    }
+    return LITHO_OK;
+}
+
+int litho_socs_image_buckets(litho_ctx* c, const LithoBucket* bk, int nb, int precision,
+                             int use_graph) {
+    int active = 0;
+    if (!c) return LITHO_EARG;
+    LITHO_TRY(check_buckets(bk, nb, precision, &active));
+    if (!active) return LITHO_OK;
+    std::vector<BandWs> w;
+    LITHO_TRY(prepare_buckets(c, bk, nb, precision, 0, w));
+    const int ns = std::min(active, 4);
+    LITHO_TRY(ensure_pipeline(c, ns));
+    while ((int)c->bev.size() < nb) {
+        hipEvent_t e;
+        LITHO_HIP(hipEventCreate(&e));
+        c->bev.push_back(e);
+    }
+    if (!c->bev0) LITHO_HIP(hipEventCreate(&c->bev0));
+    c->nstage = 0;
+    c->timed = false;
+    std::vector<BandPlan> plans(nb);
+    for (int i = 0; i < nb; i++)
+        if (bk[i].count) plans[i] = plan_of(c, bk[i].N, bk[i].bank->Nx, bk[i].bank->nk, precision);
+    auto enqueue = [&](bool timing) -> int {
+        hipStream_t s0 = c->stream;
+        if (timing) LITHO_HIP(hipEventRecord(c->bev0, s0));
+        LITHO_HIP(hipEventRecord(c->evStart, s0));
+        for (int i = 1; i < ns; i++) LITHO_HIP(hipStreamWaitEvent(c->xs[i - 1], c->evStart, 0));
+        int k = 0;
+        for (int i = 0; i < nb; i++) {
+            if (!bk[i].count) continue;
+            hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
+            k++;
+            LITHO_TRY(enqueue_band(c, bk[i].bank, bk[i].d_masks, bk[i].count, bk[i].N,
+                                   bk[i].d_images, precision, plans[i], w[i], st));
+            if (timing) LITHO_HIP(hipEventRecord(c->bev[i], st));
+        }
+        for (int i = 1; i < ns; i++) {
+            LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
+            LITHO_HIP(hipStreamWaitEvent(s0, c->xe[i - 1], 0));
+        }
+        return LITHO_OK;
+    };
+    if (!use_graph) {
+        c->nbev = 0;
+        LITHO_TRY(enqueue(true));
+        c->nbev = nb;
+        return LITHO_OK;
+    }
+    c->nbev = 0;
+    std::vector<long> key = {2, precision, nb};
+    for (int i = 0; i < nb; i++) {
+        const LithoBucket& b = bk[i];
+        for (long v : {(long)b.bank, (long)b.N, (long)b.count, (long)b.d_masks, (long)b.d_images,
+                       (long)w[i].R, (long)w[i].U, (long)w[i].M, (long)w[i].C, (long)plans[i].G,
+                       (long)plans[i].lean_socs, (long)plans[i].lean_mem})
+            key.push_back(v);
+    }
+    return run_graph(c, key, [&]() { return enqueue(false); });
+}
+
+int litho_bucket_times(litho_ctx* c, float* ms, int nb) {
+    if (!c || !ms || nb <= 0 || c->nbev == 0) return LITHO_EARG;
+    const int n = std::min(nb, c->nbev);
+    LITHO_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; i++) {
+        ms[i] = 0.f;
+        if (hipEventElapsedTime(&ms[i], c->bev0, c->bev[i]) != hipSuccess) ms[i] = -1.f;
+    }
+    return n;
+}
+
+int litho_autotune(litho_ctx* c, const LithoBucket* bk, int nb, int precision, int reps) {
+    int active = 0;
+    if (!c || reps <= 0 || reps > 100) return LITHO_EARG;
+    LITHO_TRY(check_buckets(bk, nb, precision, &active));
+    const int sample = 32;
+    std::vector<BandWs> w;
+    LITHO_TRY(prepare_buckets(c, bk, nb, precision, sample, w));
+    hipStream_t st = c->stream;
+    c->timed = false;
+    int tuned = 0;
+    for (int i = 0; i < nb; i++) {
+        const LithoBucket& b = bk[i];
+        if (!b.count) continue;
+        const int Nx = b.bank->Nx;
+        const auto key = std::make_tuple(b.N, Nx, b.bank->nk, precision);
+        if (c->plans.count(key)) continue;
+        const int B = std::min(sample, b.count);
+        float best = 1e30f;
+        std::array<int, 3> pick = {band_grid(Nx), 0, 0};
+        for (int G : {36, 81, 144}) {
+            if (G < 4 * Nx + 1 || G > b.N) continue;
+            for (int ls = 0; ls < 2; ls++) {
+                if (ls && G != 144) continue;
+                for (int lm = 0; lm < 2; lm++) {
+                    if (lm && b.N != 1024) continue;
+                    BandPlan pl;
+                    pl.G = G;
+                    pl.lean_socs = ls;
+                    pl.lean_mem = lm;
+                    int rc = enqueue_band(c, b.bank, b.d_masks, B, b.N, b.d_images, precision, pl,
+                                          w[i], st);
+                    if (rc == LITHO_EARG) continue;  // shape not instantiated
+                    LITHO_TRY(rc);
+                    LITHO_HIP(hipEventRecord(c->ev[0], st));
+                    for (int r = 0; r < reps; r++)
+                        LITHO_TRY(enqueue_band(c, b.bank, b.d_masks, B, b.N, b.d_images,
+                                               precision, pl, w[i], st));
+                    LITHO_HIP(hipEventRecord(c->ev[1], st));
+                    LITHO_HIP(hipEventSynchronize(c->ev[1]));
+                    float ms = 0.f;
+                    LITHO_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                    if (ms < best) {
+                        best = ms;
+                        pick = {G, ls, lm};
+                    }
+                }
+            }
+        }
+        c->plans[key] = pick;
+        tuned++;
+    }
+    return tuned;
+}
+
+int litho_plan_get(litho_ctx* c, int N, int Nx, int nk, int precision, int* G, int* lean_socs,
+                   int* lean_mem) {
+    if (!c || !G || !lean_socs || !lean_mem) return LITHO_EARG;
+    const BandPlan p = plan_of(c, N, Nx, nk, precision);
+    *G = p.G;
+    *lean_socs = p.lean_socs;
+    *lean_mem = p.lean_mem;
+    return c->plans.count(std::make_tuple(N, Nx, nk, precision)) ? 1 : 0;
 }
 
 int litho_last_timing(litho_ctx* c, float* total_ms, float* stage_ms, int nstage) {

@@ -469,7 +469,7 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict
     float* o2 = o1 + L;
     fft2_stage2<Geo, true>(t, xb, [&](int, int x, float2 v) {
         // undo the index shift: multiply by e^{-2 pi i band x / L}
-        const float2 w = twG[(int)(((long)band * x) & (L - 1))];
+        const float2 w = twG[(int)(((long)band * x) % L)];
         const float2 r = cmul(v, w);
         o1[x] = r.x;
         o2[x] = r.y;
@@ -1097,6 +1097,7 @@ int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, i
     }
     LITHO_RT(16, 16, 256)
     LITHO_RT(16, 32, 256)
+    LITHO_RT(16, 48, 128)
     if (lean) {
         LITHO_RT(32, 32, 128)
     }
@@ -1121,6 +1122,7 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
     }
     LITHO_MC(16, 16, 256)
     LITHO_MC(16, 32, 256)
+    LITHO_MC(16, 48, 128)
     if (lean) {
         LITHO_MC(32, 32, 128)
     }
@@ -1147,6 +1149,8 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
     LITHO_FC(16, 16, 16, 256)
     LITHO_FC(16, 32, 3, 256)
     LITHO_FC(16, 32, 16, 256)
+    LITHO_FC(16, 48, 3, 128)
+    LITHO_FC(16, 48, 16, 128)
     if (lean) {
         LITHO_FC(32, 32, 5, 128)
     }
@@ -1175,6 +1179,8 @@ int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, 
     LITHO_IT(16, 16, 16, 256)
     LITHO_IT(16, 32, 3, 256)
     LITHO_IT(16, 32, 16, 256)
+    LITHO_IT(16, 48, 3, 128)
+    LITHO_IT(16, 48, 16, 128)
     if (lean) {
         LITHO_IT(32, 32, 5, 128)
     }

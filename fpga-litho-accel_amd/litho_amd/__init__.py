@@ -35,6 +35,7 @@ EXPORTS = [
     "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_ctx_set_path",
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
     "litho_tcc_build_prec", "litho_kernels_from_source_prec",
+    "litho_socs_image_buckets", "litho_bucket_times", "litho_autotune", "litho_plan_get",
 ]
 LITHO_TCC_FP64 = 0
 LITHO_TCC_SPLIT_BF16 = 1
@@ -63,6 +64,12 @@ class LithoOptics(C.Structure):
     _fields_ = [("lambda_", C.c_float), ("NA", C.c_float), ("defocus", C.c_float),
                 ("outerSigma", C.c_float), ("Lx", C.c_int), ("Ly", C.c_int),
                 ("Nx", C.c_int), ("Ny", C.c_int)]
+
+
+class LithoBucket(C.Structure):
+    """One clip size of a mixed-size step (litho_socs_image_buckets)."""
+    _fields_ = [("bank", C.c_void_p), ("N", C.c_int), ("count", C.c_int),
+                ("d_masks", C.c_void_p), ("d_images", C.c_void_p)]
 
 
 def library_path():
@@ -103,6 +110,11 @@ def lib():
                                          C.POINTER(i)]
         L.litho_kernels_from_source.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p,
                                                 _f32p, _f32p, C.POINTER(i)]
+        bp = C.POINTER(LithoBucket)
+        L.litho_socs_image_buckets.argtypes = [vp, bp, i, i, i]
+        L.litho_bucket_times.argtypes = [vp, _f32p, i]
+        L.litho_autotune.argtypes = [vp, bp, i, i, i]
+        L.litho_plan_get.argtypes = [vp, i, i, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
         L.litho_tcc_build_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p, _f64p]
         L.litho_kernels_from_source_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, i,
                                                      _f64p, _f32p, _f32p, C.POINTER(i)]
@@ -345,6 +357,42 @@ class Context:
         _check(lib().litho_socs_image_batch_pipelined(self.h, bank, d_mask, B, N, d_image,
                                                       precision, chunks, int(graph)),
                "litho_socs_image_batch_pipelined")
+
+    @staticmethod
+    def _buckets(buckets):
+        """[(bank, N, count, d_masks, d_images), ...] -> ctypes array."""
+        arr = (LithoBucket * len(buckets))()
+        for k, (bank, N, count, dm, di) in enumerate(buckets):
+            arr[k] = LithoBucket(bank, int(N), int(count), getattr(dm, "value", dm),
+                                 getattr(di, "value", di))
+        return arr
+
+    def socs_image_buckets(self, buckets, precision=0, graph=True):
+        """Mixed clip sizes in one step: buckets = [(bank, N, count, d_masks,
+        d_images), ...]; precision flags as socs_image_batch_pipelined."""
+        arr = self._buckets(buckets)
+        _check(lib().litho_socs_image_buckets(self.h, arr, len(buckets), precision, int(graph)),
+               "litho_socs_image_buckets")
+
+    def bucket_times(self, nb):
+        """Per-bucket device end times (ms) of the last eager bucket step."""
+        t = np.zeros(nb, np.float32)
+        n = lib().litho_bucket_times(self.h, t, nb)
+        _check(min(n, 0), "litho_bucket_times")
+        return t[:n]
+
+    def autotune(self, buckets, precision=0, reps=3):
+        n = lib().litho_autotune(self.h, self._buckets(buckets), len(buckets), precision, reps)
+        _check(min(n, 0), "litho_autotune")
+        return n
+
+    def plan(self, N, Nx, nk, precision=0):
+        """(conv grid, lean SOCS block, lean FFT block, tuned?) for a shape."""
+        g, ls, lm = C.c_int(), C.c_int(), C.c_int()
+        t = lib().litho_plan_get(self.h, N, Nx, nk, precision, C.byref(g), C.byref(ls),
+                                 C.byref(lm))
+        _check(min(t, 0), "litho_plan_get")
+        return g.value, bool(ls.value), bool(lm.value), bool(t)
 
     def set_path(self, path):
         _check(lib().litho_ctx_set_path(self.h, path), "litho_ctx_set_path")

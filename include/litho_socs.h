@@ -207,6 +207,37 @@ int litho_socs_image_batch_pipelined(litho_ctx* ctx, const litho_bank* bank, con
                                      int B, int N, void* d_image, int precision, int chunks,
                                      int use_graph);
 
+/* Mixed clip sizes (no counterpart in the reference): one bucket per clip
+ * size, each with its own kernel bank (optics of that period), all buckets
+ * in one step -- their launches spread over up to four streams and, with
+ * use_graph, captured once as one HIP graph and replayed.  Precision flags as
+ * litho_socs_image_batch_pipelined; N in {256, 512, 768, 1024}.  Each bucket
+ * runs under the plan litho_autotune chose for its shape (default: the
+ * smallest band grid, 48-slot SOCS blocks, 256-thread FFT blocks). */
+typedef struct {
+    const litho_bank* bank;  /* kernel set of this clip size */
+    int N;                   /* clip size in pixels */
+    int count;               /* clips in the bucket (0 = skip) */
+    const void* d_masks;     /* count x N x N: f32, or uint8 levels with MASK_U8 */
+    void* d_images;          /* count x N x N: f32, or fp16 with IMAGE_FP16 */
+} LithoBucket;
+int litho_socs_image_buckets(litho_ctx* ctx, const LithoBucket* buckets, int nbuckets,
+                             int precision, int use_graph);
+/* Device time (ms) from the start of the last eager (use_graph = 0) bucket
+ * step to the end of each bucket's last launch; -1 for a skipped bucket.
+ * Returns the number of buckets written. */
+int litho_bucket_times(litho_ctx* ctx, float* ms, int nbuckets);
+/* Autotuner: for every bucket shape (N, Nx, nk, precision) not tuned yet on
+ * this context, time each candidate plan -- conv grid 36/81/144 (SOCS FFT
+ * radix 6/9/12), SOCS block of 48 or 24 kernel slots, FFT row/column blocks
+ * of 256 or 128 threads -- on up to 32 of the bucket's clips (their images
+ * are written) and keep the fastest.  Returns the number of shapes tuned. */
+int litho_autotune(litho_ctx* ctx, const LithoBucket* buckets, int nbuckets, int precision,
+                   int reps);
+/* The plan used for a shape: returns 1 if tuned, 0 for the default. */
+int litho_plan_get(litho_ctx* ctx, int N, int Nx, int nk, int precision, int* G, int* lean_socs,
+                   int* lean_mem);
+
 #ifdef __cplusplus
 }
 #endif
