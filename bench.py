@@ -83,6 +83,66 @@ def cpu_baseline(masks_host, K, sc, Nx, threads, budget_s, gpu_image_sets):
     return done / dt, done, dt, [max(e) for e in errs]
 
 
+def mixed_size_bench(ctx, src, sigma, steps):
+    """Batching form: 512 clips with sizes drawn from {256, 512, 768, 1024}^2
+    at 4 nm, one K=16 kernel set per size (four TCC builds), all clips in one
+    graph-captured step over four buckets; images/s and per-clip latency (a
+    clip's latency is the device time from the step's start to the end of its
+    bucket's last launch, all clips submitted together)."""
+    rng = np.random.default_rng(2024)
+    sizes = rng.choice([256, 512, 768, 1024], size=512)
+    out = {"clips": 512, "K": 16, "counts": {}, "plans": {}}
+    t0 = time.perf_counter()
+    banks, bl, bl8, dev = [], [], [], []
+    for N in (256, 512, 768, 1024):
+        cnt = int((sizes == N).sum())
+        out["counts"][str(N)] = cnt
+        Nx = L.compute_nx(NA, PIX_NM * N, sigma, LAMBDA)
+        sc, K, _ = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, PIX_NM * N, PIX_NM * N,
+                                         Nx, Nx, 16)
+        bank = ctx.bank(K, sc, Nx, Nx)
+        banks.append(bank)
+        uniq = np.stack([L.manhattan_mask(N, 5000 + 100 * N + i, WMIN, WMAX, FILL)
+                         for i in range(8)])
+        masks = uniq[np.arange(cnt) % 8]
+        dm, dm8, di = ctx.alloc(masks.nbytes), ctx.alloc(masks.size), ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        ctx.h2d(dm8, np.rint(masks * 255).astype(np.uint8))
+        dev += [dm, dm8, di]
+        bl.append((bank, N, cnt, dm, di))
+        bl8.append((bank, N, cnt, dm8, di))
+    out["tcc_builds_s"] = round(time.perf_counter() - t0, 3)
+    for name, prec, b in (("fp32", L.LITHO_PREC_FP32, bl), ("compact", L.LITHO_PREC_COMPACT, bl8)):
+        t0 = time.perf_counter()
+        ctx.autotune(b, prec, reps=3)
+        tune_s = time.perf_counter() - t0
+        ctx.socs_image_buckets(b, prec, False)
+        ends = ctx.bucket_times(len(b))
+        lat = np.concatenate([np.full(c, e) for (_, _, c, _, _), e in zip(b, ends)])
+        for _ in range(3):
+            ctx.socs_image_buckets(b, prec, True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.socs_image_buckets(b, prec, True)
+        ctx.sync()
+        rate = 512 * steps / (time.perf_counter() - t0)
+        out[name] = {"value": round(rate, 1), "unit": "images/s",
+                     "p50_latency_ms": round(float(np.percentile(lat, 50)), 4),
+                     "p99_latency_ms": round(float(np.percentile(lat, 99)), 4),
+                     "autotune_s": round(tune_s, 3)}
+        out["plans"][name] = {str(N): list(ctx.plan(N, L.compute_nx(NA, PIX_NM * N, sigma, LAMBDA),
+                                                     16, prec)[:3]) for N in (256, 512, 768, 1024)}
+        log(f"mixed sizes ({name}): {rate:.1f} images/s, per-clip latency p50 "
+            f"{out[name]['p50_latency_ms']} ms p99 {out[name]['p99_latency_ms']} ms, "
+            f"autotune {tune_s:.2f}s")
+    for p_ in dev:
+        ctx.free(p_)
+    for bk in banks:
+        L.lib().litho_bank_destroy(bk)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +155,8 @@ def main():
                     help="conv grid: auto (band grid 144 at Nx=34) or the reference nextPow2 grid")
     ap.add_argument("--chunks", type=int, default=1,
                     help="sub-batches of the pipelined (chosen) form")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-size (batching form) measurement")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -216,6 +278,8 @@ def main():
             traffic = int(tj["total_bytes_per_step"] * B / tj["batch"])
 
     out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
+    mixed_sizes = None if args.no_mixed else mixed_size_bench(ctx, src, sigma,
+                                                              max(3, args.steps // 4))
 
     cpu = None
     max_err = None
@@ -282,6 +346,7 @@ def main():
             "bf16_bank_value": round(bf16_bank, 2),
             "bf16_bank_max_rel_err_vs_fp64": bf16_err},
         "setup_s": round(setup_s, 3),
+        "batching_form": mixed_sizes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
