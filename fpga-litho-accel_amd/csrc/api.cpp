@@ -101,6 +101,11 @@ static int ctx_ws(litho_ctx* c, int slot, size_t bytes, void** out) {
     return LITHO_OK;
 }
 
+namespace litho {
+hipStream_t ctx_stream(litho_ctx* c) { return c->stream; }
+int ctx_device(litho_ctx* c) { return c->device; }
+}  // namespace litho
+
 extern "C" {
 
 const char* litho_status_string(int status) {

@@ -36,6 +36,8 @@ EXPORTS = [
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
     "litho_tcc_build_prec", "litho_kernels_from_source_prec",
     "litho_socs_image_buckets", "litho_bucket_times", "litho_autotune", "litho_plan_get",
+    "litho_fno_create", "litho_fno_destroy", "litho_fno_param_count", "litho_fno_get_params",
+    "litho_fno_set_params", "litho_fno_forward", "litho_fno_loss_grad", "litho_fno_adam",
 ]
 LITHO_TCC_FP64 = 0
 LITHO_TCC_SPLIT_BF16 = 1
@@ -110,6 +112,15 @@ def lib():
                                          C.POINTER(i)]
         L.litho_kernels_from_source.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p,
                                                 _f32p, _f32p, C.POINTER(i)]
+        L.litho_fno_create.argtypes = [vp, i, C.c_ulonglong, C.POINTER(vp)]
+        L.litho_fno_destroy.argtypes = [vp]
+        L.litho_fno_param_count.argtypes = [vp]
+        L.litho_fno_param_count.restype = C.c_longlong
+        L.litho_fno_get_params.argtypes = [vp, _f32p, C.c_longlong, i]
+        L.litho_fno_set_params.argtypes = [vp, _f32p, C.c_longlong]
+        L.litho_fno_forward.argtypes = [vp, vp, i, vp]
+        L.litho_fno_loss_grad.argtypes = [vp, vp, vp, i, C.POINTER(f)]
+        L.litho_fno_adam.argtypes = [vp, f, f, f, f]
         bp = C.POINTER(LithoBucket)
         L.litho_socs_image_buckets.argtypes = [vp, bp, i, i, i]
         L.litho_bucket_times.argtypes = [vp, _f32p, i]

@@ -238,6 +238,26 @@ int litho_autotune(litho_ctx* ctx, const LithoBucket* buckets, int nbuckets, int
 int litho_plan_get(litho_ctx* ctx, int N, int Nx, int nk, int precision, int* G, int* lean_socs,
                    int* lean_mem);
 
+/* ---- learned surrogate (no counterpart in the reference) ------------------
+ * A Fourier neural operator mask -> aerial image (width 32, 16 x 16 retained
+ * modes per sign of ky, GELU, 4 Fourier layers, 128-channel projection),
+ * trained on the GPU by its own forward / backward kernels.  Weights are
+ * FP32 masters read through a bf16 copy; arithmetic FP32.  N: power of two,
+ * 32..256.  Masks and images are device f32 [B][N][N] on the context stream. */
+typedef struct litho_fno litho_fno;
+int litho_fno_create(litho_ctx* ctx, int N, unsigned long long seed, litho_fno** out);
+int litho_fno_destroy(litho_fno* f);
+long long litho_fno_param_count(const litho_fno* f);
+/* which = 0: FP32 weights, 1: gradients of the last litho_fno_loss_grad */
+int litho_fno_get_params(litho_fno* f, float* host, long long n, int which);
+int litho_fno_set_params(litho_fno* f, const float* host, long long n);
+int litho_fno_forward(litho_fno* f, const float* d_masks, int B, float* d_images);
+/* MSE loss over the batch and the gradients of every weight (synchronous
+ * when loss is non-NULL) */
+int litho_fno_loss_grad(litho_fno* f, const float* d_masks, const float* d_targets, int B,
+                        float* loss);
+int litho_fno_adam(litho_fno* f, float lr, float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif
