@@ -143,6 +143,68 @@ def mixed_size_bench(ctx, src, sigma, steps):
     return out
 
 
+def surrogate_bench(ctx, src, sigma, train_s):
+    """Learned form: the FNO trained on 20k SOCS pairs (256^2 at 4 nm, K=16)
+    by its own GPU kernels for `train_s` seconds; held-out max relative error
+    against SOCS, inference and served images/s (1 in 16 clips re-run by SOCS,
+    fallback above 1e-2), SOCS images/s at the same size."""
+    from litho_amd import fno
+    N, ntr, nte, batch = 256, 20000, 512, 32
+    Nx = L.compute_nx(NA, PIX_NM * N, sigma, LAMBDA)
+    sc, K, _ = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, PIX_NM * N, PIX_NM * N, Nx,
+                                     Nx, 16)
+    bank = ctx.bank(K, sc, Nx, Nx)
+    t0 = time.perf_counter()
+    tm, ti = fno.socs_pairs(ctx, bank, N, ntr, 10_000, 256, WMIN, WMAX, FILL)
+    vm, vi = fno.socs_pairs(ctx, bank, N, nte, 900_000, 256, WMIN, WMAX, FILL)
+    data_s = time.perf_counter() - t0
+    model = fno.FNO(ctx, N, seed=5)
+    steps = int(train_s / 0.014) + 1  # the cosine schedule spans the expected step count
+    t0 = time.perf_counter()
+    losses = fno.train(model, tm, ti, ntr, steps=steps, batch=batch, lr=2e-3, time_budget=train_s)
+    tr_s = time.perf_counter() - t0
+    e = fno.max_rel_errors(ctx, model, vm, vi, nte)
+    d_o = ctx.alloc(max(256, nte) * N * N * 4)
+
+    def rate(fn, B, reps):
+        for _ in range(2):
+            fn()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        return B * reps / (time.perf_counter() - t)
+
+    inf = rate(lambda: model.forward(vm, 256, d_o), 256, 10)
+    socs = rate(lambda: ctx.socs_image_batch(bank, vm, 256, N, d_o), 256, 20)
+    srv = fno.SurrogateServer(ctx, model, bank, tol=1e-2, verify_every=16)
+    served = rate(lambda: srv.serve(vm, nte, d_o), nte, 3)
+    out = {"form": "Fourier neural operator, width 32, 16x16 retained modes per ky sign, 4 layers, "
+                   "GELU, bf16 weights read with FP32 accumulate; trained by its own HIP "
+                   "forward/backward kernels (Adam, MSE) on SOCS images of seeded Manhattan masks",
+           "N": N, "K_socs": 16, "train_pairs": ntr, "data_s": round(data_s, 2),
+           "train_s": round(tr_s, 1), "steps": int(len(losses)), "batch": batch,
+           "samples_per_s": round(len(losses) * batch / tr_s, 1),
+           "loss": [float(losses[0]), float(losses[-1])],
+           "heldout_clips": nte,
+           "max_rel_err": {"median": float(np.median(e)), "p90": float(np.percentile(e, 90)),
+                           "max": float(e.max()), "frac_le_1e-2": float((e <= 1e-2).mean())},
+           "inference_images_per_s": round(inf, 1),
+           "served_images_per_s": round(served, 1),
+           "served_fallbacks": srv.fallbacks, "served_verified": srv.verified,
+           "socs_images_per_s_same_size": round(socs, 1),
+           "inference_vs_socs": round(inf / socs, 5)}
+    log(f"surrogate: {len(losses)} steps in {tr_s:.1f}s, held-out max rel err median "
+        f"{np.median(e):.2e} p90 {np.percentile(e, 90):.2e} max {e.max():.2e}; inference "
+        f"{inf:.0f} images/s vs SOCS {socs:.0f} images/s at 256^2")
+    for p_ in (tm, ti, vm, vi, d_o):
+        ctx.free(p_)
+    model.close()
+    L.lib().litho_bank_destroy(bank)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +217,8 @@ def main():
                     help="conv grid: auto (band grid 144 at Nx=34) or the reference nextPow2 grid")
     ap.add_argument("--chunks", type=int, default=1,
                     help="sub-batches of the pipelined (chosen) form")
+    ap.add_argument("--surrogate-s", type=float, default=60.0,
+                    help="FNO training budget in seconds (0: skip the learned form)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-size (batching form) measurement")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -280,6 +344,7 @@ def main():
     out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
     mixed_sizes = None if args.no_mixed else mixed_size_bench(ctx, src, sigma,
                                                               max(3, args.steps // 4))
+    surrogate = surrogate_bench(ctx, src, sigma, args.surrogate_s) if args.surrogate_s > 0 else None
 
     cpu = None
     max_err = None
@@ -347,6 +412,7 @@ def main():
             "bf16_bank_max_rel_err_vs_fp64": bf16_err},
         "setup_s": round(setup_s, 3),
         "batching_form": mixed_sizes,
+        "learned_form": surrogate,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
