@@ -847,7 +847,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
                                       "image_rows"};
         LITHO_HIP(hipEventRecord(c->ev[0], st));
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, -Nx, W, false, false, st));
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, -Nx, W, false, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[1], st));
         LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, W, -Nx, W, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[2], st));
@@ -856,7 +856,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         LITHO_HIP(hipEventRecord(c->ev[3], st));
         LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[4], st));
-        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, st));
+        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[5], st));
         c->nstage = 5;
         for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
@@ -936,7 +936,7 @@ static int ensure_pipeline(litho_ctx* c, int nstreams) {
 // ---------------------------------------------------------------------------
 struct BandPlan {
     int G = 0;
-    bool lean_socs = false;
+    int socs_shape = 0;  // 0: 48-slot blocks, 1: 24-slot blocks, 2: wave-synced transforms
     bool lean_mem = false;
 };
 
@@ -945,7 +945,7 @@ static BandPlan plan_of(litho_ctx* c, int N, int Nx, int nk, int prec) {
     auto it = c->plans.find(std::make_tuple(N, Nx, nk, prec));
     if (it != c->plans.end()) {
         p.G = it->second[0];
-        p.lean_socs = it->second[1] != 0;
+        p.socs_shape = it->second[1];
         p.lean_mem = it->second[2] != 0;
     } else {
         p.G = band_grid(Nx);
@@ -958,8 +958,6 @@ static bool prec_ok(int prec) {
                       LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA;
     if (prec & ~known) return false;
     if ((prec & LITHO_PREC_BANK_BF16) && (prec & LITHO_PREC_BANK_FP16)) return false;
-    if (!(prec & LITHO_PREC_DFT_MFMA) && (prec & (LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16)))
-        return false;
     return true;
 }
 
@@ -1031,11 +1029,11 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     if (mfma)
         LITHO_TRY(launch_mask_rows_mfma(N, d_mask, u8, tm, w.R, B, W, mixed, st));
     else
-        LITHO_TRY(launch_rows_dft_t(N, (const float*)d_mask, w.R, (long)B * N, -Nx, W, mixed,
-                                    pl.lean_mem, st));
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, w.R, (long)B * N, -Nx, W, mixed, pl.lean_mem, u8,
+                                    st));
     LITHO_TRY(launch_mask_cols_t(N, w.R, (float2*)w.M, B, W, -Nx, W, mixed, pl.lean_mem, st));
     LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, B,
-                               (float2*)w.C, pl.lean_socs, st));
+                               (float2*)w.C, pl.socs_shape, st));
     if (mfma) {
         // FP32 U: the image-side product splits it into hi + lo parts
         LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st));
@@ -1043,8 +1041,8 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
                                          st));
     } else {
         LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st));
-        LITHO_TRY(launch_image_rows_t(N, w.U, (float*)d_image, (long)B * N, band, mixed,
-                                      pl.lean_mem, st));
+        LITHO_TRY(launch_image_rows_t(N, w.U, d_image, (long)B * N, band, mixed, pl.lean_mem,
+                                      out16, st));
     }
     return LITHO_OK;
 }
@@ -1135,7 +1133,10 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
             wk.M = (float2*)w[0].M + (size_t)b0 * W * W;
             wk.C = (float2*)w[0].C + (size_t)b0 * (2 * band + 1) * (band + 1);
             BandPlan pl = tuned;
-            if (chunks > 1) pl.lean_socs = pl.lean_mem = true;
+            if (chunks > 1) {
+                if (pl.G == 144 && pl.socs_shape == 0) pl.socs_shape = 1;
+                pl.lean_mem = true;
+            }
             LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN * mb, nb, N,
                                    (char*)d_image + (size_t)b0 * NN * ib, precision, pl, wk, st));
         }
@@ -1148,7 +1149,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
     if (!use_graph) return enqueue();
     std::vector<long> key = {1, (long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
                              (long)w[0].R, (long)w[0].U, (long)w[0].M, (long)w[0].C, tuned.G,
-                             tuned.lean_socs, tuned.lean_mem};
+                             tuned.socs_shape, tuned.lean_mem};
     return run_graph(c, key, enqueue);
 }
 
@@ -1240,7 +1241,7 @@ int litho_socs_image_buckets(litho_ctx* c, const LithoBucket* bk, int nb, int pr
         const LithoBucket& b = bk[i];
         for (long v : {(long)b.bank, (long)b.N, (long)b.count, (long)b.d_masks, (long)b.d_images,
                        (long)w[i].R, (long)w[i].U, (long)w[i].M, (long)w[i].C, (long)plans[i].G,
-                       (long)plans[i].lean_socs, (long)plans[i].lean_mem})
+                       (long)plans[i].socs_shape, (long)plans[i].lean_mem})
             key.push_back(v);
     }
     return run_graph(c, key, [&]() { return enqueue(false); });
@@ -1278,13 +1279,13 @@ int litho_autotune(litho_ctx* c, const LithoBucket* bk, int nb, int precision, i
         std::array<int, 3> pick = {band_grid(Nx), 0, 0};
         for (int G : {36, 81, 144}) {
             if (G < 4 * Nx + 1 || G > b.N) continue;
-            for (int ls = 0; ls < 2; ls++) {
-                if (ls && G != 144) continue;
+            for (int ls = 0; ls < 3; ls++) {
+                if (ls == 1 && G != 144) continue;
                 for (int lm = 0; lm < 2; lm++) {
                     if (lm && b.N != 1024) continue;
                     BandPlan pl;
                     pl.G = G;
-                    pl.lean_socs = ls;
+                    pl.socs_shape = ls;
                     pl.lean_mem = lm;
                     int rc = enqueue_band(c, b.bank, b.d_masks, B, b.N, b.d_images, precision, pl,
                                           w[i], st);
@@ -1311,12 +1312,22 @@ int litho_autotune(litho_ctx* c, const LithoBucket* bk, int nb, int precision, i
     return tuned;
 }
 
-int litho_plan_get(litho_ctx* c, int N, int Nx, int nk, int precision, int* G, int* lean_socs,
+int litho_plan_set(litho_ctx* c, int N, int Nx, int nk, int precision, int G, int socs_shape,
+                   int lean_mem) {
+    if (!c || !band_shape_ok(N, Nx) || nk <= 0 || !prec_ok(precision) || G < 4 * Nx + 1 ||
+        G > N || (G != 36 && G != 81 && G != 144) || socs_shape < 0 || socs_shape > 2 ||
+        (socs_shape == 1 && G != 144) || lean_mem < 0 || lean_mem > 1 || (lean_mem && N != 1024))
+        return LITHO_EARG;
+    c->plans[std::make_tuple(N, Nx, nk, precision)] = {G, socs_shape, lean_mem};
+    return LITHO_OK;
+}
+
+int litho_plan_get(litho_ctx* c, int N, int Nx, int nk, int precision, int* G, int* socs_shape,
                    int* lean_mem) {
-    if (!c || !G || !lean_socs || !lean_mem) return LITHO_EARG;
+    if (!c || !G || !socs_shape || !lean_mem) return LITHO_EARG;
     const BandPlan p = plan_of(c, N, Nx, nk, precision);
     *G = p.G;
-    *lean_socs = p.lean_socs;
+    *socs_shape = p.socs_shape;
     *lean_mem = p.lean_mem;
     return c->plans.count(std::make_tuple(N, Nx, nk, precision)) ? 1 : 0;
 }
@@ -1414,12 +1425,12 @@ int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double
     LITHO_TRY(dI.alloc(NN * sizeof(float)));
     LITHO_HIP(hipMemcpyAsync(dm.p, mask, NN * sizeof(float), hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(dT.p, ht.data(), ht.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, -Nx, W, false, false, st));
+    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, -Nx, W, false, false, false, st));
     LITHO_TRY(launch_mask_cols_t(N, dR.p, dM.as<float2>(), 1, W, -Nx, W, false, false, st));
     LITHO_TRY(launch_tcc_image_spectrum(dM.as<float2>(), dT.as<double2>(), Nx, Ny,
                                         dC.as<float2>(), st));
     LITHO_TRY(launch_fi_cols_t(N, dC.as<float2>(), dU.p, 1, band, false, false, st));
-    LITHO_TRY(launch_image_rows_t(N, dU.p, dI.as<float>(), N, band, false, false, st));
+    LITHO_TRY(launch_image_rows_t(N, dU.p, dI.as<float>(), N, band, false, false, false, st));
     LITHO_HIP(hipMemcpyAsync(image, dI.p, NN * sizeof(float), hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
     return LITHO_OK;

@@ -51,6 +51,14 @@ __device__ __forceinline__ unsigned u8pair_to_h2(unsigned d, int hi) {
     return __builtin_bit_cast(unsigned, v);
 }
 
+// LDS written by some lanes of a wave and read by others: order it within the
+// wavefront (no block barrier needed)
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ unsigned f2_to_h2(float a, float b) {
     h2v v = {(_Float16)a, (_Float16)b};
     return __builtin_bit_cast(unsigned, v);
@@ -129,6 +137,9 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
             }
         }
         const u4v* fb = fs[c & 1];
+        // B fragments one (k-step, tile) ahead: each LDS read overlaps the
+        // MFMAs of the previous fragment
+        u4v bnext = fb[lane];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             u4v as[RT];
@@ -141,7 +152,10 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
                             : af[rt][s];
 #pragma unroll
             for (int t = 0; t < NT; t++) {
-                const h8 bf = __builtin_bit_cast(h8, fb[(s * NT + t) * 64 + lane]);
+                const u4v bcur = bnext;
+                const int nx = s * NT + t + 1;
+                if (nx < 4 * NT) bnext = fb[nx * 64 + lane];
+                const h8 bf = __builtin_bit_cast(h8, bcur);
 #pragma unroll
                 for (int rt = 0; rt < RT; rt++)
                     acc[rt][t] = mfma16(__builtin_bit_cast(h8, as[rt]), bf, acc[rt][t]);
@@ -189,18 +203,24 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
 // ---------------------------------------------------------------------------
 // U [B][nb][N] (float2, FP32) -> image [B][N][N] (f32 or fp16).  NKS k-steps of
 // 32 cover 8 kx values each: (re, im) x (hi, lo) for two kx per lane group.
-// The product is formed transposed (table as the A operand, U as B) so that a
-// lane's D fragment is 4 consecutive pixels of one image row: one vector store.
+// The product is formed transposed (table as the A operand, U as B): a lane's
+// D fragment is 4 consecutive pixels of one image row.  Fragments are staged
+// per wave in LDS until every row has a full 128-byte segment, which is then
+// stored 16 bytes per lane (8 rows per instruction), so HBM sees whole lines.
 // ---------------------------------------------------------------------------
 template <int NKS, bool OUT16>
 __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* __restrict__ U,
                                                                  const u4v* __restrict__ Hpk,
                                                                  void* __restrict__ img_, int N,
                                                                  int nb) {
-    constexpr int CH = 4 * NKS * 64;  // uint4 per XCH-column chunk (4 column tiles)
+    constexpr int CH = 4 * NKS * 64;  // u4v per XCH-column chunk (4 column tiles)
     constexpr int PER = CH / 256;
     static_assert(CH % 256 == 0, "chunk copy");
-    __shared__ u4v fs[2][CH];
+    constexpr int OB = OUT16 ? 2 : 4;         // bytes per pixel
+    constexpr int TPS = OUT16 ? 4 : 2;        // column tiles per 128-byte row segment
+    constexpr int SROW = 16 * TPS * OB + 16;  // staged row pitch in bytes (padded)
+    __shared__ u4v fs[CH];
+    __shared__ __attribute__((aligned(16))) unsigned char stg[4][64 * SROW];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
@@ -231,9 +251,10 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
         }
     }
 #pragma unroll
-    for (int i = 0; i < PER; i++) fs[0][tid + 256 * i] = pre[i];
+    for (int i = 0; i < PER; i++) fs[tid + 256 * i] = pre[i];
     __syncthreads();
-    const long ib = (long)b * N * N;
+    unsigned char* sw = stg[w];
+    unsigned char* out = (unsigned char*)img_ + ((long)b * N * N + (long)ybase * N) * OB;
     const int nch = N / XCH;
     for (int c = 0; c < nch; c++) {
         const bool more = c + 1 < nch;
@@ -241,38 +262,54 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
 #pragma unroll
             for (int i = 0; i < PER; i++) pre[i] = Hpk[(long)(c + 1) * CH + tid + 256 * i];
         }
-        const u4v* fb = fs[c & 1];
+        u4v tnext = fs[lane];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            f4 acc[RT];
+        for (int tp = 0; tp < 4 / TPS; tp++) {
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) acc[rt] = (f4){0.f, 0.f, 0.f, 0.f};
+            for (int tt = 0; tt < TPS; tt++) {
+                const int t = tp * TPS + tt;
+                f4 acc[RT];
 #pragma unroll
-            for (int s = 0; s < NKS; s++) {
-                const h8 tf = __builtin_bit_cast(h8, fb[(t * NKS + s) * 64 + lane]);
+                for (int rt = 0; rt < RT; rt++) acc[rt] = (f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int rt = 0; rt < RT; rt++)
-                    acc[rt] = mfma16(tf, __builtin_bit_cast(h8, a[rt][s]), acc[rt]);
-            }
-            // D[x][y]: x = 4g + i of the column tile, y = r16 of the row tile
-            const int x = c * XCH + 16 * t + 4 * g;
+                for (int s = 0; s < NKS; s++) {
+                    const u4v tcur = tnext;
+                    const int nx = t * NKS + s + 1;
+                    if (nx < 4 * NKS) tnext = fs[nx * 64 + lane];
+                    const h8 tf = __builtin_bit_cast(h8, tcur);
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) {
-                const long o = ib + (long)(ybase + 16 * rt + r16) * N + x;
-                if (OUT16) {
-                    uint2 v;
-                    v.x = f2_to_h2(acc[rt][0], acc[rt][1]);
-                    v.y = f2_to_h2(acc[rt][2], acc[rt][3]);
-                    *(uint2*)((_Float16*)img_ + o) = v;
-                } else {
-                    *(float4*)((float*)img_ + o) =
-                        make_float4(acc[rt][0], acc[rt][1], acc[rt][2], acc[rt][3]);
+                    for (int rt = 0; rt < RT; rt++)
+                        acc[rt] = mfma16(tf, __builtin_bit_cast(h8, a[rt][s]), acc[rt]);
+                }
+                // D[x][y]: pixels x = 4g..4g+3 of the column tile, row y = 16 rt + r16
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    unsigned char* d = sw + (16 * rt + r16) * SROW + (16 * tt + 4 * g) * OB;
+                    if (OUT16) {
+                        uint2 v;
+                        v.x = f2_to_h2(acc[rt][0], acc[rt][1]);
+                        v.y = f2_to_h2(acc[rt][2], acc[rt][3]);
+                        *(uint2*)d = v;
+                    } else {
+                        *(float4*)d = make_float4(acc[rt][0], acc[rt][1], acc[rt][2], acc[rt][3]);
+                    }
                 }
             }
+            wave_sync_lds();
+            // 64 rows x 128 bytes: 16 bytes per lane, 8 rows per store
+            const long x0 = (long)c * XCH + 16 * TPS * tp;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int row = 8 * q + (lane >> 3), seg = (lane & 7) * 16;
+                const uint4 v = *(const uint4*)(sw + row * SROW + seg);
+                *(uint4*)(out + ((long)row * N + x0) * OB + seg) = v;
+            }
+            wave_sync_lds();
         }
+        __syncthreads();  // every wave is done with this table chunk
         if (more) {
 #pragma unroll
-            for (int i = 0; i < PER; i++) fs[(c + 1) & 1][tid + 256 * i] = pre[i];
+            for (int i = 0; i < PER; i++) fs[tid + 256 * i] = pre[i];
         }
         __syncthreads();
     }

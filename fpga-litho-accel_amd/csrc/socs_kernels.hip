@@ -353,8 +353,8 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
 // rows_fwd2: same contract as rows_dft (window [kmin, kmin+count) of the
 // forward DFT of each real row, (-1)^k when ALT).
 // ---------------------------------------------------------------------------
-template <int P, int Q, int NTH, bool ALT, bool OUT_T, bool H16 = false>
-__global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict__ in,
+template <int P, int Q, int NTH, bool ALT, bool OUT_T, bool H16 = false, bool IN8 = false>
+__global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__ in_,
                                                         void* __restrict__ out_, int nrows,
                                                         int kmin, int count,
                                                         const float2* __restrict__ tw) {
@@ -374,9 +374,14 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
     fill_twiddles<L>(twG, tid, NTH);
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
-    const float* r0 = in + (2 * (act ? p : 0)) * (long)L;
+    const long roff = (2 * (act ? p : 0)) * (long)L;
+    // 8-bit masks: level / 255 (the PNG -> BIN convention), exact in FP32
+    const unsigned char* r8 = (const unsigned char*)in_ + roff;
+    const float* r0 = (const float*)in_ + roff;
     fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
-        return act ? make_float2(r0[n], r0[L + n]) : make_float2(0.f, 0.f);
+        if (!act) return make_float2(0.f, 0.f);
+        if (IN8) return make_float2((float)r8[n] * (1.0f / 255.0f), (float)r8[L + n] * (1.0f / 255.0f));
+        return make_float2(r0[n], r0[L + n]);
     });
     __syncthreads();
     float2 res[Geo::PER];
@@ -419,9 +424,9 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const float* __restrict_
 // (|kx| <= band) is placed at indices kx + band in [0, 2 band] so stage-1
 // inputs past ceil((2band+1)/Q) rows are compile-time zeros; the shift is
 // undone on the output by e^{-2 pi i band x / N}.
-template <int P, int Q, int NZ, int NTH, bool IN_T, bool H16 = false>
+template <int P, int Q, int NZ, int NTH, bool IN_T, bool H16 = false, bool O16 = false>
 __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict__ U_,
-                                                          float* __restrict__ img, int nrows,
+                                                          void* __restrict__ img, int nrows,
                                                           int band,
                                                           const float2* __restrict__ tw) {
     using Geo = Fft2Geom<P, Q>;
@@ -465,14 +470,18 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict
     });
     __syncthreads();
     if (!act) return;
-    float* o1 = img + (2 * p) * (long)L;
-    float* o2 = o1 + L;
+    const long o = (2 * p) * (long)L;
     fft2_stage2<Geo, true>(t, xb, [&](int, int x, float2 v) {
         // undo the index shift: multiply by e^{-2 pi i band x / L}
         const float2 w = twG[(int)(((long)band * x) % L)];
         const float2 r = cmul(v, w);
-        o1[x] = r.x;
-        o2[x] = r.y;
+        if (O16) {
+            ((__half*)img)[o + x] = __float2half_rn(r.x);
+            ((__half*)img)[o + L + x] = __float2half_rn(r.y);
+        } else {
+            ((float*)img)[o + x] = r.x;
+            ((float*)img)[o + L + x] = r.y;
+        }
     });
 }
 
@@ -1035,6 +1044,148 @@ int band_grid(int Nx) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// socs_wave: socs_band with every transform inside one wavefront.  A P-point
+// transform uses P lanes; a wave holds TPW = 64 / P of them (60 of 64 lanes
+// at P = 12) and NW waves make the block.  The LDS transposes inside each
+// two-stage FFT then need only a wavefront fence, so a kernel costs two block
+// barriers (around the column -> row hand-off of Y) instead of two per round.
+// Lanes past TPW * P sit out every FFT (they must not touch the slot buffers).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int P, int NZ, int NW, int BK = 0>
+__global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __restrict__ M,
+                                                            const void* __restrict__ bank,
+                                                            const float* __restrict__ kscale,
+                                                            const float* __restrict__ scales,
+                                                            int nk, int Nx,
+                                                            float2* __restrict__ Cout) {
+    using Geo = Fft2Geom<P, P>;
+    constexpr int G = Geo::G;
+    constexpr int TPW = 64 / P;
+    constexpr int SL = TPW * NW;
+    constexpr int RB = (G + SL - 1) / SL;
+    extern __shared__ float2 lds[];
+    float2* twG = lds;
+    float2* xbuf = lds + G;
+    float2* Y = xbuf + SL * Geo::XBUF;  // [G][W]
+    const int W = 2 * Nx + 1;
+    const int band = 2 * Nx;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int j = lane / P, t = lane - j * P;
+    const bool on = j < TPW;
+    const int slot = wv * TPW + (on ? j : 0);
+    const long b = blockIdx.x;
+    const float2* Mb = M + b * (long)W * W;
+    float2* xb = xbuf + slot * Geo::XBUF;
+    fill_twiddles<G>(twG, tid, 64 * NW);
+    float acc[RB * P];
+#pragma unroll
+    for (int i = 0; i < RB * P; i++) acc[i] = 0.f;
+    __syncthreads();
+
+    for (int k = 0; k < nk; k++) {
+        const long kbase = (long)k * W * W;
+        const float lam = scales[k];
+        for (int c0 = 0; c0 < W; c0 += SL) {
+            const int col = c0 + slot;
+            const bool act = on && col < W;
+            if (on)
+                fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+                    float2 v = make_float2(0.f, 0.f);
+                    if (act && n < W) {
+                        const int jj = n * W + col;
+                        v = cmul(Bank<BK>::ld(bank, kscale, kbase + jj, k), Mb[jj]);
+                    }
+                    return v;
+                });
+            wave_sync();
+            if (act)
+                fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) { Y[g * W + col] = val; });
+            wave_sync();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int r = rb * SL + slot;
+            const bool act = on && r < G;
+            const float2* yr = Y + (act ? r : 0) * W;
+            if (on)
+                fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+                    return (act && n < W) ? yr[n] : make_float2(0.f, 0.f);
+                });
+            wave_sync();
+            if (act)
+                fft2_stage2<Geo, true>(t, xb, [&](int sidx, int, float2 val) {
+                    acc[rb * P + sidx] += lam * (val.x * val.x + val.y * val.y);
+                });
+            wave_sync();
+        }
+        __syncthreads();
+    }
+    // forward R2C of the intensity rows straight from registers
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int r = rb * SL + slot;
+        const bool act = on && r < G;
+        if (on)
+            fft2_stage1_ij<Geo, P, false>(t, xb, twG, [&](int n1, int) {
+                return make_float2(acc[rb * P + n1], 0.f);
+            });
+        wave_sync();
+        if (act)
+            fft2_stage2<Geo, false>(t, xb, [&](int, int kx, float2 val) {
+                if (kx <= band) Y[r * W + kx] = val;
+            });
+        wave_sync();
+    }
+    __syncthreads();
+    // columns kx = 0..band: forward FFT over the rows, keep |ky| <= band
+    const float inv = 1.0f / ((float)G * (float)G);
+    float2* Cb = Cout + b * (long)(2 * band + 1) * W;
+    for (int c0 = 0; c0 < W; c0 += SL) {
+        const int col = c0 + slot;
+        const bool act = on && col < W;
+        if (on)
+            fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
+                return act ? Y[n * W + col] : make_float2(0.f, 0.f);
+            });
+        wave_sync();
+        if (act)
+            fft2_stage2<Geo, false>(t, xb, [&](int, int kk, float2 val) {
+                const int ky = kk <= G / 2 ? kk : kk - G;
+                if (ky >= -band && ky <= band) Cb[(ky + band) * W + col] = cscale(val, inv);
+            });
+        wave_sync();
+    }
+}
+
+template <int P, int NZ, int NW, int BK>
+static int launch_wave(const float2* M, const void* bank, const float* kscale, const float* scales,
+                       int nk, int Nx, int B, float2* C, hipStream_t st) {
+    using Geo = Fft2Geom<P, P>;
+    constexpr int SL = (64 / P) * NW;
+    const int W = 2 * Nx + 1;
+    size_t lds = (size_t)(Geo::G + SL * Geo::XBUF + Geo::G * W) * sizeof(float2);
+    if (lds > 160 * 1024) return LITHO_EARG;
+    auto kern = socs_wave_kernel<P, NZ, NW, BK>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024) != hipSuccess)
+            return LITHO_EHIP;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(B), dim3(64 * NW), lds, st, M, bank, kscale, scales, nk, Nx, C);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 template <int P, int NZ, int SLOTS, int BK>
 static int launch_band(const float2* M, const void* bank, const float* kscale, const float* scales,
                        int nk, int Nx, int B, float2* C, hipStream_t st) {
@@ -1055,11 +1206,12 @@ static int launch_band(const float2* M, const void* bank, const float* kscale, c
 }
 
 int launch_socs_band(int G, const float2* M, const void* bank, const float* kscale, int bank_fmt,
-                     const float* scales, int nk, int Nx, int B, float2* C, bool lean,
+                     const float* scales, int nk, int Nx, int B, float2* C, int shape,
                      hipStream_t st) {
     const int W = 2 * Nx + 1;
-    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 2) return LITHO_EARG;
-
+    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 2 || shape < 0 ||
+        shape > 2)
+        return LITHO_EARG;
 #define LITHO_BAND(p, nz, slots)                                                                  \
     if (G == p * p && W <= nz * p) {                                                              \
         if (bank_fmt == 1)                                                                        \
@@ -1068,7 +1220,23 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float* ksca
             return launch_band<p, nz, slots, 2>(M, bank, kscale, scales, nk, Nx, B, C, st);       \
         return launch_band<p, nz, slots, 0>(M, bank, kscale, scales, nk, Nx, B, C, st);           \
     }
-    if (lean) {
+#define LITHO_WAVE(p, nz, nw)                                                                     \
+    if (G == p * p && W <= nz * p) {                                                              \
+        if (bank_fmt == 1)                                                                        \
+            return launch_wave<p, nz, nw, 1>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
+        if (bank_fmt == 2)                                                                        \
+            return launch_wave<p, nz, nw, 2>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
+        return launch_wave<p, nz, nw, 0>(M, bank, kscale, scales, nk, Nx, B, C, st);             \
+    }
+    if (shape == 2) {
+        LITHO_WAVE(6, 6, 4)
+        LITHO_WAVE(9, 4, 8)
+        LITHO_WAVE(9, 9, 8)
+        LITHO_WAVE(12, 6, 9)
+        LITHO_WAVE(12, 12, 9)
+        return LITHO_EARG;
+    }
+    if (shape == 1) {
         LITHO_BAND(12, 6, 24)
     }
     LITHO_BAND(6, 6, 36)
@@ -1077,22 +1245,24 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float* ksca
     LITHO_BAND(12, 6, 48)
     LITHO_BAND(12, 12, 48)
 #undef LITHO_BAND
+#undef LITHO_WAVE
     return LITHO_EARG;
 }
 
-int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, int count, bool h16,
-                      bool lean, hipStream_t st) {
+int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, int count, bool h16,
+                      bool lean, bool u8, hipStream_t st) {
     if (nrows % L || nrows % 2 || count <= 0) return LITHO_EARG;
+#define LITHO_RT1(p, q, nth, hh, uu)                                                            \
+    hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true, hh, uu>), dim3(grid), dim3(nth), \
+                       0, st, in, out, (int)nrows, kmin, count, nullptr)
 #define LITHO_RT(p, q, nth)                                                                   \
     if (L == p * q) {                                                                         \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                          \
         unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                       \
-        if (h16)                                                                              \
-            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true, true>), dim3(grid),   \
-                               dim3(nth), 0, st, in, out, (int)nrows, kmin, count, nullptr);  \
-        else                                                                                  \
-            hipLaunchKernelGGL((rows_fwd2_kernel<p, q, nth, true, true, false>), dim3(grid),  \
-                               dim3(nth), 0, st, in, out, (int)nrows, kmin, count, nullptr);  \
+        if (h16 && u8) LITHO_RT1(p, q, nth, true, true);                                      \
+        else if (h16) LITHO_RT1(p, q, nth, true, false);                                      \
+        else if (u8) LITHO_RT1(p, q, nth, false, true);                                       \
+        else LITHO_RT1(p, q, nth, false, false);                                              \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
     }
     LITHO_RT(16, 16, 256)
@@ -1103,6 +1273,7 @@ int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, i
     }
     LITHO_RT(32, 32, 256)
 #undef LITHO_RT
+#undef LITHO_RT1
     return LITHO_EARG;
 }
 
@@ -1160,19 +1331,20 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
     return LITHO_EARG;
 }
 
-int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, bool h16,
-                        bool lean, hipStream_t st) {
+int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, bool h16,
+                        bool lean, bool out16, hipStream_t st) {
     if (nrows % N || nrows % 2 || 2 * band + 1 > N) return LITHO_EARG;
+#define LITHO_IT1(p, q, nz, nth, hh, oo)                                                         \
+    hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true, hh, oo>), dim3(grid), dim3(nth), \
+                       0, st, U, img, (int)nrows, band, nullptr)
 #define LITHO_IT(p, q, nz, nth)                                                                \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                                \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
         unsigned grid = cdiv((size_t)(nrows / 2), FPB);                                        \
-        if (h16)                                                                               \
-            hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true, true>), dim3(grid),    \
-                               dim3(nth), 0, st, U, img, (int)nrows, band, nullptr);           \
-        else                                                                                   \
-            hipLaunchKernelGGL((image_rows2_kernel<p, q, nz, nth, true, false>), dim3(grid),   \
-                               dim3(nth), 0, st, U, img, (int)nrows, band, nullptr);           \
+        if (h16 && out16) LITHO_IT1(p, q, nz, nth, true, true);                                \
+        else if (h16) LITHO_IT1(p, q, nz, nth, true, false);                                   \
+        else if (out16) LITHO_IT1(p, q, nz, nth, false, true);                                 \
+        else LITHO_IT1(p, q, nz, nth, false, false);                                           \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     LITHO_IT(16, 16, 3, 256)
@@ -1187,6 +1359,7 @@ int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, 
     LITHO_IT(32, 32, 5, 256)
     LITHO_IT(32, 32, 32, 256)
 #undef LITHO_IT
+#undef LITHO_IT1
     return LITHO_EARG;
 }
 

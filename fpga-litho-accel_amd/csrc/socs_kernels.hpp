@@ -31,21 +31,25 @@ int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, 
 // smallest such grid >= 4Nx+1, or 0 when none applies.
 int band_grid(int Nx);
 // mask rows -> transposed row-DFT window R [B][cx][N] (window kx in [kmin, kmin+count))
-int launch_rows_dft_t(int L, const float* in, void* out, long nrows, int kmin, int count, bool h16,
-                      bool lean, hipStream_t st);
+// in: f32 masks, or uint8 levels (mask = level / 255) when u8
+int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, int count, bool h16,
+                      bool lean, bool u8, hipStream_t st);
 // R [B][cx][N] -> M [B][cy][cx]
 int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
                        bool lean, hipStream_t st);
 // M x bank -> band spectrum C [B][4Nx+1][2Nx+1] of the intensity, /G^2.
 // bank_fmt 0: float2; 1: bf16 (re, im) pairs x kscale[k]; 2: fp16 pairs x kscale[k].
+// shape 0: 48-slot blocks, 1: 24-slot (lean) blocks at G = 144, 2: wave-synced
+// transforms (socs_wave_kernel)
 int launch_socs_band(int G, const float2* M, const void* bank, const float* kscale, int bank_fmt,
-                     const float* scales, int nk, int Nx, int B, float2* C, bool lean,
+                     const float* scales, int nk, int Nx, int B, float2* C, int shape,
                      hipStream_t st);
 // C -> U [B][2Nx+1][N]
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
                      hipStream_t st);
 // U [B][band+1][N] -> image [B*N][N]
-int launch_image_rows_t(int N, const void* U, float* img, long nrows, int band, bool h16,
-                        bool lean, hipStream_t st);
+// img: f32, or fp16 when out16
+int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, bool h16,
+                        bool lean, bool out16, hipStream_t st);
 
 }  // namespace litho

@@ -36,6 +36,7 @@ EXPORTS = [
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
     "litho_tcc_build_prec", "litho_kernels_from_source_prec",
     "litho_socs_image_buckets", "litho_bucket_times", "litho_autotune", "litho_plan_get",
+    "litho_plan_set",
     "litho_fno_create", "litho_fno_destroy", "litho_fno_param_count", "litho_fno_get_params",
     "litho_fno_set_params", "litho_fno_forward", "litho_fno_loss_grad", "litho_fno_adam",
 ]
@@ -126,6 +127,7 @@ def lib():
         L.litho_bucket_times.argtypes = [vp, _f32p, i]
         L.litho_autotune.argtypes = [vp, bp, i, i, i]
         L.litho_plan_get.argtypes = [vp, i, i, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+        L.litho_plan_set.argtypes = [vp, i, i, i, i, i, i, i]
         L.litho_tcc_build_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p, _f64p]
         L.litho_kernels_from_source_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, i,
                                                      _f64p, _f32p, _f32p, C.POINTER(i)]
@@ -397,13 +399,17 @@ class Context:
         _check(min(n, 0), "litho_autotune")
         return n
 
+    def set_plan(self, N, Nx, nk, precision, G, socs_shape, lean_fft):
+        _check(lib().litho_plan_set(self.h, N, Nx, nk, precision, G, socs_shape, int(lean_fft)),
+               "litho_plan_set")
+
     def plan(self, N, Nx, nk, precision=0):
-        """(conv grid, lean SOCS block, lean FFT block, tuned?) for a shape."""
+        """(conv grid, SOCS block shape 0/1/2, lean FFT block, tuned?) for a shape."""
         g, ls, lm = C.c_int(), C.c_int(), C.c_int()
         t = lib().litho_plan_get(self.h, N, Nx, nk, precision, C.byref(g), C.byref(ls),
                                  C.byref(lm))
         _check(min(t, 0), "litho_plan_get")
-        return g.value, bool(ls.value), bool(lm.value), bool(t)
+        return g.value, int(ls.value), bool(lm.value), bool(t)
 
     def set_path(self, path):
         _check(lib().litho_ctx_set_path(self.h, path), "litho_ctx_set_path")

@@ -229,13 +229,19 @@ int litho_socs_image_buckets(litho_ctx* ctx, const LithoBucket* buckets, int nbu
 int litho_bucket_times(litho_ctx* ctx, float* ms, int nbuckets);
 /* Autotuner: for every bucket shape (N, Nx, nk, precision) not tuned yet on
  * this context, time each candidate plan -- conv grid 36/81/144 (SOCS FFT
- * radix 6/9/12), SOCS block of 48 or 24 kernel slots, FFT row/column blocks
- * of 256 or 128 threads -- on up to 32 of the bucket's clips (their images
- * are written) and keep the fastest.  Returns the number of shapes tuned. */
+ * radix 6/9/12), SOCS block shape (0: 48 transform slots, 1: 24 slots at
+ * grid 144, 2: transforms aligned to wavefronts, wave-level syncs), FFT
+ * row/column blocks of 256 or 128 threads -- on up to 32 of the bucket's clips
+ * (their images are written) and keep the fastest.  Returns the number of
+ * shapes tuned. */
 int litho_autotune(litho_ctx* ctx, const LithoBucket* buckets, int nbuckets, int precision,
                    int reps);
-/* The plan used for a shape: returns 1 if tuned, 0 for the default. */
-int litho_plan_get(litho_ctx* ctx, int N, int Nx, int nk, int precision, int* G, int* lean_socs,
+/* Pin the plan of a shape without tuning (same candidate set; LITHO_EARG for
+ * a combination that is not built). */
+int litho_plan_set(litho_ctx* ctx, int N, int Nx, int nk, int precision, int G, int socs_shape,
+                   int lean_mem);
+/* The plan used for a shape: returns 1 if tuned or set, 0 for the default. */
+int litho_plan_get(litho_ctx* ctx, int N, int Nx, int nk, int precision, int* G, int* socs_shape,
                    int* lean_mem);
 
 /* ---- learned surrogate (no counterpart in the reference) ------------------

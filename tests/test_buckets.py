@@ -78,14 +78,35 @@ def test_autotuned_plans_keep_parity(gpu, orc, setup):
     assert n == 4
     assert ctx.autotune(bl, 0, reps=2) == 0  # already tuned
     for (N, Nx, sc, K, masks) in host:
-        G, ls, lm, tuned = ctx.plan(N, Nx, 16, 0)
-        print(f"[plan N={N} Nx={Nx}] grid {G} lean_socs {ls} lean_fft {lm}", flush=True)
+        G, shape, lm, tuned = ctx.plan(N, Nx, 16, 0)
+        print(f"[plan N={N} Nx={Nx}] grid {G} socs shape {shape} lean_fft {lm}", flush=True)
         assert tuned and G >= 4 * Nx + 1
     ctx.socs_image_buckets(bl, 0, True)
     ctx.sync()
     for (N, Nx, sc, K, masks), (_, _, c, _, di) in zip(host, bl):
         g = ctx.d2h(np.zeros((c, N, N), np.float32), di)
         assert max_rel(g[0], oracle_image(orc, N, Nx, sc, K, masks[0])) < 1e-5
+
+
+def test_every_plan_matches_oracle(gpu, orc, setup):
+    """Each conv grid and SOCS block shape (incl. the wave-synced transforms)
+    gives the FP64 image to 1e-5 on every clip size."""
+    ctx, buckets, host = setup
+    print("[case] every band plan", flush=True)
+    for (b, N, c, dm, di, _), (_, Nx, sc, K, masks) in zip(buckets, host):
+        ref = oracle_image(orc, N, Nx, sc, K, masks[0])
+        for G in (36, 81, 144):
+            for shape in (0, 1, 2):
+                for lean in (0, 1):
+                    try:
+                        ctx.set_plan(N, Nx, 16, 0, G, shape, lean)
+                    except gpu.LithoError:
+                        continue  # not a built combination for this shape
+                    ctx.socs_image_buckets([(b, N, c, dm, di)], 0, False)
+                    g = ctx.d2h(np.zeros((c, N, N), np.float32), di)
+                    e = max_rel(g[0], ref)
+                    print(f"[plan N={N} G={G} shape={shape} lean={lean}] {e:.2e}", flush=True)
+                    assert e < 1e-5, (N, G, shape, lean)
 
 
 def test_buckets_matrix_core_form(gpu, orc, setup):

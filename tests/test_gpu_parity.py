@@ -310,11 +310,12 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
 
 
 @pytest.mark.parametrize("N,prec", [(1024, 32), (1024, 8 | 32 | 1 | 4), (1024, 8 | 32 | 4 | 16),
-                                    (512, 8 | 32 | 1 | 4), (256, 8 | 32)])
+                                    (512, 8 | 32 | 1 | 4), (256, 8 | 32), (1024, 8 | 4 | 16),
+                                    (768, 8 | 16), (512, 8 | 1 | 4 | 16)])
 def test_matrix_core_forms(gpu, orc, base_source, N, prec):
-    """Band DFTs on the fp16 matrix cores (u8 / f32 masks, f32 / fp16 images)
-    stay within 1e-3 of FP64 on binary and 8-bit grey masks; graph replays
-    reproduce the eager result bit for bit."""
+    """u8 / f32 masks and f32 / fp16 images, with the band DFTs on the fp16
+    matrix cores (flag 32) or as FP32 FFTs, stay within 1e-3 of FP64 on binary
+    and 8-bit grey masks; graph replays reproduce the eager result bit for bit."""
     case(f"matrix-core form N={N} precision={prec}")
     src, s = base_source
     Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
@@ -363,12 +364,12 @@ def test_matrix_core_forms(gpu, orc, base_source, N, prec):
 
 
 def test_matrix_core_flag_rules(gpu, c1):
-    """u8 masks and fp16 images need the matrix-core DFTs; unknown flags fail."""
+    """Unknown flags and two bank formats at once fail."""
     ctx = gpu.Context()
     try:
         bank = ctx.bank(c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
         d = ctx.alloc(4 * 512 * 512)
-        for bad in (8, 16, 8 | 16, 64, 2 | 4 | 32):
+        for bad in (64, 128, 2 | 4, 2 | 4 | 32):
             with pytest.raises(gpu.LithoError):
                 ctx.socs_image_batch_pipelined(bank, d, 1, 512, d, bad, 1, False)
         ctx.free(d)
