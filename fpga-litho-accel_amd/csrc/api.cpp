@@ -852,7 +852,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, W, -Nx, W, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[2], st));
         LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, nullptr, 0, bank->d_scales,
-                                   bank->nk, Nx, B, (float2*)C, false, st));
+                                   bank->nk, Nx, B, (float2*)C, 2, st));
         LITHO_HIP(hipEventRecord(c->ev[3], st));
         LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[4], st));
@@ -936,7 +936,7 @@ static int ensure_pipeline(litho_ctx* c, int nstreams) {
 // ---------------------------------------------------------------------------
 struct BandPlan {
     int G = 0;
-    int socs_shape = 0;  // 0: 48-slot blocks, 1: 24-slot blocks, 2: wave-synced transforms
+    int socs_shape = 2;  // 0: 48-slot blocks, 1: 24-slot blocks, 2: wave-synced transforms
     bool lean_mem = false;
 };
 
@@ -1133,10 +1133,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
             wk.M = (float2*)w[0].M + (size_t)b0 * W * W;
             wk.C = (float2*)w[0].C + (size_t)b0 * (2 * band + 1) * (band + 1);
             BandPlan pl = tuned;
-            if (chunks > 1) {
-                if (pl.G == 144 && pl.socs_shape == 0) pl.socs_shape = 1;
-                pl.lean_mem = true;
-            }
+            if (chunks > 1) pl.lean_mem = true;
             LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN * mb, nb, N,
                                    (char*)d_image + (size_t)b0 * NN * ib, precision, pl, wk, st));
         }

@@ -84,8 +84,10 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
     constexpr int CH = 4 * NT * 64;  // uint4 per K chunk (4 k-steps)
     constexpr int PER = CH / NTH;
     static_assert(CH % NTH == 0, "chunk copy");
-    // double-buffered table chunk: chunk c+1 is fetched while chunk c is used
-    __shared__ u4v fs[2][CH];
+    // double-buffered table chunk: chunk c+1 is fetched while chunk c is used.
+    // Two separate arrays (not one [2][CH]) let the compiler see that reads of
+    // one buffer never alias the DMA into the other, so it need not wait.
+    __shared__ u4v fsA[CH], fsB[CH];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
@@ -108,20 +110,30 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
             raw[rt][1] = p[1];
         }
     };
-    u4v raw[RT][2], rawn[RT][2];
-    u4v pre[PER];
-    if (U8) load_u8(0, raw);
+    // two register sets for the u8 fragments (ping-pong, no loop-carried
+    // copy): chunk c+1 loads into one set while chunk c computes from the other
+    u4v rawA[RT][2], rawB[RT][2];
+    // table chunk c -> dst straight from global memory into LDS (16 bytes per
+    // lane, the wave's 64 lanes contiguous), so no registers hold it
+    auto load_table = [&](int c, u4v* dst) {
 #pragma unroll
-    for (int i = 0; i < PER; i++) fs[0][tid + NTH * i] = Fpk[tid + NTH * i];
-    __syncthreads();
+        for (int i = 0; i < PER; i++)
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(Fpk + (long)c * CH + NTH * i + tid),
+                (__attribute__((address_space(3))) void*)(dst + NTH * i + 64 * w), 16, 0, 0);
+    };
+    // N is a multiple of ROWS_PER_BLOCK (checked by the launcher), so there
+    // are at least two chunks; stating it keeps the first chunk's loads ahead
+    // of the barrier below (a guarded loop would let them sink past it and
+    // leave them pending at the loop head, costing a full wait per pass)
     const int nch = N / KCH;
-    for (int c = 0; c < nch; c++) {
-        const bool more = c + 1 < nch;
-        if (more) {
-            if (U8) load_u8(c + 1, rawn);
-#pragma unroll
-            for (int i = 0; i < PER; i++) pre[i] = Fpk[(long)(c + 1) * CH + tid + NTH * i];
-        }
+    __builtin_assume(nch >= ROWS_PER_BLOCK / KCH);
+    load_table(0, fsA);
+    if (U8) load_u8(0, rawA);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    auto chunk = [&](int c, const u4v (&raw)[RT][2], const u4v* fb, u4v* fnext) {
+        if (c + 1 < nch) load_table(c + 1, fnext);
         u4v af[RT][4];
         if (!U8) {
 #pragma unroll
@@ -136,7 +148,6 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
                 }
             }
         }
-        const u4v* fb = fs[c & 1];
         // B fragments one (k-step, tile) ahead: each LDS read overlaps the
         // MFMAs of the previous fragment
         u4v bnext = fb[lane];
@@ -161,18 +172,17 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
                     acc[rt][t] = mfma16(__builtin_bit_cast(h8, as[rt]), bf, acc[rt][t]);
             }
         }
-        if (more) {
-#pragma unroll
-            for (int i = 0; i < PER; i++) fs[(c + 1) & 1][tid + NTH * i] = pre[i];
-            if (U8) {
-#pragma unroll
-                for (int rt = 0; rt < RT; rt++) {
-                    raw[rt][0] = rawn[rt][0];
-                    raw[rt][1] = rawn[rt][1];
-                }
-            }
-        }
+        // the next chunk's table (and u8 fragments) have landed before any
+        // wave reads them; every wave is done with fb before it is refilled
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+    };
+    for (int c = 0; c < nch; c += 2) {
+        if (U8 && c + 1 < nch) load_u8(c + 1, rawB);
+        chunk(c, rawA, fsA, fsB);
+        if (c + 1 >= nch) break;
+        if (U8 && c + 2 < nch) load_u8(c + 2, rawA);
+        chunk(c + 1, rawB, fsB, fsA);
     }
     // D fragment: rows 4g + i of the tile, column r16 = frequency 16 fg + r16
 #pragma unroll

@@ -1232,8 +1232,8 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float* ksca
         LITHO_WAVE(6, 6, 4)
         LITHO_WAVE(9, 4, 8)
         LITHO_WAVE(9, 9, 8)
-        LITHO_WAVE(12, 6, 9)
-        LITHO_WAVE(12, 12, 9)
+        LITHO_WAVE(12, 6, 8)
+        LITHO_WAVE(12, 12, 8)
         return LITHO_EARG;
     }
     if (shape == 1) {

@@ -213,7 +213,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* ctx, const litho_bank* bank, con
  * use_graph, captured once as one HIP graph and replayed.  Precision flags as
  * litho_socs_image_batch_pipelined; N in {256, 512, 768, 1024}.  Each bucket
  * runs under the plan litho_autotune chose for its shape (default: the
- * smallest band grid, 48-slot SOCS blocks, 256-thread FFT blocks). */
+ * smallest band grid, wave-synced SOCS blocks, 256-thread FFT blocks). */
 typedef struct {
     const litho_bank* bank;  /* kernel set of this clip size */
     int N;                   /* clip size in pixels */

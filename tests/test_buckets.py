@@ -131,5 +131,8 @@ def test_bucket_argument_checks(gpu, setup):
     with pytest.raises(gpu.LithoError):
         ctx.socs_image_buckets([(b, N, -1, dm, di)], 0, False)
     with pytest.raises(gpu.LithoError):
-        ctx.socs_image_buckets([(b, N, c, dm, di)], gpu.LITHO_PREC_MASK_U8, False)
+        ctx.socs_image_buckets([(b, N, c, dm, di)], 64, False)  # unknown precision flag
+    with pytest.raises(gpu.LithoError):  # two bank formats at once
+        ctx.socs_image_buckets([(b, N, c, dm, di)],
+                               gpu.LITHO_PREC_BANK_BF16 | gpu.LITHO_PREC_BANK_FP16, False)
     ctx.socs_image_buckets([(b, N, 0, 0, 0)], 0, False)  # empty bucket: nothing to do
