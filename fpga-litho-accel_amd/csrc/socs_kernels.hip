@@ -47,30 +47,47 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
     return __uint_as_float(((unsigned)h) << 16);
 }
 
-// BK = 0: float2; 1: bf16 pairs x kscale[k]; 2: fp16 pairs (half2) x kscale[k]
+// BK = 0: float2; 1: bf16 pairs x kscale[k]; 2: fp16 pairs (half2) x kscale[k].
+// ld = dec(raw(...)); raw / dec apart let a kernel prefetch the stored word.
 template <int BK>
 struct Bank;
 template <>
 struct Bank<0> {
-    __device__ static __forceinline__ float2 ld(const void* b, const float*, long idx, int) {
+    using R = float2;
+    __device__ static __forceinline__ R raw(const void* b, long idx) {
         return ((const float2*)b)[idx];
+    }
+    __device__ static __forceinline__ float2 dec(R v, float) { return v; }
+    __device__ static __forceinline__ float2 ld(const void* b, const float*, long idx, int) {
+        return raw(b, idx);
     }
 };
 template <>
 struct Bank<1> {
-    __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
-        const unsigned v = ((const unsigned*)b)[idx];
-        const float s = ks[k];
+    using R = unsigned;
+    __device__ static __forceinline__ R raw(const void* b, long idx) {
+        return ((const unsigned*)b)[idx];
+    }
+    __device__ static __forceinline__ float2 dec(R v, float s) {
         return make_float2(bf16_to_f32((unsigned short)(v & 0xffffu)) * s,
                            bf16_to_f32((unsigned short)(v >> 16)) * s);
+    }
+    __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
+        return dec(raw(b, idx), ks[k]);
     }
 };
 template <>
 struct Bank<2> {
+    using R = unsigned;
+    __device__ static __forceinline__ R raw(const void* b, long idx) {
+        return ((const unsigned*)b)[idx];
+    }
+    __device__ static __forceinline__ float2 dec(R v, float s) {
+        const float2 f = __half22float2(__builtin_bit_cast(__half2, v));
+        return make_float2(f.x * s, f.y * s);
+    }
     __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
-        const float2 v = __half22float2(((const __half2*)b)[idx]);
-        const float s = ks[k];
-        return make_float2(v.x * s, v.y * s);
+        return dec(raw(b, idx), ks[k]);
     }
 };
 
@@ -1051,11 +1068,29 @@ int band_grid(int Nx) {
 // two-stage FFT then need only a wavefront fence, so a kernel costs two block
 // barriers (around the column -> row hand-off of Y) instead of two per round.
 // Lanes past TPW * P sit out every FFT (they must not touch the slot buffers).
+// What a lane needs in every kernel stays in registers: its inter-stage
+// twiddles W_G^(t k1) (lane t is stage-1 index n2 = t), and its mask-spectrum
+// inputs of the column pass; the next kernel's bank words are fetched while
+// the current kernel runs.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stage 1 of a P x P two-stage FFT for the lane's n2 = t, inputs v[n1] in
+// registers (zeros past NZ fold away), twiddles twr[k1] = W_G^(t k1) forward.
+template <int P, bool INV>
+__device__ __forceinline__ void stage1_regs(int t, float2 (&v)[P], const float2 (&twr)[P],
+                                            float2* __restrict__ xb) {
+    dft_reg<P, INV>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) {
+        float2 w = twr[k1];
+        if (INV) w.y = -w.y;
+        xb[k1 * (P + 1) + t] = k1 == 0 ? v[0] : cmul_w<false>(v[k1], w);
+    }
 }
 
 template <int P, int NZ, int NW, int BK = 0>
@@ -1066,10 +1101,14 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
                                                             int nk, int Nx,
                                                             float2* __restrict__ Cout) {
     using Geo = Fft2Geom<P, P>;
+    static_assert(Geo::LDQ == P + 1 && Geo::S1 == 1 && Geo::S2 == 1, "square two-stage FFT");
     constexpr int G = Geo::G;
     constexpr int TPW = 64 / P;
     constexpr int SL = TPW * NW;
     constexpr int RB = (G + SL - 1) / SL;
+    constexpr int WMAX = 2 * ((G - 1) / 4) + 1;  // widest band of a G grid (4 Nx + 1 <= G)
+    constexpr int CR = (WMAX + SL - 1) / SL;     // column rounds at most
+    using BR = typename Bank<BK>::R;
     extern __shared__ float2 lds[];
     float2* twG = lds;
     float2* xbuf = lds + G;
@@ -1088,23 +1127,56 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
     float acc[RB * P];
 #pragma unroll
     for (int i = 0; i < RB * P; i++) acc[i] = 0.f;
+    // column-pass inputs of this lane: round cr, column cr * SL + slot, rows
+    // n = P n1 + t (n1 < NZ); zero where the column or row is outside the band
+    float2 mreg[CR][NZ];
+    BR braw[CR][NZ];
+#pragma unroll
+    for (int cr = 0; cr < CR; cr++)
+#pragma unroll
+        for (int n1 = 0; n1 < NZ; n1++) {
+            const int col = cr * SL + slot, n = P * n1 + t;
+            const bool ok = on && col < W && n < W;
+            mreg[cr][n1] = ok ? Mb[n * W + col] : make_float2(0.f, 0.f);
+            braw[cr][n1] = BR{};
+        }
+    auto load_bank = [&](int k) {
+        const long kbase = (long)k * W * W;
+#pragma unroll
+        for (int cr = 0; cr < CR; cr++)
+#pragma unroll
+            for (int n1 = 0; n1 < NZ; n1++) {
+                const int col = cr * SL + slot, n = P * n1 + t;
+                if (on && col < W && n < W) braw[cr][n1] = Bank<BK>::raw(bank, kbase + n * W + col);
+            }
+    };
+    if (nk > 0) load_bank(0);
     __syncthreads();
+    float2 twr[P];
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) twr[k1] = twG[t * k1];  // t, k1 < P: index < G
 
     for (int k = 0; k < nk; k++) {
-        const long kbase = (long)k * W * W;
         const float lam = scales[k];
-        for (int c0 = 0; c0 < W; c0 += SL) {
-            const int col = c0 + slot;
+        const float ksc = BK ? kscale[k] : 1.f;
+        float2 xin[CR][NZ];
+#pragma unroll
+        for (int cr = 0; cr < CR; cr++)
+#pragma unroll
+            for (int n1 = 0; n1 < NZ; n1++)
+                xin[cr][n1] = cmul(Bank<BK>::dec(braw[cr][n1], ksc), mreg[cr][n1]);
+        if (k + 1 < nk) load_bank(k + 1);
+#pragma unroll
+        for (int cr = 0; cr < CR; cr++) {
+            if (cr * SL >= W) break;
+            const int col = cr * SL + slot;
             const bool act = on && col < W;
-            if (on)
-                fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
-                    float2 v = make_float2(0.f, 0.f);
-                    if (act && n < W) {
-                        const int jj = n * W + col;
-                        v = cmul(Bank<BK>::ld(bank, kscale, kbase + jj, k), Mb[jj]);
-                    }
-                    return v;
-                });
+            if (on) {
+                float2 v[P];
+#pragma unroll
+                for (int n1 = 0; n1 < P; n1++) v[n1] = n1 < NZ ? xin[cr][n1] : make_float2(0.f, 0.f);
+                stage1_regs<P, true>(t, v, twr, xb);
+            }
             wave_sync();
             if (act)
                 fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) { Y[g * W + col] = val; });
@@ -1116,10 +1188,15 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
             const int r = rb * SL + slot;
             const bool act = on && r < G;
             const float2* yr = Y + (act ? r : 0) * W;
-            if (on)
-                fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
-                    return (act && n < W) ? yr[n] : make_float2(0.f, 0.f);
-                });
+            if (on) {
+                float2 v[P];
+#pragma unroll
+                for (int n1 = 0; n1 < P; n1++) {
+                    const int n = P * n1 + t;
+                    v[n1] = (n1 < NZ && act && n < W) ? yr[n] : make_float2(0.f, 0.f);
+                }
+                stage1_regs<P, true>(t, v, twr, xb);
+            }
             wave_sync();
             if (act)
                 fft2_stage2<Geo, true>(t, xb, [&](int sidx, int, float2 val) {
@@ -1134,10 +1211,12 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
     for (int rb = 0; rb < RB; rb++) {
         const int r = rb * SL + slot;
         const bool act = on && r < G;
-        if (on)
-            fft2_stage1_ij<Geo, P, false>(t, xb, twG, [&](int n1, int) {
-                return make_float2(acc[rb * P + n1], 0.f);
-            });
+        if (on) {
+            float2 v[P];
+#pragma unroll
+            for (int n1 = 0; n1 < P; n1++) v[n1] = make_float2(acc[rb * P + n1], 0.f);
+            stage1_regs<P, false>(t, v, twr, xb);
+        }
         wave_sync();
         if (act)
             fft2_stage2<Geo, false>(t, xb, [&](int, int kx, float2 val) {
@@ -1152,10 +1231,13 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
     for (int c0 = 0; c0 < W; c0 += SL) {
         const int col = c0 + slot;
         const bool act = on && col < W;
-        if (on)
-            fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
-                return act ? Y[n * W + col] : make_float2(0.f, 0.f);
-            });
+        if (on) {
+            float2 v[P];
+#pragma unroll
+            for (int n1 = 0; n1 < P; n1++)
+                v[n1] = act ? Y[(P * n1 + t) * W + col] : make_float2(0.f, 0.f);
+            stage1_regs<P, false>(t, v, twr, xb);
+        }
         wave_sync();
         if (act)
             fft2_stage2<Geo, false>(t, xb, [&](int, int kk, float2 val) {
@@ -1233,7 +1315,6 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float* ksca
         LITHO_WAVE(9, 4, 8)
         LITHO_WAVE(9, 9, 8)
         LITHO_WAVE(12, 6, 8)
-        LITHO_WAVE(12, 12, 8)
         return LITHO_EARG;
     }
     if (shape == 1) {
