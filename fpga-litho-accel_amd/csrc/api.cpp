@@ -840,16 +840,17 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     if (c->path == LITHO_PATH_AUTO && Gb && Gb <= N && N >= 256 && N <= 1024) {
         // band pipeline: 5 launches
         void *R, *M, *C, *U;
-        LITHO_TRY(ctx_ws(c, 0, (size_t)B * W * N * sizeof(float2), &R));
+        LITHO_TRY(ctx_ws(c, 0, (size_t)B * (Nx + 1) * N * sizeof(float2), &R));
         LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
         LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
         LITHO_TRY(ctx_ws(c, 4, (size_t)B * (band + 1) * N * sizeof(float2), &U));
         static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
                                       "image_rows"};
         LITHO_HIP(hipEventRecord(c->ev[0], st));
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, -Nx, W, false, false, false, st));
+        // a real mask: kx = 0..Nx of the row spectra, the rest by symmetry
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, 0, Nx + 1, false, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[1], st));
-        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, W, -Nx, W, false, false, st));
+        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
         LITHO_HIP(hipEventRecord(c->ev[2], st));
         LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, nullptr, 0, bank->d_scales,
                                    bank->nk, Nx, B, (float2*)C, 2, st));
@@ -985,14 +986,14 @@ struct BandWs {
     void *R = nullptr, *U = nullptr, *M = nullptr, *C = nullptr;
 };
 
-// four workspaces (row window R, mask window M, intensity band C, FI column
-// output U) for B clips in the given slots
+// four workspaces (row window R: kx = 0..Nx of each row, mask window M,
+// intensity band C, FI column output U) for B clips in the given slots
 static int band_ws(DevBuf* slots, int B, int N, int Nx, int prec, BandWs* w) {
     const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
     const bool mfma = (prec & LITHO_PREC_DFT_MFMA) != 0;
     const size_t W = 2 * Nx + 1, band = 2 * Nx;
     const size_t es = mixed ? 4 : sizeof(float2), ues = (mixed && !mfma) ? 4 : sizeof(float2);
-    LITHO_TRY(grow(slots[0], (size_t)B * W * N * es, &w->R));
+    LITHO_TRY(grow(slots[0], (size_t)B * (Nx + 1) * N * es, &w->R));
     LITHO_TRY(grow(slots[1], (size_t)B * W * W * sizeof(float2), &w->M));
     LITHO_TRY(grow(slots[2], (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &w->C));
     LITHO_TRY(grow(slots[3], (size_t)B * (band + 1) * N * ues, &w->U));
@@ -1000,9 +1001,17 @@ static int band_ws(DevBuf* slots, int B, int N, int Nx, int prec, BandWs* w) {
 }
 
 // matrix-core tables of one clip shape (uploaded outside any capture)
+// the matrix-core tables of a shape: mask rows kx = 0..Nx (the other half
+// follows from the real mask), image rows kx = 0..2Nx
+static std::tuple<int, int, int, int> mask_table_key(int N, int Nx) { return {0, N, 0, Nx + 1}; }
+static std::tuple<int, int, int, int> image_table_key(int N, int Nx) {
+    return {1, N, 2 * Nx + 1, 0};
+}
+
 static int mfma_tables(litho_ctx* c, int N, int Nx, const void** tm, const void** ti) {
-    LITHO_TRY(dft_table(c, 0, N, -Nx, 2 * Nx + 1, tm));
-    return dft_table(c, 1, N, 2 * Nx + 1, 0, ti);
+    const auto km = mask_table_key(N, Nx), ki = image_table_key(N, Nx);
+    LITHO_TRY(dft_table(c, 0, N, std::get<2>(km), std::get<3>(km), tm));
+    return dft_table(c, 1, N, std::get<2>(ki), std::get<3>(ki), ti);
 }
 
 // the five launches of one same-size batch on stream st
@@ -1020,18 +1029,19 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     const float* ks = bfmt ? bank->d_kscale : nullptr;
     const void *tm = nullptr, *ti = nullptr;
     if (mfma) {
-        auto a = c->tables.find(std::make_tuple(0, N, -Nx, W));
-        auto b = c->tables.find(std::make_tuple(1, N, band + 1, 0));
+        auto a = c->tables.find(mask_table_key(N, Nx));
+        auto b = c->tables.find(image_table_key(N, Nx));
         if (a == c->tables.end() || b == c->tables.end()) return LITHO_EARG;
         tm = a->second.p;
         ti = b->second.p;
     }
     if (mfma)
-        LITHO_TRY(launch_mask_rows_mfma(N, d_mask, u8, tm, w.R, B, W, mixed, st));
+        LITHO_TRY(launch_mask_rows_mfma(N, d_mask, u8, tm, w.R, B, Nx + 1, mixed, st));
     else
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, w.R, (long)B * N, -Nx, W, mixed, pl.lean_mem, u8,
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, w.R, (long)B * N, 0, Nx + 1, mixed, pl.lean_mem, u8,
                                     st));
-    LITHO_TRY(launch_mask_cols_t(N, w.R, (float2*)w.M, B, W, -Nx, W, mixed, pl.lean_mem, st));
+    LITHO_TRY(
+        launch_mask_cols_t(N, w.R, (float2*)w.M, B, Nx + 1, -Nx, W, mixed, pl.lean_mem, st, W));
     LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, B,
                                (float2*)w.C, pl.socs_shape, st));
     if (mfma) {
@@ -1422,8 +1432,8 @@ int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double
     LITHO_TRY(dI.alloc(NN * sizeof(float)));
     LITHO_HIP(hipMemcpyAsync(dm.p, mask, NN * sizeof(float), hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(dT.p, ht.data(), ht.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, -Nx, W, false, false, false, st));
-    LITHO_TRY(launch_mask_cols_t(N, dR.p, dM.as<float2>(), 1, W, -Nx, W, false, false, st));
+    LITHO_TRY(launch_rows_dft_t(N, dm.as<float>(), dR.p, N, 0, Nx + 1, false, false, false, st));
+    LITHO_TRY(launch_mask_cols_t(N, dR.p, dM.as<float2>(), 1, Nx + 1, -Nx, W, false, false, st, W));
     LITHO_TRY(launch_tcc_image_spectrum(dM.as<float2>(), dT.as<double2>(), Nx, Ny,
                                         dC.as<float2>(), st));
     LITHO_TRY(launch_fi_cols_t(N, dC.as<float2>(), dU.p, 1, band, false, false, st));

@@ -306,13 +306,15 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
                 }
             }
             wave_sync_lds();
-            // 64 rows x 128 bytes: 16 bytes per lane, 8 rows per store
+            // 64 rows x 128 bytes: 16 bytes per lane, 8 rows per store; the
+            // image is not read again here, so the stores stream past the
+            // caches (nontemporal)
             const long x0 = (long)c * XCH + 16 * TPS * tp;
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int row = 8 * q + (lane >> 3), seg = (lane & 7) * 16;
-                const uint4 v = *(const uint4*)(sw + row * SROW + seg);
-                *(uint4*)(out + ((long)row * N + x0) * OB + seg) = v;
+                const u4v v = *(const u4v*)(sw + row * SROW + seg);
+                __builtin_nontemporal_store(v, (u4v*)(out + ((long)row * N + x0) * OB + seg));
             }
             wave_sync_lds();
         }

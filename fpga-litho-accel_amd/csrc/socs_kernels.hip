@@ -610,10 +610,14 @@ __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __re
 
 // mask_cols2: R [B][cx][N] (row-DFT window, transposed) -> M [B][cy][cx],
 // M[b][j][c] = (1/N^2) (-1)^ky DFT_N(R[b][c][:])[ky mod N], ky = kymin + j.
+// Mirror mode (mw = 2h + 1 > 0, cx = h + 1, kymin = -h, cy = mw): R holds only
+// kx = 0..h of a real mask's spectrum, and M [B][mw][mw] gets both halves
+// through M(-ky, -kx) = conj(M(ky, kx)) (the (-1)^(kx+ky) centring is real);
+// the kx = 0 column is computed for every ky, so it is written once.
 template <int P, int Q, int NTH, bool H16 = false>
 __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict__ R_,
                                                          float2* __restrict__ M, int cx,
-                                                         int kymin, int cy) {
+                                                         int kymin, int cy, int mw) {
     using Geo = Fft2Geom<P, Q>;
     constexpr int N = Geo::G;
     constexpr int T = Geo::T;
@@ -636,11 +640,20 @@ __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict_
     __syncthreads();
     if (!act) return;
     const float inv = 1.0f / ((float)N * (float)N);
-    float2* Mb = M + b * (long)cy * cx;
+    float2* Mb = M + b * (long)cy * (mw ? mw : cx);
+    const int h = mw >> 1;
     fft2_stage2<Geo, false>(t, xb, [&](int, int k, float2 val) {
         const int ky = k < N / 2 ? k : k - N;
         const int j = ky - kymin;
-        if (j >= 0 && j < cy) Mb[(long)j * cx + col] = cscale(val, (ky & 1) ? -inv : inv);
+        if (j >= 0 && j < cy) {
+            const float2 v = cscale(val, (ky & 1) ? -inv : inv);
+            if (mw) {
+                Mb[(long)j * mw + h + col] = v;
+                if (col > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(v.x, -v.y);
+            } else {
+                Mb[(long)j * cx + col] = v;
+            }
+        }
     });
 }
 
@@ -1359,17 +1372,19 @@ int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, in
 }
 
 int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
-                       bool lean, hipStream_t st) {
+                       bool lean, hipStream_t st, int mw) {
     if (B <= 0 || B > 65535) return LITHO_EARG;
+    if (mw && (mw % 2 == 0 || cx != mw / 2 + 1 || kymin != -(mw / 2) || cy != mw))
+        return LITHO_EARG;
 #define LITHO_MC(p, q, nth)                                                                    \
     if (N == p * q) {                                                                          \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
         if (h16)                                                                               \
             hipLaunchKernelGGL((mask_cols2_kernel<p, q, nth, true>), dim3(cdiv(cx, FPB), B),   \
-                               dim3(nth), 0, st, R, M, cx, kymin, cy);                         \
+                               dim3(nth), 0, st, R, M, cx, kymin, cy, mw);                     \
         else                                                                                   \
             hipLaunchKernelGGL((mask_cols2_kernel<p, q, nth, false>), dim3(cdiv(cx, FPB), B),  \
-                               dim3(nth), 0, st, R, M, cx, kymin, cy);                         \
+                               dim3(nth), 0, st, R, M, cx, kymin, cy, mw);                     \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     LITHO_MC(16, 16, 256)

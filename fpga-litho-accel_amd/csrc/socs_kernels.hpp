@@ -34,9 +34,11 @@ int band_grid(int Nx);
 // in: f32 masks, or uint8 levels (mask = level / 255) when u8
 int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, int count, bool h16,
                       bool lean, bool u8, hipStream_t st);
-// R [B][cx][N] -> M [B][cy][cx]
+// R [B][cx][N] -> M [B][cy][cx]; mirror mode mw = 2h+1 > 0: R holds kx = 0..h
+// of a real mask (cx = h+1, kymin = -h, cy = mw) and M [B][mw][mw] is filled
+// through M(-ky, -kx) = conj(M(ky, kx))
 int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
-                       bool lean, hipStream_t st);
+                       bool lean, hipStream_t st, int mw = 0);
 // M x bank -> band spectrum C [B][4Nx+1][2Nx+1] of the intensity, /G^2.
 // bank_fmt 0: float2; 1: bf16 (re, im) pairs x kscale[k]; 2: fp16 pairs x kscale[k].
 // shape 0: 48-slot blocks, 1: 24-slot (lean) blocks at G = 144, 2: wave-synced
