@@ -397,8 +397,11 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__
     const float* r0 = (const float*)in_ + roff;
     fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
         if (!act) return make_float2(0.f, 0.f);
-        if (IN8) return make_float2((float)r8[n] * (1.0f / 255.0f), (float)r8[L + n] * (1.0f / 255.0f));
-        return make_float2(r0[n], r0[L + n]);
+        // each mask row is read once: streaming (nontemporal) loads
+        if (IN8)
+            return make_float2((float)__builtin_nontemporal_load(r8 + n) * (1.0f / 255.0f),
+                               (float)__builtin_nontemporal_load(r8 + L + n) * (1.0f / 255.0f));
+        return make_float2(__builtin_nontemporal_load(r0 + n), __builtin_nontemporal_load(r0 + L + n));
     });
     __syncthreads();
     float2 res[Geo::PER];
@@ -492,12 +495,15 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict
         // undo the index shift: multiply by e^{-2 pi i band x / L}
         const float2 w = twG[(int)(((long)band * x) % L)];
         const float2 r = cmul(v, w);
+        // the image is written once and not read here: streaming stores
         if (O16) {
-            ((__half*)img)[o + x] = __float2half_rn(r.x);
-            ((__half*)img)[o + L + x] = __float2half_rn(r.y);
+            __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, __float2half_rn(r.x)),
+                                        (unsigned short*)img + o + x);
+            __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, __float2half_rn(r.y)),
+                                        (unsigned short*)img + o + L + x);
         } else {
-            ((float*)img)[o + x] = r.x;
-            ((float*)img)[o + L + x] = r.y;
+            __builtin_nontemporal_store(r.x, (float*)img + o + x);
+            __builtin_nontemporal_store(r.y, (float*)img + o + L + x);
         }
     });
 }
