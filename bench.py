@@ -39,16 +39,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_alg_bytes(N, Nx, G, K, B):
+def stage_alg_bytes(N, Nx, G, K, B, band=True):
     """Algorithmic HBM bytes per launch of each stage: its inputs and outputs
-    (intermediates of this design counted once each way)."""
+    (intermediates of this design counted once each way).  The band path keeps
+    kx = 0..Nx of each mask row (a real mask's spectrum is Hermitian), the
+    nextPow2 path the whole window."""
     W = 2 * Nx + 1
+    WR = Nx + 1 if band else W  # row-DFT columns kept
     nb = 2 * Nx + 1
     nby = 4 * Nx + 1
     f2 = 8
     return {
-        "mask_rows": B * (4 * N * N + N * W * f2),
-        "mask_cols": B * (N * W * f2 + W * W * f2),
+        "mask_rows": B * (4 * N * N + N * WR * f2),
+        "mask_cols": B * (N * WR * f2 + W * W * f2),
         "socs_accum": B * (W * W * f2 + 4 * G * G) + K * W * W * f2,
         "socs_band": B * (W * W * f2 + nby * nb * f2) + K * W * W * f2,
         "fi_rows": B * (4 * G * G + G * nb * f2),
@@ -294,7 +297,7 @@ def main():
     log(f"{images_per_s:.1f} images/s, {ms_per_step:.3f} ms/step, stages(ms) "
         + ", ".join(f"{n}={t:.3f}" for n, t in stage_acc.items()))
     Gimg = G if ("socs_accum" in stage_acc) else next(g for g in (36, 81, 144) if g >= 4 * Nx + 1)
-    alg = stage_alg_bytes(N, Nx, Gimg, NK, B)
+    alg = stage_alg_bytes(N, Nx, Gimg, NK, B, band="socs_band" in stage_acc)
     kernels = {n: {"ms": round(float(t), 4),
                    "alg_bytes": int(alg[n]),
                    "GBps": round(alg[n] / (t * 1e-3) / 1e9, 1)}
