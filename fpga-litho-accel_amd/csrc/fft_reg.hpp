@@ -105,6 +105,23 @@ __device__ __forceinline__ float2 cmul_w(float2 a, float2 w) {
     return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
 }
 
+// a * W_N^e (the conjugate root when INV).  The roots 1, -i, -1, +i are applied
+// exactly as swaps and sign changes: the table's quarter-period entries are
+// round-off-sized, not zero, so a full complex product there would cost four
+// VALU operations for nothing.  e is a constant once the callers' loops are
+// unrolled, so only one branch survives.
+template <int N, bool INV>
+__device__ __forceinline__ float2 twiddle(float2 a, int e) {
+    e = ((e % N) + N) % N;
+    if (e == 0) return a;
+    if (2 * e == N) return make_float2(-a.x, -a.y);
+    if (4 * e == N)  // forward -i, inverse +i
+        return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+    if (4 * e == 3 * N)  // forward +i, inverse -i
+        return INV ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+    return cmul_w<INV>(a, wroot<N, INV>(e));
+}
+
 template <int N, bool INV>
 __device__ __forceinline__ void dft_reg(float2* v);
 
@@ -173,7 +190,7 @@ __device__ __forceinline__ void dft_reg(float2* v) {
             float2 t[R];
             t[0] = a[0][k];
 #pragma unroll
-            for (int r = 1; r < R; r++) t[r] = cmul_w<INV>(a[r][k], wroot<N, INV>(r * k));
+            for (int r = 1; r < R; r++) t[r] = twiddle<N, INV>(a[r][k], r * k);
             dft_reg<R, INV>(t);
 #pragma unroll
             for (int q = 0; q < R; q++) v[k + M * q] = t[q];
