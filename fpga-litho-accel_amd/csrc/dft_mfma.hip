@@ -251,7 +251,10 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
 #pragma unroll
             for (int e = 0; e < 2; e++) {
                 const int kx = 8 * s + 2 * g + e;
-                const float2 v = kx < nb ? Ub[(long)kx * N + y] : make_float2(0.f, 0.f);
+                // clamped index and a select, not a branch: every load is in
+                // flight at once
+                float2 v = Ub[(long)min(kx, nb - 1) * N + y];
+                if (kx >= nb) v = make_float2(0.f, 0.f);
                 const _Float16 hr = (_Float16)v.x, hi = (_Float16)v.y;
                 const float lr = (v.x - (float)hr) * 4096.f, li = (v.y - (float)hi) * 4096.f;
                 q[2 * e] = __builtin_bit_cast(unsigned, (h2v){hr, hi});
