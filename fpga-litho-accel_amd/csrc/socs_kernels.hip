@@ -21,7 +21,28 @@
 #include "fft_reg.hpp"
 #include "socs_kernels.hpp"
 
+#include <mutex>
+#include <set>
+#include <utility>
+
 namespace litho {
+
+// The dynamic-LDS limit (hipFuncAttributeMaxDynamicSharedMemorySize) belongs
+// to a function on one device: raise it to the full 160 KB once per (kernel,
+// device), from any host thread.
+static int allow_full_lds(const void* kern) {
+    static std::mutex mu;
+    static std::set<std::pair<const void*, int>> done;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LITHO_EHIP;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count({kern, dev})) return LITHO_OK;
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=
+        hipSuccess)
+        return LITHO_EHIP;
+    done.insert({kern, dev});
+    return LITHO_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Storage of the two HBM intermediates (row-DFT window R, FI column output U):
@@ -888,13 +909,7 @@ static int launch_socs_GR(const float2* M, const float2* bank, const float* scal
     size_t lds = (size_t)(2 * NF * G + GR * W) * sizeof(float2);
     if (lds > 160 * 1024) return LITHO_EARG;
     auto kern = socs_accum_kernel<G, GR>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024) != hipSuccess)
-            return LITHO_EHIP;
-        attr_set = true;
-    }
+    LITHO_TRY(allow_full_lds((const void*)kern));
     unsigned grid = (unsigned)((size_t)B * (G / GR));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, st, M, bank, scales, nk, Nx, out, shift,
                        tw);
@@ -929,13 +944,7 @@ static int launch_socs2(const float2* M, const float2* bank, const float* scales
     size_t lds = (size_t)(G + SLOTS * Geo::XBUF + GR * W) * sizeof(float2);
     if (lds > 160 * 1024) return LITHO_EARG;
     auto kern = socs_accum2_kernel<P, Q, GR, NZ, NTH>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024) != hipSuccess)
-            return LITHO_EHIP;
-        attr_set = true;
-    }
+    LITHO_TRY(allow_full_lds((const void*)kern));
     unsigned grid = (unsigned)((size_t)B * (G / GR));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NTH), lds, st, M, bank, scales, nk, Nx, out, shift,
                        tw);
@@ -1276,13 +1285,7 @@ static int launch_wave(const float2* M, const void* bank, const float* kscale, c
     size_t lds = (size_t)(Geo::G + SL * Geo::XBUF + Geo::G * W) * sizeof(float2);
     if (lds > 160 * 1024) return LITHO_EARG;
     auto kern = socs_wave_kernel<P, NZ, NW, BK>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024) != hipSuccess)
-            return LITHO_EHIP;
-        attr_set = true;
-    }
+    LITHO_TRY(allow_full_lds((const void*)kern));
     hipLaunchKernelGGL(kern, dim3(B), dim3(64 * NW), lds, st, M, bank, kscale, scales, nk, Nx, C);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
@@ -1295,13 +1298,7 @@ static int launch_band(const float2* M, const void* bank, const float* kscale, c
     size_t lds = (size_t)(Geo::G + SLOTS * Geo::XBUF + Geo::G * W) * sizeof(float2);
     if (lds > 160 * 1024) return LITHO_EARG;
     auto kern = socs_band_kernel<P, NZ, SLOTS, BK>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024) != hipSuccess)
-            return LITHO_EHIP;
-        attr_set = true;
-    }
+    LITHO_TRY(allow_full_lds((const void*)kern));
     hipLaunchKernelGGL(kern, dim3(B), dim3(P * SLOTS), lds, st, M, bank, kscale, scales, nk, Nx, C);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
