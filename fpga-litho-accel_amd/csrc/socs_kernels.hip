@@ -461,6 +461,95 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__
     }
 }
 
+// rows_band1024: rows_fwd2 for 1024-point rows and the band window kx = 0..count-1
+// (count <= 64), transposed output with the centring sign.  Two changes keep
+// more blocks on a CU: the stage-1 -> stage-2 transpose goes through LDS one
+// component plane (re, then im) at a time, halving the buffer, and the output
+// skips the natural-order spectrum in LDS: lane t of a transform holds
+// X[t + 32 k2] in u[k2] after stage 2, and X[N - k] comes from lane (32 - t) mod 32
+// by ds_bpermute (u[31] for k < 32, u[30] for 32 <= k < 64).
+template <bool H16, bool IN8>
+__global__ __launch_bounds__(256) void rows_band1024_kernel(const void* __restrict__ in_,
+                                                            void* __restrict__ out_, int nrows,
+                                                            int count) {
+    using Geo = Fft2Geom<32, 32>;
+    using S = Store<H16>;
+    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T, LDQ = Geo::LDQ;
+    typename S::T* out = (typename S::T*)out_;
+    __shared__ float2 twG[L];
+    __shared__ float xplane[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
+    float* xb = xplane + f * Geo::XBUF;
+    const long roff = (2 * (act ? p : 0)) * (long)L;
+    const unsigned char* r8 = (const unsigned char*)in_ + roff;
+    const float* r0 = (const float*)in_ + roff;
+    // stage 1 for n2 = t: DFT over n1 of x[32 n1 + t], two real rows as re / im
+    float2 v[32];
+#pragma unroll
+    for (int n1 = 0; n1 < 32; n1++) {
+        const int n = 32 * n1 + t;
+        if (!act) {
+            v[n1] = make_float2(0.f, 0.f);
+        } else if (IN8) {
+            v[n1] = make_float2((float)__builtin_nontemporal_load(r8 + n) * (1.0f / 255.0f),
+                                (float)__builtin_nontemporal_load(r8 + L + n) * (1.0f / 255.0f));
+        } else {
+            v[n1] = make_float2(__builtin_nontemporal_load(r0 + n),
+                                __builtin_nontemporal_load(r0 + L + n));
+        }
+    }
+    dft_reg<32, false>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 32; k1++) v[k1] = cmul_w<false>(v[k1], twG[t * k1]);
+    // transpose, one plane at a time: stage 2 (k1 = t) reads row t of the plane
+    float2 u[32];
+#pragma unroll
+    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].x;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 32; n2++) u[n2].x = xb[t * LDQ + n2];
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].y;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 32; n2++) u[n2].y = xb[t * LDQ + n2];
+    dft_reg<32, false>(u);  // u[k2] = X[t + 32 k2]
+    // X[N - k] from lane (32 - t) mod 32 of this transform (every lane takes part)
+    const int lane = tid & 63;
+    const int src = ((lane & 32) | ((32 - t) & 31)) << 2;
+    const float2 m31 = make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[31].x))),
+                                   __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[31].y))));
+    const float2 m30 = make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[30].x))),
+                                   __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[30].y))));
+    if (!act) return;
+    const long r = 2 * p, img = r / L, y = r - img * L;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int k = t + 32 * h;  // window column j = k (kmin = 0)
+        if (k >= count) break;
+        const float2 z1 = h ? u[1] : u[0];
+        const float2 zm = h ? (t == 0 ? m31 : m30) : (t == 0 ? u[0] : m31);  // X[N - k]
+        const float2 z2 = cconj(zm);
+        float2 x1 = cscale(cadd(z1, z2), 0.5f);
+        const float2 d = csub(z1, z2);
+        float2 x2 = make_float2(0.5f * d.y, -0.5f * d.x);
+        if (k & 1) {
+            x1 = cscale(x1, -1.f);
+            x2 = cscale(x2, -1.f);
+        }
+        typename S::T* o = out + (img * count + k) * (long)L + y;
+        o[0] = S::st(x1);
+        o[1] = S::st(x2);
+    }
+}
+
 // image_rows2: same contract as image_rows.  The Hermitian row spectrum
 // (|kx| <= band) is placed at indices kx + band in [0, 2 band] so stage-1
 // inputs past ceil((2band+1)/Q) rows are compile-time zeros; the shift is
@@ -1361,6 +1450,24 @@ int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, in
         else if (u8) LITHO_RT1(p, q, nth, false, true);                                       \
         else LITHO_RT1(p, q, nth, false, false);                                              \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
+    }
+    // the band window of 1024-point rows: 42 KB of LDS already, so the lean
+    // plan uses it too (and both plans give the same bits)
+    if (L == 1024 && kmin == 0 && count <= 64) {
+        unsigned grid = cdiv((size_t)(nrows / 2), 8);
+        if (h16 && u8)
+            hipLaunchKernelGGL((rows_band1024_kernel<true, true>), dim3(grid), dim3(256), 0, st, in,
+                               out, (int)nrows, count);
+        else if (h16)
+            hipLaunchKernelGGL((rows_band1024_kernel<true, false>), dim3(grid), dim3(256), 0, st,
+                               in, out, (int)nrows, count);
+        else if (u8)
+            hipLaunchKernelGGL((rows_band1024_kernel<false, true>), dim3(grid), dim3(256), 0, st,
+                               in, out, (int)nrows, count);
+        else
+            hipLaunchKernelGGL((rows_band1024_kernel<false, false>), dim3(grid), dim3(256), 0, st,
+                               in, out, (int)nrows, count);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
     LITHO_RT(16, 16, 256)
     LITHO_RT(16, 32, 256)
