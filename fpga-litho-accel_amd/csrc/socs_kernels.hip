@@ -618,6 +618,91 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict
     });
 }
 
+// image_band1024: image_rows2 for 1024-point rows of the band path (U
+// transposed [img][kx][row], 2 band + 1 <= 160 so five stage-1 inputs per
+// lane), with the stage transpose through LDS one component plane at a time
+// (42 KB per block: three blocks per CU).
+template <bool H16, bool O16>
+__global__ __launch_bounds__(256) void image_band1024_kernel(const void* __restrict__ U_,
+                                                             void* __restrict__ img, int nrows,
+                                                             int band) {
+    using Geo = Fft2Geom<32, 32>;
+    using S = Store<H16>;
+    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T, LDQ = Geo::LDQ, NZ = 5;
+    __shared__ float2 twG[L];
+    __shared__ float xplane[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const int nb = band + 1;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
+    float* xb = xplane + f * Geo::XBUF;
+    const typename S::T* U = (const typename S::T*)U_;
+    const long r0 = 2 * (act ? p : 0);
+    const long img0 = r0 / L, y0 = r0 - img0 * L;
+    const typename S::T* u1 = U + img0 * nb * (long)L + y0;
+    // stage 1 for n2 = t: band row n = kx + band as the two rows' Hermitian
+    // extensions packed as re / im (u1 + i u2 for kx > 0)
+    float2 v[32];
+#pragma unroll
+    for (int n1 = 0; n1 < 32; n1++) {
+        float2 z = make_float2(0.f, 0.f);
+        const int n = 32 * n1 + t;
+        const int kx = n - band;
+        if (n1 < NZ && act && kx >= -band && kx <= band) {
+            const int a = kx < 0 ? -kx : kx;
+            const float2 a1 = S::ld(u1[(long)a * L]), a2 = S::ld(u1[(long)a * L + 1]);
+            if (a == 0)
+                z = make_float2(a1.x, a2.x);
+            else if (kx > 0)
+                z = make_float2(a1.x - a2.y, a1.y + a2.x);
+            else
+                z = make_float2(a1.x + a2.y, a2.x - a1.y);
+        }
+        v[n1] = z;
+    }
+    dft_reg<32, true>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 32; k1++) {
+        float2 w = twG[t * k1];
+        w.y = -w.y;
+        v[k1] = cmul_w<false>(v[k1], w);
+    }
+    float2 u[32];
+#pragma unroll
+    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].x;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 32; n2++) u[n2].x = xb[t * LDQ + n2];
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].y;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 32; n2++) u[n2].y = xb[t * LDQ + n2];
+    if (!act) return;
+    dft_reg<32, true>(u);  // u[k2] = pixel x = t + 32 k2 of the two rows
+    const long o = (2 * p) * (long)L;
+#pragma unroll
+    for (int k2 = 0; k2 < 32; k2++) {
+        const int x = t + 32 * k2;
+        // undo the index shift: multiply by e^{-2 pi i band x / L}; streaming stores
+        const float2 r = cmul(u[k2], twG[(int)(((long)band * x) % L)]);
+        if (O16) {
+            __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, __float2half_rn(r.x)),
+                                        (unsigned short*)img + o + x);
+            __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, __float2half_rn(r.y)),
+                                        (unsigned short*)img + o + L + x);
+        } else {
+            __builtin_nontemporal_store(r.x, (float*)img + o + x);
+            __builtin_nontemporal_store(r.y, (float*)img + o + L + x);
+        }
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // Band pipeline (image path): square conv grid G = P*P of any size with
@@ -1552,6 +1637,22 @@ int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, b
         else if (out16) LITHO_IT1(p, q, nz, nth, false, true);                                 \
         else LITHO_IT1(p, q, nz, nth, false, false);                                           \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
+    }
+    if (N == 1024 && 2 * band + 1 <= 160) {  // every plan: one kernel, the same bits
+        unsigned grid = cdiv((size_t)(nrows / 2), 8);
+        if (h16 && out16)
+            hipLaunchKernelGGL((image_band1024_kernel<true, true>), dim3(grid), dim3(256), 0, st, U,
+                               img, (int)nrows, band);
+        else if (h16)
+            hipLaunchKernelGGL((image_band1024_kernel<true, false>), dim3(grid), dim3(256), 0, st,
+                               U, img, (int)nrows, band);
+        else if (out16)
+            hipLaunchKernelGGL((image_band1024_kernel<false, true>), dim3(grid), dim3(256), 0, st,
+                               U, img, (int)nrows, band);
+        else
+            hipLaunchKernelGGL((image_band1024_kernel<false, false>), dim3(grid), dim3(256), 0, st,
+                               U, img, (int)nrows, band);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
     LITHO_IT(16, 16, 3, 256)
     LITHO_IT(16, 16, 16, 256)
