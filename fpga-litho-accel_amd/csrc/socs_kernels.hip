@@ -461,6 +461,27 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__
     }
 }
 
+// Stage transpose of a 32 x 32 two-stage FFT through a float plane buffer xb
+// (32 rows of pitch 33), one component at a time: lane t holds the twiddled
+// stage-1 outputs v[k1] of n2 = t and receives the stage-2 inputs u[n2] of
+// k1 = t.  Half the LDS of a float2 buffer; every thread of the block calls it
+// (it holds block barriers).
+__device__ __forceinline__ void transpose32_planes(int t, float* __restrict__ xb,
+                                                   const float2 (&v)[32], float2 (&u)[32]) {
+    constexpr int LDQ = 33;
+#pragma unroll
+    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].x;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 32; n2++) u[n2].x = xb[t * LDQ + n2];
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].y;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 32; n2++) u[n2].y = xb[t * LDQ + n2];
+}
+
 // rows_band1024: rows_fwd2 for 1024-point rows and the band window kx = 0..count-1
 // (count <= 64), transposed output with the centring sign.  Two changes keep
 // more blocks on a CU: the stage-1 -> stage-2 transpose goes through LDS one
@@ -474,7 +495,7 @@ __global__ __launch_bounds__(256) void rows_band1024_kernel(const void* __restri
                                                             int count) {
     using Geo = Fft2Geom<32, 32>;
     using S = Store<H16>;
-    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T, LDQ = Geo::LDQ;
+    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T;
     typename S::T* out = (typename S::T*)out_;
     __shared__ float2 twG[L];
     __shared__ float xplane[FPB * Geo::XBUF];
@@ -507,19 +528,8 @@ __global__ __launch_bounds__(256) void rows_band1024_kernel(const void* __restri
     dft_reg<32, false>(v);
 #pragma unroll
     for (int k1 = 1; k1 < 32; k1++) v[k1] = cmul_w<false>(v[k1], twG[t * k1]);
-    // transpose, one plane at a time: stage 2 (k1 = t) reads row t of the plane
     float2 u[32];
-#pragma unroll
-    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].x;
-    __syncthreads();
-#pragma unroll
-    for (int n2 = 0; n2 < 32; n2++) u[n2].x = xb[t * LDQ + n2];
-    __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].y;
-    __syncthreads();
-#pragma unroll
-    for (int n2 = 0; n2 < 32; n2++) u[n2].y = xb[t * LDQ + n2];
+    transpose32_planes(t, xb, v, u);
     dft_reg<32, false>(u);  // u[k2] = X[t + 32 k2]
     // X[N - k] from lane (32 - t) mod 32 of this transform (every lane takes part)
     const int lane = tid & 63;
@@ -628,7 +638,7 @@ __global__ __launch_bounds__(256) void image_band1024_kernel(const void* __restr
                                                              int band) {
     using Geo = Fft2Geom<32, 32>;
     using S = Store<H16>;
-    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T, LDQ = Geo::LDQ, NZ = 5;
+    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T, NZ = 5;
     __shared__ float2 twG[L];
     __shared__ float xplane[FPB * Geo::XBUF];
     const int tid = threadIdx.x;
@@ -672,17 +682,7 @@ __global__ __launch_bounds__(256) void image_band1024_kernel(const void* __restr
         v[k1] = cmul_w<false>(v[k1], w);
     }
     float2 u[32];
-#pragma unroll
-    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].x;
-    __syncthreads();
-#pragma unroll
-    for (int n2 = 0; n2 < 32; n2++) u[n2].x = xb[t * LDQ + n2];
-    __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < 32; k1++) xb[k1 * LDQ + t] = v[k1].y;
-    __syncthreads();
-#pragma unroll
-    for (int n2 = 0; n2 < 32; n2++) u[n2].y = xb[t * LDQ + n2];
+    transpose32_planes(t, xb, v, u);
     if (!act) return;
     dft_reg<32, true>(u);  // u[k2] = pixel x = t + 32 k2 of the two rows
     const long o = (2 * p) * (long)L;
@@ -897,6 +897,105 @@ __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict_
         const float2 w = twG[(int)(((long)band * y) % N)];  // e^{-2 pi i band y / N}
         Uc[y] = S::st(cscale(cmul(val, w), sx));
     });
+}
+
+// mask_cols_band1024 / fi_cols_band1024: mask_cols2 and fi_cols2 for 1024-point
+// columns with the plane-split stage transpose (42 KB per block: three blocks
+// per CU); same inputs, outputs and index conventions.
+template <bool H16>
+__global__ __launch_bounds__(256) void mask_cols_band1024_kernel(const void* __restrict__ R_,
+                                                                 float2* __restrict__ M, int cx,
+                                                                 int kymin, int cy, int mw) {
+    using Geo = Fft2Geom<32, 32>;
+    constexpr int N = 1024, T = 32, NTH = 256, FPB = NTH / T;
+    __shared__ float2 twG[N];
+    __shared__ float xplane[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const long b = blockIdx.y;
+    const int col = blockIdx.x * FPB + f;
+    const bool act = col < cx;
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
+    using S = Store<H16>;
+    const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N;
+    float2 v[32];
+#pragma unroll
+    for (int n1 = 0; n1 < 32; n1++) v[n1] = act ? S::ld(Rc[32 * n1 + t]) : make_float2(0.f, 0.f);
+    dft_reg<32, false>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 32; k1++) v[k1] = cmul_w<false>(v[k1], twG[t * k1]);
+    float2 u[32];
+    transpose32_planes(t, xplane + f * Geo::XBUF, v, u);
+    if (!act) return;
+    dft_reg<32, false>(u);  // u[k2] = X[t + 32 k2]
+    const float inv = 1.0f / ((float)N * (float)N);
+    float2* Mb = M + b * (long)cy * (mw ? mw : cx);
+    const int h = mw >> 1;
+#pragma unroll
+    for (int k2 = 0; k2 < 32; k2++) {
+        const int k = t + 32 * k2;
+        const int ky = k < N / 2 ? k : k - N;
+        const int j = ky - kymin;
+        if (j >= 0 && j < cy) {
+            const float2 val = cscale(u[k2], (ky & 1) ? -inv : inv);
+            if (mw) {
+                Mb[(long)j * mw + h + col] = val;
+                if (col > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(val.x, -val.y);
+            } else {
+                Mb[(long)j * cx + col] = val;
+            }
+        }
+    }
+}
+
+template <bool H16>
+__global__ __launch_bounds__(256) void fi_cols_band1024_kernel(const float2* __restrict__ C,
+                                                               void* __restrict__ U_, int band) {
+    using Geo = Fft2Geom<32, 32>;
+    constexpr int N = 1024, T = 32, NTH = 256, FPB = NTH / T, NZ = 5;
+    __shared__ float2 twG[N];
+    __shared__ float xplane[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const int nb = band + 1;
+    const long b = blockIdx.y;
+    const int col = blockIdx.x * FPB + f;
+    const bool act = col < nb;
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
+    const float2* Cb = C + b * (long)(2 * band + 1) * nb;
+    float2 v[32];
+#pragma unroll
+    for (int n1 = 0; n1 < 32; n1++) {
+        const int n = 32 * n1 + t;
+        float2 z = make_float2(0.f, 0.f);
+        if (n1 < NZ && act && n <= 2 * band) {
+            z = Cb[(long)n * nb + col];
+            if ((n - band) & 1) z = cscale(z, -1.f);
+        }
+        v[n1] = z;
+    }
+    dft_reg<32, true>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 32; k1++) {
+        float2 w = twG[t * k1];
+        w.y = -w.y;
+        v[k1] = cmul_w<false>(v[k1], w);
+    }
+    float2 u[32];
+    transpose32_planes(t, xplane + f * Geo::XBUF, v, u);
+    if (!act) return;
+    dft_reg<32, true>(u);  // u[k2] = row y = t + 32 k2
+    using S = Store<H16>;
+    typename S::T* Uc = (typename S::T*)U_ + (b * nb + col) * (long)N;
+    const float sx = (col & 1) ? -1.f : 1.f;
+#pragma unroll
+    for (int k2 = 0; k2 < 32; k2++) {
+        const int y = t + 32 * k2;
+        const float2 w = twG[(int)(((long)band * y) % N)];  // e^{-2 pi i band y / N}
+        Uc[y] = S::st(cscale(cmul(u[k2], w), sx));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1571,6 +1670,15 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
     if (B <= 0 || B > 65535) return LITHO_EARG;
     if (mw && (mw % 2 == 0 || cx != mw / 2 + 1 || kymin != -(mw / 2) || cy != mw))
         return LITHO_EARG;
+    if (N == 1024) {  // every plan: one kernel, the same bits
+        if (h16)
+            hipLaunchKernelGGL((mask_cols_band1024_kernel<true>), dim3(cdiv(cx, 8), B), dim3(256), 0,
+                               st, R, M, cx, kymin, cy, mw);
+        else
+            hipLaunchKernelGGL((mask_cols_band1024_kernel<false>), dim3(cdiv(cx, 8), B), dim3(256),
+                               0, st, R, M, cx, kymin, cy, mw);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    }
 #define LITHO_MC(p, q, nth)                                                                    \
     if (N == p * q) {                                                                          \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                           \
@@ -1596,6 +1704,14 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
                      hipStream_t st) {
     if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
+    if (N == 1024 && 2 * band + 1 <= 160) {  // every plan: one kernel, the same bits
+        const dim3 grid(cdiv(band + 1, 8), B);
+        if (h16)
+            hipLaunchKernelGGL((fi_cols_band1024_kernel<true>), grid, dim3(256), 0, st, C, U, band);
+        else
+            hipLaunchKernelGGL((fi_cols_band1024_kernel<false>), grid, dim3(256), 0, st, C, U, band);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    }
 #define LITHO_FC(p, q, nz, nth)                                                                 \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                                 \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                            \
