@@ -416,8 +416,8 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__
     // 8-bit masks: level / 255 (the PNG -> BIN convention), exact in FP32
     const unsigned char* r8 = (const unsigned char*)in_ + roff;
     const float* r0 = (const float*)in_ + roff;
+    // an idle transform reads pair 0 (in bounds) and writes nothing
     fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
-        if (!act) return make_float2(0.f, 0.f);
         // each mask row is read once: streaming (nontemporal) loads
         if (IN8)
             return make_float2((float)__builtin_nontemporal_load(r8 + n) * (1.0f / 255.0f),
@@ -489,13 +489,13 @@ __device__ __forceinline__ void transpose32_planes(int t, float* __restrict__ xb
 // skips the natural-order spectrum in LDS: lane t of a transform holds
 // X[t + 32 k2] in u[k2] after stage 2, and X[N - k] comes from lane (32 - t) mod 32
 // by ds_bpermute (u[31] for k < 32, u[30] for 32 <= k < 64).
-template <bool H16, bool IN8>
-__global__ __launch_bounds__(256) void rows_band1024_kernel(const void* __restrict__ in_,
+template <bool H16, bool IN8, int NTH = 256>
+__global__ __launch_bounds__(NTH) void rows_band1024_kernel(const void* __restrict__ in_,
                                                             void* __restrict__ out_, int nrows,
                                                             int count) {
     using Geo = Fft2Geom<32, 32>;
     using S = Store<H16>;
-    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T;
+    constexpr int L = 1024, T = 32, FPB = NTH / T;
     typename S::T* out = (typename S::T*)out_;
     __shared__ float2 twG[L];
     __shared__ float xplane[FPB * Geo::XBUF];
@@ -507,17 +507,17 @@ __global__ __launch_bounds__(256) void rows_band1024_kernel(const void* __restri
     fill_twiddles<L>(twG, tid, NTH);
     __syncthreads();
     float* xb = xplane + f * Geo::XBUF;
+    // an idle transform (past the last row pair) reads pair 0 and writes nothing;
+    // loading unconditionally keeps the 64 loads as one base plus immediate offsets
     const long roff = (2 * (act ? p : 0)) * (long)L;
-    const unsigned char* r8 = (const unsigned char*)in_ + roff;
-    const float* r0 = (const float*)in_ + roff;
+    const unsigned char* r8 = (const unsigned char*)in_ + roff + t;
+    const float* r0 = (const float*)in_ + roff + t;
     // stage 1 for n2 = t: DFT over n1 of x[32 n1 + t], two real rows as re / im
     float2 v[32];
 #pragma unroll
     for (int n1 = 0; n1 < 32; n1++) {
-        const int n = 32 * n1 + t;
-        if (!act) {
-            v[n1] = make_float2(0.f, 0.f);
-        } else if (IN8) {
+        const int n = 32 * n1;
+        if (IN8) {
             v[n1] = make_float2((float)__builtin_nontemporal_load(r8 + n) * (1.0f / 255.0f),
                                 (float)__builtin_nontemporal_load(r8 + L + n) * (1.0f / 255.0f));
         } else {
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict
     fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
         float2 z = make_float2(0.f, 0.f);
         const int kx = n - band;
-        if (act && kx >= -band && kx <= band) {
+        if (kx >= -band && kx <= band) {  // idle transforms read row pair 0
             const int a = kx < 0 ? -kx : kx;
             float2 a1 = S::ld(u1[a * ust]), a2 = S::ld(u1[a * ust + u2o]);
             if (a == 0) {
@@ -632,13 +632,13 @@ __global__ __launch_bounds__(NTH) void image_rows2_kernel(const void* __restrict
 // transposed [img][kx][row], 2 band + 1 <= 160 so five stage-1 inputs per
 // lane), with the stage transpose through LDS one component plane at a time
 // (42 KB per block: three blocks per CU).
-template <bool H16, bool O16>
-__global__ __launch_bounds__(256) void image_band1024_kernel(const void* __restrict__ U_,
+template <bool H16, bool O16, int NTH = 256>
+__global__ __launch_bounds__(NTH) void image_band1024_kernel(const void* __restrict__ U_,
                                                              void* __restrict__ img, int nrows,
                                                              int band) {
     using Geo = Fft2Geom<32, 32>;
     using S = Store<H16>;
-    constexpr int L = 1024, T = 32, NTH = 256, FPB = NTH / T, NZ = 5;
+    constexpr int L = 1024, T = 32, FPB = NTH / T, NZ = 5;
     __shared__ float2 twG[L];
     __shared__ float xplane[FPB * Geo::XBUF];
     const int tid = threadIdx.x;
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(256) void image_band1024_kernel(const void* __restr
         float2 z = make_float2(0.f, 0.f);
         const int n = 32 * n1 + t;
         const int kx = n - band;
-        if (n1 < NZ && act && kx >= -band && kx <= band) {
+        if (n1 < NZ && kx >= -band && kx <= band) {  // idle transforms read row pair 0
             const int a = kx < 0 ? -kx : kx;
             const float2 a1 = S::ld(u1[(long)a * L]), a2 = S::ld(u1[(long)a * L + 1]);
             if (a == 0)
@@ -835,9 +835,7 @@ __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict_
     float2* xb = xbuf + f * Geo::XBUF;
     using S = Store<H16>;
     const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N;
-    fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) {
-        return act ? S::ld(Rc[n]) : make_float2(0.f, 0.f);
-    });
+    fft2_stage1<Geo, P, false>(t, xb, twG, [&](int n) { return S::ld(Rc[n]); });
     __syncthreads();
     if (!act) return;
     const float inv = 1.0f / ((float)N * (float)N);
@@ -879,11 +877,12 @@ __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict_
     fill_twiddles<N>(twG, tid, NTH);
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
-    const float2* Cb = C + b * (long)(2 * band + 1) * nb;
+    // an idle transform reads column 0 and writes nothing
+    const float2* Cb = C + b * (long)(2 * band + 1) * nb + (act ? col : 0);
     fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
         float2 v = make_float2(0.f, 0.f);
-        if (act && n <= 2 * band) {
-            v = Cb[(long)n * nb + col];
+        if (n <= 2 * band) {
+            v = Cb[(long)n * nb];
             if ((n - band) & 1) v = cscale(v, -1.f);
         }
         return v;
@@ -902,12 +901,12 @@ __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict_
 // mask_cols_band1024 / fi_cols_band1024: mask_cols2 and fi_cols2 for 1024-point
 // columns with the plane-split stage transpose (42 KB per block: three blocks
 // per CU); same inputs, outputs and index conventions.
-template <bool H16>
-__global__ __launch_bounds__(256) void mask_cols_band1024_kernel(const void* __restrict__ R_,
+template <bool H16, int NTH = 256>
+__global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __restrict__ R_,
                                                                  float2* __restrict__ M, int cx,
                                                                  int kymin, int cy, int mw) {
     using Geo = Fft2Geom<32, 32>;
-    constexpr int N = 1024, T = 32, NTH = 256, FPB = NTH / T;
+    constexpr int N = 1024, T = 32, FPB = NTH / T;
     __shared__ float2 twG[N];
     __shared__ float xplane[FPB * Geo::XBUF];
     const int tid = threadIdx.x;
@@ -921,7 +920,7 @@ __global__ __launch_bounds__(256) void mask_cols_band1024_kernel(const void* __r
     const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N;
     float2 v[32];
 #pragma unroll
-    for (int n1 = 0; n1 < 32; n1++) v[n1] = act ? S::ld(Rc[32 * n1 + t]) : make_float2(0.f, 0.f);
+    for (int n1 = 0; n1 < 32; n1++) v[n1] = S::ld(Rc[32 * n1 + t]);  // idle: column 0
     dft_reg<32, false>(v);
 #pragma unroll
     for (int k1 = 1; k1 < 32; k1++) v[k1] = cmul_w<false>(v[k1], twG[t * k1]);
@@ -949,11 +948,11 @@ __global__ __launch_bounds__(256) void mask_cols_band1024_kernel(const void* __r
     }
 }
 
-template <bool H16>
-__global__ __launch_bounds__(256) void fi_cols_band1024_kernel(const float2* __restrict__ C,
+template <bool H16, int NTH = 256>
+__global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __restrict__ C,
                                                                void* __restrict__ U_, int band) {
     using Geo = Fft2Geom<32, 32>;
-    constexpr int N = 1024, T = 32, NTH = 256, FPB = NTH / T, NZ = 5;
+    constexpr int N = 1024, T = 32, FPB = NTH / T, NZ = 5;
     __shared__ float2 twG[N];
     __shared__ float xplane[FPB * Geo::XBUF];
     const int tid = threadIdx.x;
@@ -964,14 +963,15 @@ __global__ __launch_bounds__(256) void fi_cols_band1024_kernel(const float2* __r
     const bool act = col < nb;
     fill_twiddles<N>(twG, tid, NTH);
     __syncthreads();
-    const float2* Cb = C + b * (long)(2 * band + 1) * nb;
+    // an idle transform reads column 0 and writes nothing
+    const float2* Cb = C + b * (long)(2 * band + 1) * nb + (act ? col : 0);
     float2 v[32];
 #pragma unroll
     for (int n1 = 0; n1 < 32; n1++) {
         const int n = 32 * n1 + t;
         float2 z = make_float2(0.f, 0.f);
-        if (n1 < NZ && act && n <= 2 * band) {
-            z = Cb[(long)n * nb + col];
+        if (n1 < NZ && n <= 2 * band) {
+            z = Cb[(long)n * nb];
             if ((n - band) & 1) z = cscale(z, -1.f);
         }
         v[n1] = z;
@@ -1635,22 +1635,23 @@ int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, in
         else LITHO_RT1(p, q, nth, false, false);                                              \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
     }
-    // the band window of 1024-point rows: 42 KB of LDS already, so the lean
-    // plan uses it too (and both plans give the same bits)
+    // the band window of 1024-point rows: one kernel for every plan (the same
+    // bits); 512-thread blocks share one twiddle table, so two blocks (76 KB of
+    // LDS each) keep 16 waves per CU where 256-thread blocks fit 12
     if (L == 1024 && kmin == 0 && count <= 64) {
-        unsigned grid = cdiv((size_t)(nrows / 2), 8);
+        unsigned grid = cdiv((size_t)(nrows / 2), 16);
         if (h16 && u8)
-            hipLaunchKernelGGL((rows_band1024_kernel<true, true>), dim3(grid), dim3(256), 0, st, in,
-                               out, (int)nrows, count);
+            hipLaunchKernelGGL((rows_band1024_kernel<true, true, 512>), dim3(grid), dim3(512), 0,
+                               st, in, out, (int)nrows, count);
         else if (h16)
-            hipLaunchKernelGGL((rows_band1024_kernel<true, false>), dim3(grid), dim3(256), 0, st,
-                               in, out, (int)nrows, count);
+            hipLaunchKernelGGL((rows_band1024_kernel<true, false, 512>), dim3(grid), dim3(512), 0,
+                               st, in, out, (int)nrows, count);
         else if (u8)
-            hipLaunchKernelGGL((rows_band1024_kernel<false, true>), dim3(grid), dim3(256), 0, st,
-                               in, out, (int)nrows, count);
+            hipLaunchKernelGGL((rows_band1024_kernel<false, true, 512>), dim3(grid), dim3(512), 0,
+                               st, in, out, (int)nrows, count);
         else
-            hipLaunchKernelGGL((rows_band1024_kernel<false, false>), dim3(grid), dim3(256), 0, st,
-                               in, out, (int)nrows, count);
+            hipLaunchKernelGGL((rows_band1024_kernel<false, false, 512>), dim3(grid), dim3(512), 0,
+                               st, in, out, (int)nrows, count);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
     LITHO_RT(16, 16, 256)
