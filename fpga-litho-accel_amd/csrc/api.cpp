@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -75,14 +76,30 @@ struct litho_ctx {
     int nbev = 0;
 };
 
-struct litho_bank {
-    litho_ctx* ctx = nullptr;
+// One kernel set in HBM: n entries of W x W float2, the weight of each entry,
+// and 16-bit copies with one scale per entry for the real and one for the
+// imaginary parts.
+struct KernelSet {
     float2* d_bank = nullptr;
     float* d_scales = nullptr;
-    unsigned* d_bank16 = nullptr;  // bf16 (re | im << 16), scaled per kernel
-    unsigned* d_bankh = nullptr;   // fp16 (re, im) pairs, scaled per kernel
-    float* d_kscale = nullptr;
+    unsigned* d_bank16 = nullptr;  // bf16 (re | im << 16), scaled per entry
+    unsigned* d_bankh = nullptr;   // fp16 (re, im) pairs, scaled per entry
+    float2* d_kscale = nullptr;
+    int n = 0;
+};
+
+// The kernels as given (calcSOCS semantics: any mask spectrum), and the set the
+// band pipeline runs, whose mask spectra come from real masks.  There the window
+// spectrum M is conjugate-symmetric about the window centre, so a kernel with
+// K(-f) = e^{ib} conj(K(f)) (real kernels of definite parity are such) gives a
+// field IFFT(K M) that is real up to the phase ramp common to every field of
+// the clip; two such kernels a, b then share one transform: the entry
+// sqrt(la) Ka + i sqrt(lb) Kb, weight 1, has |IFFT|^2 = la |Ea|^2 + lb |Eb|^2.
+struct litho_bank {
+    litho_ctx* ctx = nullptr;
+    KernelSet full, band;  // band aliases full when no two kernels pair
     int nk = 0, Nx = 0, Ny = 0;
+    int pairs = 0;
 };
 
 static int ctx_ws(litho_ctx* c, int slot, size_t bytes, void** out) {
@@ -748,6 +765,127 @@ static int kernels_from_source_impl(const LithoOptics* op, const float* src, int
 
 extern "C" {
 
+}  // extern "C"
+
+// Upload one kernel set: float2 entries, weights, and the 16-bit copies (round
+// to nearest even) scaled per entry by max |re| and max |im|.
+static int upload_set(litho_ctx* c, const std::vector<float2>& h, const std::vector<float>& w,
+                      size_t per, KernelSet* s) {
+    const int n = (int)w.size();
+    s->n = n;
+    std::vector<unsigned> h16(h.size());
+    std::vector<__half2> hh(h.size());
+    std::vector<float2> ks(n);
+    auto bf16 = [](float f) -> unsigned {
+        unsigned u;
+        std::memcpy(&u, &f, 4);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return u >> 16;
+    };
+    for (int k = 0; k < n; k++) {
+        float mx = 0.f, my = 0.f;
+        for (size_t j = 0; j < per; j++) {
+            mx = std::max(mx, std::fabs(h[k * per + j].x));
+            my = std::max(my, std::fabs(h[k * per + j].y));
+        }
+        ks[k] = make_float2(mx > 0.f ? mx : 1.0f, my > 0.f ? my : 1.0f);
+        for (size_t j = 0; j < per; j++) {
+            const float re = h[k * per + j].x / ks[k].x, im = h[k * per + j].y / ks[k].y;
+            h16[k * per + j] = bf16(re) | (bf16(im) << 16);
+            hh[k * per + j] = __floats2half2_rn(re, im);
+        }
+    }
+    if (hipMalloc(&s->d_bank, h.size() * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&s->d_scales, n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->d_bank16, h16.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&s->d_bankh, hh.size() * sizeof(__half2)) != hipSuccess ||
+        hipMalloc(&s->d_kscale, n * sizeof(float2)) != hipSuccess)
+        return LITHO_ENOMEM;
+    hipStream_t st = c->stream;
+    LITHO_HIP(hipMemcpyAsync(s->d_bank, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(s->d_scales, w.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(s->d_bank16, h16.data(), h16.size() * sizeof(unsigned),
+                             hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(s->d_bankh, hh.data(), hh.size() * sizeof(__half2),
+                             hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(s->d_kscale, ks.data(), n * sizeof(float2), hipMemcpyHostToDevice, st));
+    return LITHO_OK;
+}
+
+// The band pipeline's set (see litho_bank).  Entry j of a W x W kernel and
+// entry per - 1 - j are the frequencies f and -f about the window centre.  A
+// kernel pairs when its weight is positive and, after the phase a that
+// maximises its conjugate-symmetric part, the rest carries at most 1e-7 of
+// its energy: with s = sum_f K(f) K(-f), n = sum_f |K(f)|^2, that rest is
+// (1 - |s|/n) / 2 and a = -arg(s) / 2.  The kernel is replaced by its
+// conjugate-symmetric part (the dropped part changes its intensity by that
+// fraction, second order in the residual field).  A real kernel of even
+// parity gives a real symmetric part, one of odd parity an imaginary one; a
+// kernel pairs with the next unpaired one of its class (more energy in the
+// real or in the imaginary part), so each kernel of a pair keeps a component
+// of the entry, and with it its own 16-bit scale.  The other kernels keep
+// their own entries and weights.  Returns the number of pairs.
+static int pack_band_set(const std::vector<float2>& h, const float* scales, int nk, size_t per,
+                         std::vector<float2>* hp, std::vector<float>* sp) {
+    using cd = std::complex<double>;
+    std::vector<std::vector<cd>> sym(nk);
+    std::vector<int> cls(nk, -1);  // -1: keeps its own entry
+    for (int k = 0; k < nk; k++) {
+        if (!(scales[k] > 0.f)) continue;
+        const float2* K = h.data() + k * per;
+        double n = 0.0;
+        cd s = 0.0;
+        for (size_t j = 0; j < per; j++) {
+            const cd a(K[j].x, K[j].y), m(K[per - 1 - j].x, K[per - 1 - j].y);
+            n += std::norm(a);
+            s += a * m;
+        }
+        if (!(n > 0.0) || (1.0 - std::abs(s) / n) / 2.0 > 1e-7) continue;
+        const cd ph = std::polar(1.0, -std::arg(s) / 2.0);
+        std::vector<cd>& v = sym[k];
+        v.resize(per);
+        double er = 0.0, ei = 0.0;
+        for (size_t j = 0; j < per; j++) {
+            const cd a = ph * cd(K[j].x, K[j].y), m = ph * cd(K[per - 1 - j].x, K[per - 1 - j].y);
+            v[j] = 0.5 * (a + std::conj(m));
+            er += v[j].real() * v[j].real();
+            ei += v[j].imag() * v[j].imag();
+        }
+        cls[k] = ei > er ? 1 : 0;
+    }
+    hp->clear();
+    sp->clear();
+    auto keep = [&](int k) {
+        hp->insert(hp->end(), h.begin() + k * per, h.begin() + (k + 1) * per);
+        sp->push_back(scales[k]);
+    };
+    std::vector<char> used(nk, 0);
+    int pairs = 0;
+    for (int k = 0; k < nk; k++) {
+        if (used[k]) continue;
+        used[k] = 1;
+        int q = -1;
+        if (cls[k] >= 0)
+            for (int r = k + 1; r < nk && q < 0; r++)
+                if (!used[r] && cls[r] == cls[k]) q = r;
+        if (q < 0) {
+            keep(k);
+            continue;
+        }
+        used[q] = 1;
+        const double ra = std::sqrt((double)scales[k]), rb = std::sqrt((double)scales[q]);
+        for (size_t j = 0; j < per; j++) {
+            const cd e = ra * sym[k][j] + cd(0.0, 1.0) * rb * sym[q][j];
+            hp->push_back(make_float2((float)e.real(), (float)e.imag()));
+        }
+        sp->push_back(1.0f);
+        pairs++;
+    }
+    return pairs;
+}
+
+extern "C" {
+
 // ---------------------------------------------------------------------------
 // Module 7
 // ---------------------------------------------------------------------------
@@ -760,65 +898,46 @@ int litho_bank_create(litho_ctx* c, const float* krn_r, const float* krn_i, cons
     const size_t per = (size_t)W * W;
     std::vector<float2> h((size_t)nk * per);
     for (size_t i = 0; i < h.size(); i++) h[i] = make_float2(krn_r[i], krn_i[i]);
-    std::unique_ptr<litho_bank> b(new litho_bank());
+    std::unique_ptr<litho_bank, int (*)(litho_bank*)> b(new litho_bank(), litho_bank_destroy);
     b->ctx = c;
     b->nk = nk;
     b->Nx = Nx;
     b->Ny = Ny;
     LITHO_HIP(hipSetDevice(c->device));
-    if (hipMalloc(&b->d_bank, h.size() * sizeof(float2)) != hipSuccess) return LITHO_ENOMEM;
-    if (hipMalloc(&b->d_scales, nk * sizeof(float)) != hipSuccess) {
-        (void)hipFree(b->d_bank);
-        return LITHO_ENOMEM;
-    }
-    LITHO_HIP(hipMemcpyAsync(b->d_bank, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice,
-                             c->stream));
-    LITHO_HIP(hipMemcpyAsync(b->d_scales, scales, nk * sizeof(float), hipMemcpyHostToDevice,
-                             c->stream));
-    // bf16 copy with one scale per kernel (max |re|, |im|), round to nearest even
-    std::vector<unsigned> h16(h.size());
-    std::vector<float> ks(nk, 1.0f);
-    auto bf16 = [](float f) -> unsigned {
-        unsigned u;
-        std::memcpy(&u, &f, 4);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return u >> 16;
-    };
-    for (int k = 0; k < nk; k++) {
-        float m = 0.f;
-        for (size_t j = 0; j < per; j++)
-            m = std::max(m, std::max(std::fabs(h[k * per + j].x), std::fabs(h[k * per + j].y)));
-        ks[k] = m > 0.f ? m : 1.0f;
-        for (size_t j = 0; j < per; j++)
-            h16[k * per + j] = bf16(h[k * per + j].x / ks[k]) | (bf16(h[k * per + j].y / ks[k]) << 16);
-    }
-    std::vector<__half2> hh(h.size());
-    for (int k = 0; k < nk; k++)
-        for (size_t j = 0; j < per; j++)
-            hh[k * per + j] = __floats2half2_rn(h[k * per + j].x / ks[k], h[k * per + j].y / ks[k]);
-    if (hipMalloc(&b->d_bank16, h16.size() * sizeof(unsigned)) != hipSuccess ||
-        hipMalloc(&b->d_bankh, hh.size() * sizeof(__half2)) != hipSuccess ||
-        hipMalloc(&b->d_kscale, nk * sizeof(float)) != hipSuccess)
-        return LITHO_ENOMEM;
-    LITHO_HIP(hipMemcpyAsync(b->d_bankh, hh.data(), hh.size() * sizeof(__half2),
-                             hipMemcpyHostToDevice, c->stream));
-    LITHO_HIP(hipMemcpyAsync(b->d_bank16, h16.data(), h16.size() * sizeof(unsigned),
-                             hipMemcpyHostToDevice, c->stream));
-    LITHO_HIP(hipMemcpyAsync(b->d_kscale, ks.data(), nk * sizeof(float), hipMemcpyHostToDevice,
-                             c->stream));
+    LITHO_TRY(upload_set(c, h, std::vector<float>(scales, scales + nk), per, &b->full));
+    std::vector<float2> hp;
+    std::vector<float> sp;
+    b->pairs = pack_band_set(h, scales, nk, per, &hp, &sp);
+    if (b->pairs)
+        LITHO_TRY(upload_set(c, hp, sp, per, &b->band));
+    else
+        b->band = b->full;
     LITHO_HIP(hipStreamSynchronize(c->stream));
     *out = b.release();
     return LITHO_OK;
 }
 
+static void free_set(KernelSet& s) {
+    if (s.d_bank) (void)hipFree(s.d_bank);
+    if (s.d_scales) (void)hipFree(s.d_scales);
+    if (s.d_bank16) (void)hipFree(s.d_bank16);
+    if (s.d_bankh) (void)hipFree(s.d_bankh);
+    if (s.d_kscale) (void)hipFree(s.d_kscale);
+    s = KernelSet();
+}
+
 int litho_bank_destroy(litho_bank* b) {
     if (!b) return LITHO_EARG;
-    if (b->d_bank) (void)hipFree(b->d_bank);
-    if (b->d_scales) (void)hipFree(b->d_scales);
-    if (b->d_bank16) (void)hipFree(b->d_bank16);
-    if (b->d_bankh) (void)hipFree(b->d_bankh);
-    if (b->d_kscale) (void)hipFree(b->d_kscale);
+    if (b->band.d_bank != b->full.d_bank) free_set(b->band);
+    free_set(b->full);
     delete b;
+    return LITHO_OK;
+}
+
+int litho_bank_info(const litho_bank* b, int* entries, int* pairs) {
+    if (!b) return LITHO_EARG;
+    if (entries) *entries = b->band.n;
+    if (pairs) *pairs = b->pairs;
     return LITHO_OK;
 }
 
@@ -852,8 +971,8 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         LITHO_HIP(hipEventRecord(c->ev[1], st));
         LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
         LITHO_HIP(hipEventRecord(c->ev[2], st));
-        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->d_bank, nullptr, 0, bank->d_scales,
-                                   bank->nk, Nx, B, (float2*)C, 2, st));
+        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->band.d_bank, nullptr, 0, bank->band.d_scales,
+                                   bank->band.n, Nx, B, (float2*)C, 2, st));
         LITHO_HIP(hipEventRecord(c->ev[3], st));
         LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
         LITHO_HIP(hipEventRecord(c->ev[4], st));
@@ -879,7 +998,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     LITHO_HIP(hipEventRecord(c->ev[1], st));
     LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
     LITHO_HIP(hipEventRecord(c->ev[2], st));
-    LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->d_bank, bank->d_scales, bank->nk, Nx, B,
+    LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->full.d_bank, bank->full.d_scales, bank->full.n, Nx, B,
                                 (float*)T, 0, c->tw, st));
     LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
@@ -1024,9 +1143,9 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     const bool out16 = (prec & LITHO_PREC_IMAGE_FP16) != 0;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
     const int bfmt = (prec & LITHO_PREC_BANK_BF16) ? 1 : ((prec & LITHO_PREC_BANK_FP16) ? 2 : 0);
-    const void* kb = bfmt == 1 ? (const void*)bank->d_bank16
-                               : (bfmt == 2 ? (const void*)bank->d_bankh : (const void*)bank->d_bank);
-    const float* ks = bfmt ? bank->d_kscale : nullptr;
+    const void* kb = bfmt == 1 ? (const void*)bank->band.d_bank16
+                               : (bfmt == 2 ? (const void*)bank->band.d_bankh : (const void*)bank->band.d_bank);
+    const float2* ks = bfmt ? bank->band.d_kscale : nullptr;
     const void *tm = nullptr, *ti = nullptr;
     if (mfma) {
         auto a = c->tables.find(mask_table_key(N, Nx));
@@ -1042,7 +1161,7 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
                                     st));
     LITHO_TRY(
         launch_mask_cols_t(N, w.R, (float2*)w.M, B, Nx + 1, -Nx, W, mixed, pl.lean_mem, st, W));
-    LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->d_scales, bank->nk, Nx, B,
+    LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->band.d_scales, bank->band.n, Nx, B,
                                (float2*)w.C, pl.socs_shape, st));
     if (mfma) {
         // FP32 U: the image-side product splits it into hi + lo parts
@@ -1400,7 +1519,7 @@ int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn
     LITHO_TRY(dM.alloc(win.size() * sizeof(float2)));
     LITHO_TRY(dT.alloc((size_t)G * G * sizeof(float)));
     LITHO_HIP(hipMemcpyAsync(dM.p, win.data(), win.size() * sizeof(float2), hipMemcpyHostToDevice, st));
-    LITHO_TRY(launch_socs_accum(G, dM.as<float2>(), bank->d_bank, bank->d_scales, nk, nx, 1,
+    LITHO_TRY(launch_socs_accum(G, dM.as<float2>(), bank->full.d_bank, bank->full.d_scales, nk, nx, 1,
                                 dT.as<float>(), 1, c->tw, st));
     std::vector<float> h((size_t)G * G);
     LITHO_HIP(hipMemcpyAsync(h.data(), dT.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));

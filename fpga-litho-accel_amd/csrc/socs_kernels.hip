@@ -68,7 +68,8 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
     return __uint_as_float(((unsigned)h) << 16);
 }
 
-// BK = 0: float2; 1: bf16 pairs x kscale[k]; 2: fp16 pairs (half2) x kscale[k].
+// BK = 0: float2; 1: bf16 pairs x kscale[k]; 2: fp16 pairs (half2) x kscale[k]; kscale[k]
+// holds one scale for the real and one for the imaginary parts of entry k.
 // ld = dec(raw(...)); raw / dec apart let a kernel prefetch the stored word.
 template <int BK>
 struct Bank;
@@ -78,8 +79,8 @@ struct Bank<0> {
     __device__ static __forceinline__ R raw(const void* b, long idx) {
         return ((const float2*)b)[idx];
     }
-    __device__ static __forceinline__ float2 dec(R v, float) { return v; }
-    __device__ static __forceinline__ float2 ld(const void* b, const float*, long idx, int) {
+    __device__ static __forceinline__ float2 dec(R v, float2) { return v; }
+    __device__ static __forceinline__ float2 ld(const void* b, const float2*, long idx, int) {
         return raw(b, idx);
     }
 };
@@ -89,11 +90,11 @@ struct Bank<1> {
     __device__ static __forceinline__ R raw(const void* b, long idx) {
         return ((const unsigned*)b)[idx];
     }
-    __device__ static __forceinline__ float2 dec(R v, float s) {
-        return make_float2(bf16_to_f32((unsigned short)(v & 0xffffu)) * s,
-                           bf16_to_f32((unsigned short)(v >> 16)) * s);
+    __device__ static __forceinline__ float2 dec(R v, float2 s) {
+        return make_float2(bf16_to_f32((unsigned short)(v & 0xffffu)) * s.x,
+                           bf16_to_f32((unsigned short)(v >> 16)) * s.y);
     }
-    __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
+    __device__ static __forceinline__ float2 ld(const void* b, const float2* ks, long idx, int k) {
         return dec(raw(b, idx), ks[k]);
     }
 };
@@ -103,11 +104,11 @@ struct Bank<2> {
     __device__ static __forceinline__ R raw(const void* b, long idx) {
         return ((const unsigned*)b)[idx];
     }
-    __device__ static __forceinline__ float2 dec(R v, float s) {
+    __device__ static __forceinline__ float2 dec(R v, float2 s) {
         const float2 f = __half22float2(__builtin_bit_cast(__half2, v));
-        return make_float2(f.x * s, f.y * s);
+        return make_float2(f.x * s.x, f.y * s.y);
     }
-    __device__ static __forceinline__ float2 ld(const void* b, const float* ks, long idx, int k) {
+    __device__ static __forceinline__ float2 ld(const void* b, const float2* ks, long idx, int k) {
         return dec(raw(b, idx), ks[k]);
     }
 };
@@ -719,7 +720,7 @@ __global__ __launch_bounds__(NTH) void image_band1024_kernel(const void* __restr
 template <int P, int NZ, int SLOTS, int BK = 0>
 __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __restrict__ M,
                                                               const void* __restrict__ bank,
-                                                              const float* __restrict__ kscale,
+                                                              const float2* __restrict__ kscale,
                                                               const float* __restrict__ scales,
                                                               int nk, int Nx,
                                                               float2* __restrict__ Cout) {
@@ -814,7 +815,8 @@ __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __re
 // Mirror mode (mw = 2h + 1 > 0, cx = h + 1, kymin = -h, cy = mw): R holds only
 // kx = 0..h of a real mask's spectrum, and M [B][mw][mw] gets both halves
 // through M(-ky, -kx) = conj(M(ky, kx)) (the (-1)^(kx+ky) centring is real);
-// the kx = 0 column is computed for every ky, so it is written once.
+// in the kx = 0 column the ky < 0 half is the conjugate of the ky > 0 half too,
+// so M is exactly conjugate-symmetric about the window centre.
 template <int P, int Q, int NTH, bool H16 = false>
 __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict__ R_,
                                                          float2* __restrict__ M, int cx,
@@ -847,8 +849,10 @@ __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict_
         if (j >= 0 && j < cy) {
             const float2 v = cscale(val, (ky & 1) ? -inv : inv);
             if (mw) {
-                Mb[(long)j * mw + h + col] = v;
-                if (col > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(v.x, -v.y);
+                // column kx = 0: ky >= 0 computed, ky < 0 written as the conjugate,
+                // so M(-f) = conj(M(f)) holds exactly
+                if (col > 0 || ky >= 0) Mb[(long)j * mw + h + col] = v;
+                if (col > 0 || ky > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(v.x, -v.y);
             } else {
                 Mb[(long)j * cx + col] = v;
             }
@@ -939,8 +943,10 @@ __global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __r
         if (j >= 0 && j < cy) {
             const float2 val = cscale(u[k2], (ky & 1) ? -inv : inv);
             if (mw) {
-                Mb[(long)j * mw + h + col] = val;
-                if (col > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(val.x, -val.y);
+                // column kx = 0: ky >= 0 computed, ky < 0 written as the conjugate,
+                // so M(-f) = conj(M(f)) holds exactly
+                if (col > 0 || ky >= 0) Mb[(long)j * mw + h + col] = val;
+                if (col > 0 || ky > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(val.x, -val.y);
             } else {
                 Mb[(long)j * cx + col] = val;
             }
@@ -1397,7 +1403,7 @@ __device__ __forceinline__ void stage1_regs(int t, float2 (&v)[P], const float2 
 template <int P, int NZ, int NW, int BK = 0>
 __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __restrict__ M,
                                                             const void* __restrict__ bank,
-                                                            const float* __restrict__ kscale,
+                                                            const float2* __restrict__ kscale,
                                                             const float* __restrict__ scales,
                                                             int nk, int Nx,
                                                             float2* __restrict__ Cout) {
@@ -1459,7 +1465,7 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
 
     for (int k = 0; k < nk; k++) {
         const float lam = scales[k];
-        const float ksc = BK ? kscale[k] : 1.f;
+        const float2 ksc = BK ? kscale[k] : make_float2(1.f, 1.f);
         float2 xin[CR][NZ];
 #pragma unroll
         for (int cr = 0; cr < CR; cr++)
@@ -1550,7 +1556,7 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
 }
 
 template <int P, int NZ, int NW, int BK>
-static int launch_wave(const float2* M, const void* bank, const float* kscale, const float* scales,
+static int launch_wave(const float2* M, const void* bank, const float2* kscale, const float* scales,
                        int nk, int Nx, int B, float2* C, hipStream_t st) {
     using Geo = Fft2Geom<P, P>;
     constexpr int SL = (64 / P) * NW;
@@ -1564,7 +1570,7 @@ static int launch_wave(const float2* M, const void* bank, const float* kscale, c
 }
 
 template <int P, int NZ, int SLOTS, int BK>
-static int launch_band(const float2* M, const void* bank, const float* kscale, const float* scales,
+static int launch_band(const float2* M, const void* bank, const float2* kscale, const float* scales,
                        int nk, int Nx, int B, float2* C, hipStream_t st) {
     using Geo = Fft2Geom<P, P>;
     const int W = 2 * Nx + 1;
@@ -1576,7 +1582,7 @@ static int launch_band(const float2* M, const void* bank, const float* kscale, c
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
-int launch_socs_band(int G, const float2* M, const void* bank, const float* kscale, int bank_fmt,
+int launch_socs_band(int G, const float2* M, const void* bank, const float2* kscale, int bank_fmt,
                      const float* scales, int nk, int Nx, int B, float2* C, int shape,
                      hipStream_t st) {
     const int W = 2 * Nx + 1;

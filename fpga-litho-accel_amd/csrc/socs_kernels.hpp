@@ -40,10 +40,11 @@ int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, in
 int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
                        bool lean, hipStream_t st, int mw = 0);
 // M x bank -> band spectrum C [B][4Nx+1][2Nx+1] of the intensity, /G^2.
-// bank_fmt 0: float2; 1: bf16 (re, im) pairs x kscale[k]; 2: fp16 pairs x kscale[k].
+// bank_fmt 0: float2; 1: bf16 (re, im) pairs; 2: fp16 pairs; kscale[k] scales the real (x)
+// and imaginary (y) parts of entry k.
 // shape 0: 48-slot blocks, 1: 24-slot (lean) blocks at G = 144, 2: wave-synced
 // transforms (socs_wave_kernel)
-int launch_socs_band(int G, const float2* M, const void* bank, const float* kscale, int bank_fmt,
+int launch_socs_band(int G, const float2* M, const void* bank, const float2* kscale, int bank_fmt,
                      const float* scales, int nk, int Nx, int B, float2* C, int shape,
                      hipStream_t st);
 // C -> U [B][2Nx+1][N]

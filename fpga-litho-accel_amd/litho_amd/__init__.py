@@ -32,7 +32,8 @@ EXPORTS = [
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
     "litho_image_tcc",
-    "litho_bank_create", "litho_bank_destroy", "litho_socs_image_batch", "litho_ctx_set_path",
+    "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
+    "litho_ctx_set_path",
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
     "litho_tcc_build_prec", "litho_kernels_from_source_prec",
     "litho_socs_image_buckets", "litho_bucket_times", "litho_autotune", "litho_plan_get",
@@ -138,6 +139,7 @@ def lib():
         L.litho_image_tcc.argtypes = [_f32p, i, _f64p, _f64p, i, i, _f32p]
         L.litho_bank_create.argtypes = [vp, _f32p, _f32p, _f32p, i, i, i, C.POINTER(vp)]
         L.litho_bank_destroy.argtypes = [vp]
+        L.litho_bank_info.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
         L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
         L.litho_last_timing.argtypes = [vp, C.POINTER(f), C.POINTER(f), i]
         L.litho_ctx_set_path.argtypes = [vp, i]
@@ -357,6 +359,14 @@ class Context:
         _check(lib().litho_bank_create(self.h, kr, ki, sc, sc.size, Nx, Ny, C.byref(b)),
                "litho_bank_create")
         return b
+
+    @staticmethod
+    def bank_info(bank):
+        """(entries, pairs) of the bank's mask -> image form: kernels whose
+        fields are real on a real mask share one transform two at a time."""
+        e, p = C.c_int(0), C.c_int(0)
+        _check(lib().litho_bank_info(bank, C.byref(e), C.byref(p)), "litho_bank_info")
+        return e.value, p.value
 
     def socs_image_batch(self, bank, d_mask, B, N, d_image):
         _check(lib().litho_socs_image_batch(self.h, bank, d_mask, B, N, d_image),

@@ -150,10 +150,19 @@ int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double
                     int Ny, float* image);
 
 /* ---- device-resident batched path (the bench path) ------------------------- */
-/* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin). */
+/* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin).  The
+ * mask -> image paths below run a second form of the set: a kernel K with
+ * K(-f) = e^{ib} conj(K(f)) about the window centre (zero defocus with a
+ * point-symmetric source gives such kernels) has a real field on a real
+ * mask, so two of them share one transform as sqrt(la) Ka + i sqrt(lb) Kb;
+ * the other kernels keep their own entries.  calcSOCS-from-spectrum entry
+ * points use the set as given. */
 int litho_bank_create(litho_ctx* ctx, const float* krn_r, const float* krn_i,
                       const float* scales, int nk, int Nx, int Ny, litho_bank** bank);
 int litho_bank_destroy(litho_bank* bank);
+/* Entries of the mask -> image form of the set and how many are kernel pairs
+ * (entries = nk - pairs).  No counterpart in the reference. */
+int litho_bank_info(const litho_bank* bank, int* entries, int* pairs);
 /* B masks (device, B x N x N f32) -> B images (device), asynchronous on the
  * context stream.  Workspaces are grown by the context as needed. */
 int litho_socs_image_batch(litho_ctx* ctx, const litho_bank* bank, const float* d_mask, int B,

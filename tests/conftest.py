@@ -49,7 +49,7 @@ def max_rel(a, b):
                  max(np.abs(b).max(), 1e-300))
 
 
-# BASELINE-shape optics: lambda 193 nm, NA 0.85, zero defocus, 41x41 annulus
+# Production-clip optics: lambda 193 nm, NA 0.85, zero defocus, 41x41 annulus
 # sigma 0.6-0.9, 4 nm pixels (period = 4 N).
 BASE_LAMBDA = 193.0
 BASE_NA = 0.85

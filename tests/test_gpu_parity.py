@@ -433,7 +433,7 @@ def test_tcc_direct_image_reference_fixture(gpu, orc, fixture_tcc):
 
 
 def test_tcc_direct_image_c1_and_full_rank_socs(gpu, orc, c1):
-    """TCC-direct image on the 512^2 BASELINE shape vs the oracle, and the
+    """TCC-direct image on a 512^2 production-optics clip vs the oracle, and the
     SOCS image with every kernel (K = n via the explicit route) converging to it."""
     case("tcc-direct image C1")
     T = orc.calc_tcc(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
