@@ -41,7 +41,8 @@ def log(msg):
 
 def stage_alg_bytes(N, Nx, G, K, B, band=True):
     """Algorithmic HBM bytes per launch of each stage: its inputs and outputs
-    (intermediates of this design counted once each way).  The band path keeps
+    (intermediates of this design counted once each way); K = kernel-bank
+    entries the SOCS stage reads.  The band path keeps
     kx = 0..Nx of each mask row (a real mask's spectrum is Hermitian), the
     nextPow2 path the whole window."""
     W = 2 * Nx + 1
@@ -297,7 +298,9 @@ def main():
     log(f"{images_per_s:.1f} images/s, {ms_per_step:.3f} ms/step, stages(ms) "
         + ", ".join(f"{n}={t:.3f}" for n, t in stage_acc.items()))
     Gimg = G if ("socs_accum" in stage_acc) else next(g for g in (36, 81, 144) if g >= 4 * Nx + 1)
-    alg = stage_alg_bytes(N, Nx, Gimg, NK, B, band="socs_band" in stage_acc)
+    band = "socs_band" in stage_acc
+    entries, pairs = ctx.bank_info(bank)  # kernels sharing one transform in the band path
+    alg = stage_alg_bytes(N, Nx, Gimg, entries if band else NK, B, band=band)
     kernels = {n: {"ms": round(float(t), 4),
                    "alg_bytes": int(alg[n]),
                    "GBps": round(alg[n] / (t * 1e-3) / 1e9, 1)}
@@ -383,6 +386,7 @@ def main():
         "config": {"workload": "plain FP32 SOCS, 1024^2 Manhattan clips at 4 nm, NA 0.85, "
                                "41^2 annulus 0.6-0.9, Nx=34, K=24",
                    "N": N, "Nx": Nx, "K": NK, "conv_grid": Gimg, "batch": B,
+                   "socs_entries": entries if band else NK, "kernel_pairs": pairs if band else 0,
                    "form": "plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
