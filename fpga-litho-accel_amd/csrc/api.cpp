@@ -54,6 +54,7 @@ struct litho_ctx {
     float2* tw = nullptr;
     DevBuf ws[8];
     hipEvent_t ev[NSTAGE + 1] = {};
+    bool timing = true;  // record per-stage events in litho_socs_image_batch
     bool timed = false;
     int nstage = 0;
     const char* stage_names[NSTAGE] = {};
@@ -965,22 +966,22 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
         LITHO_TRY(ctx_ws(c, 4, (size_t)B * (band + 1) * N * sizeof(float2), &U));
         static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
                                       "image_rows"};
-        LITHO_HIP(hipEventRecord(c->ev[0], st));
+        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
         // a real mask: kx = 0..Nx of the row spectra, the rest by symmetry
         LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, 0, Nx + 1, false, false, false, st));
-        LITHO_HIP(hipEventRecord(c->ev[1], st));
+        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
         LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
-        LITHO_HIP(hipEventRecord(c->ev[2], st));
+        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
         LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->band.d_bank, nullptr, 0, bank->band.d_scales,
                                    bank->band.n, Nx, B, (float2*)C, 2, st));
-        LITHO_HIP(hipEventRecord(c->ev[3], st));
+        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
         LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
-        LITHO_HIP(hipEventRecord(c->ev[4], st));
+        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
         LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, false, st));
-        LITHO_HIP(hipEventRecord(c->ev[5], st));
+        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
         c->nstage = 5;
         for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
-        c->timed = true;
+        c->timed = c->timing;
         return LITHO_OK;
     }
     const int G = next_pow2(4 * Nx + 1);
@@ -993,23 +994,23 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     LITHO_TRY(ctx_ws(c, 4, (size_t)B * N * (band + 1) * sizeof(float2), &U));
     static const char* names[] = {"mask_rows", "mask_cols", "socs_accum", "fi_rows", "fi_cols",
                                   "image_rows"};
-    LITHO_HIP(hipEventRecord(c->ev[0], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
     LITHO_TRY(launch_rows_dft(N, d_mask, (float2*)R, (long)B * N, -Nx, W, true, c->tw, st));
-    LITHO_HIP(hipEventRecord(c->ev[1], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
     LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
-    LITHO_HIP(hipEventRecord(c->ev[2], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
     LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->full.d_bank, bank->full.d_scales, bank->full.n, Nx, B,
                                 (float*)T, 0, c->tw, st));
-    LITHO_HIP(hipEventRecord(c->ev[3], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
-    LITHO_HIP(hipEventRecord(c->ev[4], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
     LITHO_TRY(launch_fi_cols(G, N, (float2*)P, (float2*)U, B, band, 0, c->tw, st));
-    LITHO_HIP(hipEventRecord(c->ev[5], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
     LITHO_TRY(launch_image_rows(N, (float2*)U, d_image, (long)B * N, band, c->tw, st));
-    LITHO_HIP(hipEventRecord(c->ev[6], st));
+    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[6], st));
     c->nstage = 6;
     for (int i = 0; i < 6; i++) c->stage_names[i] = names[i];
-    c->timed = true;
+    c->timed = c->timing;
     return LITHO_OK;
 }
 
@@ -1456,6 +1457,12 @@ int litho_plan_get(litho_ctx* c, int N, int Nx, int nk, int precision, int* G, i
     *socs_shape = p.socs_shape;
     *lean_mem = p.lean_mem;
     return c->plans.count(std::make_tuple(N, Nx, nk, precision)) ? 1 : 0;
+}
+
+int litho_ctx_set_timing(litho_ctx* c, int on) {
+    if (!c) return LITHO_EARG;
+    c->timing = on != 0;
+    return LITHO_OK;
 }
 
 int litho_last_timing(litho_ctx* c, float* total_ms, float* stage_ms, int nstage) {

@@ -33,7 +33,7 @@ EXPORTS = [
     "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
     "litho_image_tcc",
     "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
-    "litho_ctx_set_path",
+    "litho_ctx_set_path", "litho_ctx_set_timing",
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
     "litho_tcc_build_prec", "litho_kernels_from_source_prec",
     "litho_socs_image_buckets", "litho_bucket_times", "litho_autotune", "litho_plan_get",
@@ -143,6 +143,7 @@ def lib():
         L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
         L.litho_last_timing.argtypes = [vp, C.POINTER(f), C.POINTER(f), i]
         L.litho_ctx_set_path.argtypes = [vp, i]
+        L.litho_ctx_set_timing.argtypes = [vp, i]
         L.litho_socs_image_batch_pipelined.argtypes = [vp, vp, vp, i, i, vp, i, i, i]
         L.litho_stage_name.argtypes = [vp, i]
         L.litho_stage_name.restype = C.c_char_p
@@ -423,6 +424,10 @@ class Context:
 
     def set_path(self, path):
         _check(lib().litho_ctx_set_path(self.h, path), "litho_ctx_set_path")
+
+    def set_timing(self, on):
+        """Per-stage HIP events around batch launches (on by default)."""
+        _check(lib().litho_ctx_set_timing(self.h, 1 if on else 0), "litho_ctx_set_timing")
 
     def last_timing(self, nstage=8):
         """(total ms, {stage name: ms}) of the last batch call."""

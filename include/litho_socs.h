@@ -174,6 +174,10 @@ int litho_socs_image_batch(litho_ctx* ctx, const litho_bank* bank, const float* 
 #define LITHO_PATH_AUTO 0
 #define LITHO_PATH_POW2 1
 int litho_ctx_set_path(litho_ctx* ctx, int path);
+/* Per-stage HIP events around the launches of litho_socs_image_batch (on by
+ * default; off, a batch records none and litho_last_timing reports no
+ * timing).  No counterpart in the reference. */
+int litho_ctx_set_timing(litho_ctx* ctx, int on);
 /* Timing on the context stream of the last litho_socs_image_batch call:
  * total and per-stage device time (ms).  Returns the number of stages (>= 0)
  * or a negative status. */

@@ -219,6 +219,13 @@ def test_band_and_pow2_grids_agree(gpu, orc, base_source, N):
             outs.append(ctx.d2h(np.zeros_like(masks), di))
             _, stages = ctx.last_timing()
             assert ("socs_band" in stages) == (path == gpu.LITHO_PATH_AUTO)
+            # without per-stage events: the same images, and no timing to report
+            ctx.set_timing(False)
+            ctx.socs_image_batch(bank, dm, 2, N, di)
+            assert np.array_equal(ctx.d2h(np.zeros_like(masks), di), outs[-1])
+            with pytest.raises(gpu.LithoError):
+                ctx.last_timing()
+            ctx.set_timing(True)
         ctx.free(dm)
         ctx.free(di)
         gpu.lib().litho_bank_destroy(bank)
