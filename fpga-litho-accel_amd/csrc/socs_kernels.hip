@@ -490,31 +490,15 @@ __device__ __forceinline__ void transpose32_planes(int t, float* __restrict__ xb
 // skips the natural-order spectrum in LDS: lane t of a transform holds
 // X[t + 32 k2] in u[k2] after stage 2, and X[N - k] comes from lane (32 - t) mod 32
 // by ds_bpermute (u[31] for k < 32, u[30] for 32 <= k < 64).
-template <bool H16, bool IN8, int NTH = 256>
-__global__ __launch_bounds__(NTH) void rows_band1024_kernel(const void* __restrict__ in_,
-                                                            void* __restrict__ out_, int nrows,
-                                                            int count) {
-    using Geo = Fft2Geom<32, 32>;
-    using S = Store<H16>;
-    constexpr int L = 1024, T = 32, FPB = NTH / T;
-    typename S::T* out = (typename S::T*)out_;
-    __shared__ float2 twG[L];
-    __shared__ float xplane[FPB * Geo::XBUF];
-    const int tid = threadIdx.x;
-    const int f = tid / T, t = tid - f * T;
-    const long npairs = nrows / 2;
-    const long p = (long)blockIdx.x * FPB + f;
-    const bool act = p < npairs;
-    fill_twiddles<L>(twG, tid, NTH);
-    __syncthreads();
-    float* xb = xplane + f * Geo::XBUF;
-    // an idle transform (past the last row pair) reads pair 0 and writes nothing;
-    // loading unconditionally keeps the 64 loads as one base plus immediate offsets
-    const long roff = (2 * (act ? p : 0)) * (long)L;
+//
+// Row pair p of lane t: stage-1 inputs x[32 n1 + t] of both rows as re / im.
+template <bool IN8>
+__device__ __forceinline__ void rows_band1024_load(const void* __restrict__ in_, long p, int t,
+                                                   float2 (&v)[32]) {
+    constexpr int L = 1024;
+    const long roff = 2 * p * (long)L;
     const unsigned char* r8 = (const unsigned char*)in_ + roff + t;
     const float* r0 = (const float*)in_ + roff + t;
-    // stage 1 for n2 = t: DFT over n1 of x[32 n1 + t], two real rows as re / im
-    float2 v[32];
 #pragma unroll
     for (int n1 = 0; n1 < 32; n1++) {
         const int n = 32 * n1;
@@ -526,6 +510,19 @@ __global__ __launch_bounds__(NTH) void rows_band1024_kernel(const void* __restri
                                 __builtin_nontemporal_load(r0 + L + n));
         }
     }
+}
+
+// The transform of one row pair from its loaded inputs v (clobbered) to the
+// window columns of R.  Every thread of the block calls it (block barriers);
+// act = false computes and writes nothing.
+template <bool H16>
+__device__ __forceinline__ void rows_band1024_fft(int tid, int t, const float2* __restrict__ twG,
+                                                  float* __restrict__ xb, float2 (&v)[32],
+                                                  bool act, long p, void* __restrict__ out_,
+                                                  int count) {
+    constexpr int L = 1024;
+    using S = Store<H16>;
+    typename S::T* out = (typename S::T*)out_;
     dft_reg<32, false>(v);
 #pragma unroll
     for (int k1 = 1; k1 < 32; k1++) v[k1] = cmul_w<false>(v[k1], twG[t * k1]);
@@ -559,6 +556,29 @@ __global__ __launch_bounds__(NTH) void rows_band1024_kernel(const void* __restri
         o[0] = S::st(x1);
         o[1] = S::st(x2);
     }
+}
+
+// One row pair per transform, FPB = NTH / 32 transforms per block.  An idle
+// transform (past the last row pair) reads pair 0 and writes nothing, so the
+// 64 loads are one base plus immediate offsets.
+template <bool H16, bool IN8, int NTH = 256>
+__global__ __launch_bounds__(NTH) void rows_band1024_kernel(const void* __restrict__ in_,
+                                                            void* __restrict__ out_, int nrows,
+                                                            int count) {
+    using Geo = Fft2Geom<32, 32>;
+    constexpr int L = 1024, T = 32, FPB = NTH / T;
+    __shared__ float2 twG[L];
+    __shared__ float xplane[FPB * Geo::XBUF];
+    const int tid = threadIdx.x;
+    const int f = tid / T, t = tid - f * T;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
+    float2 v[32];
+    rows_band1024_load<IN8>(in_, act ? p : 0, t, v);
+    rows_band1024_fft<H16>(tid, t, twG, xplane + f * Geo::XBUF, v, act, p, out_, count);
 }
 
 // image_rows2: same contract as image_rows.  The Hermitian row spectrum
