@@ -462,6 +462,14 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__
     }
 }
 
+// LDS written by some lanes of a wavefront and read by others of the same
+// wavefront: a wavefront-scope fence orders it (no block barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Stage transpose of a 32 x 32 two-stage FFT through a float plane buffer xb
 // (32 rows of pitch 33), one component at a time: lane t holds the twiddled
 // stage-1 outputs v[k1] of n2 = t and receives the stage-2 inputs u[n2] of
@@ -724,6 +732,217 @@ __global__ __launch_bounds__(NTH) void image_band1024_kernel(const void* __restr
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Band row transforms of 16 x Q points (N = 256, 512, 768 for Q = 16, 32, 48)
+// in 16-lane transforms, four per wavefront.  Lane t of a transform is the
+// stage-1 index n2 = t + 16 s (s < Q / 16) and holds x[Q n1 + n2], n1 < 16;
+// after the stage transpose it is k1 = t and holds X[t + 16 k2], k2 < Q.  The
+// transpose goes through LDS one component plane at a time (16 rows of pitch
+// Q + 1 floats per transform) and stays inside the wavefront, so it needs no
+// block barrier.
+// ---------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+
+template <int Q>
+__device__ __forceinline__ void transpose16_planes(int t, float* __restrict__ xb,
+                                                   const float2 (&v)[Q / 16][16],
+                                                   float2 (&u)[Q]) {
+    constexpr int LDQ = Q + 1, S1 = Q / 16;
+#pragma unroll
+    for (int s = 0; s < S1; s++)
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) xb[k1 * LDQ + t + 16 * s] = v[s][k1].x;
+    wave_sync();
+#pragma unroll
+    for (int n2 = 0; n2 < Q; n2++) u[n2].x = xb[t * LDQ + n2];
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < S1; s++)
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) xb[k1 * LDQ + t + 16 * s] = v[s][k1].y;
+    wave_sync();
+#pragma unroll
+    for (int n2 = 0; n2 < Q; n2++) u[n2].y = xb[t * LDQ + n2];
+    wave_sync();
+}
+
+// rows_band16: rows_fwd2 for N = 16 Q point rows and the band window
+// kx = 0..count-1 (count <= 32), transposed output with the centring sign.  Two
+// real rows per complex transform; X[N - k] comes from lane (16 - t) mod 16 of
+// the transform by ds_bpermute (u[Q-1] for k < 16, u[Q-2] for 16 <= k < 32).
+template <int Q, bool H16, bool IN8, int NTH>
+__global__ __launch_bounds__(NTH) void rows_band16_kernel(const void* __restrict__ in_,
+                                                          void* __restrict__ out_, int nrows,
+                                                          int count) {
+    constexpr int L = 16 * Q, S1 = Q / 16, FPB = NTH / 16, LDQ = Q + 1;
+    using S = Store<H16>;
+    typename S::T* out = (typename S::T*)out_;
+    __shared__ float2 twG[L];
+    __shared__ float xplane[FPB * 16 * LDQ];
+    const int tid = threadIdx.x;
+    const int f = tid >> 4, t = tid & 15;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
+    float* xb = xplane + f * 16 * LDQ;
+    // an idle transform reads pair 0 (in bounds) and writes nothing
+    const long roff = 2 * (act ? p : 0) * (long)L;
+    const unsigned char* r8 = (const unsigned char*)in_ + roff + t;
+    const float* r0 = (const float*)in_ + roff + t;
+    // A transform reads 64 bytes of a row per load, so the S1 pieces of one
+    // row segment are loaded back to back, and through the caches (streaming
+    // loads of half lines measured 1.2-1.5x slower)
+    float2 v[S1][16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++)
+#pragma unroll
+        for (int s = 0; s < S1; s++) {
+            const int n = Q * n1 + 16 * s;
+            if (IN8)
+                v[s][n1] = make_float2((float)r8[n] * (1.0f / 255.0f), (float)r8[L + n] * (1.0f / 255.0f));
+            else
+                v[s][n1] = make_float2(r0[n], r0[L + n]);
+        }
+#pragma unroll
+    for (int s = 0; s < S1; s++) {
+        dft_reg<16, false>(v[s]);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; k1++) v[s][k1] = cmul_w<false>(v[s][k1], twG[(t + 16 * s) * k1]);
+    }
+    float2 u[Q];
+    transpose16_planes<Q>(t, xb, v, u);
+    dft_reg<Q, false>(u);  // u[k2] = X[t + 16 k2]
+    const int lane = tid & 63;
+    const int src = ((lane & 48) | ((16 - t) & 15)) << 2;
+    const float2 m1 = make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[Q - 1].x))),
+                                  __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[Q - 1].y))));
+    const float2 m2 = make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[Q - 2].x))),
+                                  __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(u[Q - 2].y))));
+    if (!act) return;
+    const long r = 2 * p, img = r / L, y = r - img * L;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int k = t + 16 * h;  // window column j = k (kmin = 0)
+        if (k >= count) break;
+        const float2 z1 = h ? u[1] : u[0];
+        const float2 zm = h ? (t == 0 ? m1 : m2) : (t == 0 ? u[0] : m1);  // X[N - k]
+        const float2 z2 = cconj(zm);
+        float2 x1 = cscale(cadd(z1, z2), 0.5f);
+        const float2 d = csub(z1, z2);
+        float2 x2 = make_float2(0.5f * d.y, -0.5f * d.x);
+        if (k & 1) {
+            x1 = cscale(x1, -1.f);
+            x2 = cscale(x2, -1.f);
+        }
+        typename S::T* o = out + (img * count + k) * (long)L + y;
+        o[0] = S::st(x1);
+        o[1] = S::st(x2);
+    }
+}
+
+// image_band16: image_rows2 for N = 16 Q point rows of the band path (U
+// transposed [img][kx][row], 2 band + 1 <= NZ Q).  The band row n = kx + band
+// (two rows' Hermitian extensions as re / im) is stage-1 input n = Q n1 + n2,
+// zero for n1 >= NZ; the shift is undone on the output by e^{-2 pi i band x / N}.
+template <int Q, int NZ, bool H16, bool O16, int NTH>
+__global__ __launch_bounds__(NTH) void image_band16_kernel(const void* __restrict__ U_,
+                                                           void* __restrict__ img, int nrows,
+                                                           int band) {
+    constexpr int L = 16 * Q, S1 = Q / 16, FPB = NTH / 16, LDQ = Q + 1;
+    using S = Store<H16>;
+    __shared__ float2 twG[L];
+    __shared__ float xplane[FPB * 16 * LDQ];
+    const int tid = threadIdx.x;
+    const int f = tid >> 4, t = tid & 15;
+    const int nb = band + 1;
+    const long npairs = nrows / 2;
+    const long p = (long)blockIdx.x * FPB + f;
+    const bool act = p < npairs;
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
+    float* xb = xplane + f * 16 * LDQ;
+    const typename S::T* U = (const typename S::T*)U_;
+    const long r0 = 2 * (act ? p : 0);  // an idle transform reads row pair 0
+    const long img0 = r0 / L, y0 = r0 - img0 * L;
+    const typename S::T* u1 = U + img0 * nb * (long)L + y0;
+    float2 v[S1][16];
+#pragma unroll
+    for (int s = 0; s < S1; s++)
+#pragma unroll
+        for (int n1 = 0; n1 < 16; n1++) {
+            float2 z = make_float2(0.f, 0.f);
+            const int n = Q * n1 + t + 16 * s;
+            const int kx = n - band;
+            if (n1 < NZ && kx >= -band && kx <= band) {
+                const int a = kx < 0 ? -kx : kx;
+                const float2 a1 = S::ld(u1[(long)a * L]), a2 = S::ld(u1[(long)a * L + 1]);
+                if (a == 0)
+                    z = make_float2(a1.x, a2.x);
+                else if (kx > 0)
+                    z = make_float2(a1.x - a2.y, a1.y + a2.x);  // u1 + i u2
+                else
+                    z = make_float2(a1.x + a2.y, a2.x - a1.y);  // conj(u1) + i conj(u2)
+            }
+            v[s][n1] = z;
+        }
+#pragma unroll
+    for (int s = 0; s < S1; s++) {
+        dft_reg<16, true>(v[s]);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; k1++) {
+            float2 w = twG[(t + 16 * s) * k1];
+            w.y = -w.y;
+            v[s][k1] = cmul_w<false>(v[s][k1], w);
+        }
+    }
+    float2 u[Q];
+    transpose16_planes<Q>(t, xb, v, u);
+    if (!act) return;
+    dft_reg<Q, true>(u);  // u[k2] = pixel x = t + 16 k2 of the two rows
+    // undo the index shift: multiply by e^{-2 pi i band x / L}, x = t + 16 k2;
+    // the table index band x mod L advances by 16 band mod L per k2 (no
+    // division per pixel when L is not a power of two)
+    {
+        const int step = (16 * band) % L;
+        int m = (band * t) % L;
+#pragma unroll
+        for (int k2 = 0; k2 < Q; k2++) {
+            u[k2] = cmul(u[k2], twG[m]);
+            m += step;
+            m = m >= L ? m - L : m;
+        }
+    }
+    // Each row passes through the transform's LDS plane so that a lane stores
+    // 4 consecutive pixels and one store covers whole lines (16-lane stores of
+    // 64 bytes measured 1.5x slower); streaming stores, the image is not read here.
+    const long o = (2 * p) * (long)L;
+#pragma unroll
+    for (int row = 0; row < 2; row++) {
+#pragma unroll
+        for (int k2 = 0; k2 < Q; k2++) xb[t + 16 * k2] = row ? u[k2].y : u[k2].x;
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < Q / 4; c++) {
+            const int x0 = 64 * c + 4 * t;
+            const float4 q = *(const float4*)(xb + x0);
+            if (O16) {
+                const u2v h = {__builtin_bit_cast(unsigned, __halves2half2(__float2half_rn(q.x),
+                                                                           __float2half_rn(q.y))),
+                               __builtin_bit_cast(unsigned, __halves2half2(__float2half_rn(q.z),
+                                                                           __float2half_rn(q.w)))};
+                __builtin_nontemporal_store(h, (u2v*)((unsigned short*)img + o + row * L + x0));
+            } else {
+                const f4v f = {q.x, q.y, q.z, q.w};
+                __builtin_nontemporal_store(f, (f4v*)((float*)img + o + row * L + x0));
+            }
+        }
+        wave_sync();
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Band pipeline (image path): square conv grid G = P*P of any size with
@@ -1400,11 +1619,6 @@ int band_grid(int Nx) {
 // inputs of the column pass; the next kernel's bank words are fetched while
 // the current kernel runs.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Stage 1 of a P x P two-stage FFT for the lane's n2 = t, inputs v[n1] in
 // registers (zeros past NZ fold away), twiddles twr[k1] = W_G^(t k1) forward.
@@ -1645,6 +1859,47 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float2* ksc
     return LITHO_EARG;
 }
 
+// 16 x Q band row kernels: every plan uses them (the same bits in every plan)
+template <int Q, int NTH>
+static int launch_rows_band16(const void* in, void* out, long nrows, int count, bool h16, bool u8,
+                              hipStream_t st) {
+    constexpr int FPB = NTH / 16;
+    const unsigned grid = cdiv((size_t)(nrows / 2), FPB);
+    if (h16 && u8)
+        hipLaunchKernelGGL((rows_band16_kernel<Q, true, true, NTH>), dim3(grid), dim3(NTH), 0, st,
+                           in, out, (int)nrows, count);
+    else if (h16)
+        hipLaunchKernelGGL((rows_band16_kernel<Q, true, false, NTH>), dim3(grid), dim3(NTH), 0, st,
+                           in, out, (int)nrows, count);
+    else if (u8)
+        hipLaunchKernelGGL((rows_band16_kernel<Q, false, true, NTH>), dim3(grid), dim3(NTH), 0, st,
+                           in, out, (int)nrows, count);
+    else
+        hipLaunchKernelGGL((rows_band16_kernel<Q, false, false, NTH>), dim3(grid), dim3(NTH), 0,
+                           st, in, out, (int)nrows, count);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+template <int Q, int NTH>
+static int launch_image_band16(const void* U, void* img, long nrows, int band, bool h16,
+                               bool out16, hipStream_t st) {
+    constexpr int FPB = NTH / 16;
+    const unsigned grid = cdiv((size_t)(nrows / 2), FPB);
+    if (h16 && out16)
+        hipLaunchKernelGGL((image_band16_kernel<Q, 3, true, true, NTH>), dim3(grid), dim3(NTH), 0,
+                           st, U, img, (int)nrows, band);
+    else if (h16)
+        hipLaunchKernelGGL((image_band16_kernel<Q, 3, true, false, NTH>), dim3(grid), dim3(NTH), 0,
+                           st, U, img, (int)nrows, band);
+    else if (out16)
+        hipLaunchKernelGGL((image_band16_kernel<Q, 3, false, true, NTH>), dim3(grid), dim3(NTH), 0,
+                           st, U, img, (int)nrows, band);
+    else
+        hipLaunchKernelGGL((image_band16_kernel<Q, 3, false, false, NTH>), dim3(grid), dim3(NTH),
+                           0, st, U, img, (int)nrows, band);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, int count, bool h16,
                       bool lean, bool u8, hipStream_t st) {
     if (nrows % L || nrows % 2 || count <= 0) return LITHO_EARG;
@@ -1679,6 +1934,11 @@ int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, in
             hipLaunchKernelGGL((rows_band1024_kernel<false, false, 512>), dim3(grid), dim3(512), 0,
                                st, in, out, (int)nrows, count);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    }
+    if (kmin == 0 && count <= 32) {
+        if (L == 256) return launch_rows_band16<16, 256>(in, out, nrows, count, h16, u8, st);
+        if (L == 512) return launch_rows_band16<32, 512>(in, out, nrows, count, h16, u8, st);
+        if (L == 768) return launch_rows_band16<48, 128>(in, out, nrows, count, h16, u8, st);
     }
     LITHO_RT(16, 16, 256)
     LITHO_RT(16, 32, 256)
@@ -1797,6 +2057,12 @@ int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, b
                                U, img, (int)nrows, band);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
+    if (N == 256 && 2 * band + 1 <= 3 * 16)
+        return launch_image_band16<16, 256>(U, img, nrows, band, h16, out16, st);
+    if (N == 512 && 2 * band + 1 <= 3 * 32)
+        return launch_image_band16<32, 256>(U, img, nrows, band, h16, out16, st);
+    if (N == 768 && 2 * band + 1 <= 3 * 48)
+        return launch_image_band16<48, 128>(U, img, nrows, band, h16, out16, st);
     LITHO_IT(16, 16, 3, 256)
     LITHO_IT(16, 16, 16, 256)
     LITHO_IT(16, 32, 3, 256)
