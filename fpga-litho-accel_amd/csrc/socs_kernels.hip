@@ -944,6 +944,123 @@ __global__ __launch_bounds__(NTH) void image_band16_kernel(const void* __restric
     }
 }
 
+// mask_cols16 / fi_cols16: mask_cols2 and fi_cols2 for N = 16 Q point columns
+// in 16-lane transforms (per-transform LDS planes, wavefront fences only); same
+// inputs, outputs and index conventions.  A transform reads and writes 16
+// complex values (128 bytes) of a column per access.
+template <int Q, bool H16, int NTH>
+__global__ __launch_bounds__(NTH) void mask_cols16_kernel(const void* __restrict__ R_,
+                                                          float2* __restrict__ M, int cx,
+                                                          int kymin, int cy, int mw) {
+    constexpr int N = 16 * Q, S1 = Q / 16, FPB = NTH / 16, LDQ = Q + 1;
+    __shared__ float2 twG[N];
+    __shared__ float xplane[FPB * 16 * LDQ];
+    const int tid = threadIdx.x;
+    const int f = tid >> 4, t = tid & 15;
+    const long b = blockIdx.y;
+    const int col = blockIdx.x * FPB + f;
+    const bool act = col < cx;
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
+    using S = Store<H16>;
+    // an idle transform reads column 0 and writes nothing
+    const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N + t;
+    float2 v[S1][16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++)
+#pragma unroll
+        for (int s = 0; s < S1; s++) v[s][n1] = S::ld(Rc[Q * n1 + 16 * s]);
+#pragma unroll
+    for (int s = 0; s < S1; s++) {
+        dft_reg<16, false>(v[s]);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; k1++) v[s][k1] = cmul_w<false>(v[s][k1], twG[(t + 16 * s) * k1]);
+    }
+    float2 u[Q];
+    transpose16_planes<Q>(t, xplane + f * 16 * LDQ, v, u);
+    if (!act) return;
+    dft_reg<Q, false>(u);  // u[k2] = X[t + 16 k2]
+    const float inv = 1.0f / ((float)N * (float)N);
+    float2* Mb = M + b * (long)cy * (mw ? mw : cx);
+    const int h = mw >> 1;
+    // |ky| < 32 (checked by the launcher): only k2 = 0, 1, Q-2, Q-1 hold kept
+    // rows, so the rest of the stage-2 DFT folds away
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int k2 = i < 2 ? i : Q - 4 + i;
+        const int k = t + 16 * k2;
+        const int ky = k < N / 2 ? k : k - N;
+        const int j = ky - kymin;
+        if (j >= 0 && j < cy) {
+            const float2 val = cscale(u[k2], (ky & 1) ? -inv : inv);
+            if (mw) {
+                // column kx = 0: ky >= 0 computed, ky < 0 written as the conjugate
+                if (col > 0 || ky >= 0) Mb[(long)j * mw + h + col] = val;
+                if (col > 0 || ky > 0) Mb[(long)(cy - 1 - j) * mw + h - col] = make_float2(val.x, -val.y);
+            } else {
+                Mb[(long)j * cx + col] = val;
+            }
+        }
+    }
+}
+
+template <int Q, int NZ, bool H16, int NTH>
+__global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict__ C,
+                                                        void* __restrict__ U_, int band) {
+    constexpr int N = 16 * Q, S1 = Q / 16, FPB = NTH / 16, LDQ = Q + 1;
+    __shared__ float2 twG[N];
+    __shared__ float xplane[FPB * 16 * LDQ];
+    const int tid = threadIdx.x;
+    const int f = tid >> 4, t = tid & 15;
+    const int nb = band + 1;
+    const long b = blockIdx.y;
+    const int col = blockIdx.x * FPB + f;
+    const bool act = col < nb;
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
+    // an idle transform reads column 0 and writes nothing
+    const float2* Cb = C + b * (long)(2 * band + 1) * nb + (act ? col : 0);
+    float2 v[S1][16];
+#pragma unroll
+    for (int s = 0; s < S1; s++)
+#pragma unroll
+        for (int n1 = 0; n1 < 16; n1++) {
+            const int n = Q * n1 + t + 16 * s;
+            float2 z = make_float2(0.f, 0.f);
+            if (n1 < NZ && n <= 2 * band) {
+                z = Cb[(long)n * nb];
+                if ((n - band) & 1) z = cscale(z, -1.f);
+            }
+            v[s][n1] = z;
+        }
+#pragma unroll
+    for (int s = 0; s < S1; s++) {
+        dft_reg<16, true>(v[s]);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; k1++) {
+            float2 w = twG[(t + 16 * s) * k1];
+            w.y = -w.y;
+            v[s][k1] = cmul_w<false>(v[s][k1], w);
+        }
+    }
+    float2 u[Q];
+    transpose16_planes<Q>(t, xplane + f * 16 * LDQ, v, u);
+    if (!act) return;
+    dft_reg<Q, true>(u);  // u[k2] = row y = t + 16 k2
+    using S = Store<H16>;
+    typename S::T* Uc = (typename S::T*)U_ + (b * nb + col) * (long)N;
+    const float sx = (col & 1) ? -1.f : 1.f;
+    // e^{-2 pi i band y / N}: the table index advances by 16 band mod N per k2
+    const int step = (16 * band) % N;
+    int m = (band * t) % N;
+#pragma unroll
+    for (int k2 = 0; k2 < Q; k2++) {
+        Uc[t + 16 * k2] = S::st(cscale(cmul(u[k2], twG[m]), sx));
+        m += step;
+        m = m >= N ? m - N : m;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Band pipeline (image path): square conv grid G = P*P of any size with
 // factors 2 and 3 (e.g. 36, 81, 144), whole image per block.
@@ -1174,8 +1291,11 @@ __global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __r
     const float inv = 1.0f / ((float)N * (float)N);
     float2* Mb = M + b * (long)cy * (mw ? mw : cx);
     const int h = mw >> 1;
+    // |ky| < 64 (checked by the launcher): only k2 = 0, 1, 30, 31 hold kept
+    // rows, so the rest of the stage-2 DFT folds away
 #pragma unroll
-    for (int k2 = 0; k2 < 32; k2++) {
+    for (int i = 0; i < 4; i++) {
+        const int k2 = i < 2 ? i : 28 + i;
         const int k = t + 32 * k2;
         const int ky = k < N / 2 ? k : k - N;
         const int j = ky - kymin;
@@ -1957,7 +2077,22 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
     if (B <= 0 || B > 65535) return LITHO_EARG;
     if (mw && (mw % 2 == 0 || cx != mw / 2 + 1 || kymin != -(mw / 2) || cy != mw))
         return LITHO_EARG;
-    if (N == 1024) {  // every plan: one kernel, the same bits
+    const bool win16 = kymin > -32 && kymin + cy - 1 < 32;  // |ky| < 32
+#define LITHO_MC16(q)                                                                        \
+    if (N == 16 * q && win16) {                                                              \
+        if (h16)                                                                             \
+            hipLaunchKernelGGL((mask_cols16_kernel<q, true, 128>), dim3(cdiv(cx, 8), B),     \
+                               dim3(128), 0, st, R, M, cx, kymin, cy, mw);                   \
+        else                                                                                 \
+            hipLaunchKernelGGL((mask_cols16_kernel<q, false, 128>), dim3(cdiv(cx, 8), B),    \
+                               dim3(128), 0, st, R, M, cx, kymin, cy, mw);                   \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                      \
+    }
+    LITHO_MC16(16)  // every plan: one kernel, the same bits
+    LITHO_MC16(32)
+    LITHO_MC16(48)
+#undef LITHO_MC16
+    if (N == 1024 && kymin > -64 && kymin + cy - 1 < 64) {  // every plan: one kernel, the same bits
         if (h16)
             hipLaunchKernelGGL((mask_cols_band1024_kernel<true>), dim3(cdiv(cx, 8), B), dim3(256), 0,
                                st, R, M, cx, kymin, cy, mw);
@@ -1991,6 +2126,21 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
                      hipStream_t st) {
     if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
+#define LITHO_FC16(q)                                                                          \
+    if (N == 16 * q && 2 * band + 1 <= 3 * q) {                                                \
+        const dim3 grid(cdiv(band + 1, 8), B);                                                 \
+        if (h16)                                                                               \
+            hipLaunchKernelGGL((fi_cols16_kernel<q, 3, true, 128>), grid, dim3(128), 0, st, C, \
+                               U, band);                                                       \
+        else                                                                                   \
+            hipLaunchKernelGGL((fi_cols16_kernel<q, 3, false, 128>), grid, dim3(128), 0, st, C, \
+                               U, band);                                                       \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
+    }
+    LITHO_FC16(16)  // every plan: one kernel, the same bits
+    LITHO_FC16(32)
+    LITHO_FC16(48)
+#undef LITHO_FC16
     if (N == 1024 && 2 * band + 1 <= 160) {  // every plan: one kernel, the same bits
         const dim3 grid(cdiv(band + 1, 8), B);
         if (h16)
