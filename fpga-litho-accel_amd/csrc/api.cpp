@@ -999,7 +999,9 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     if (c->timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
     LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
     if (c->timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
-    LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->full.d_bank, bank->full.d_scales, bank->full.n, Nx, B,
+    // a real mask's window spectrum, conjugate-symmetric to FP32 round-off: the
+    // paired set (cross-talk of that order, far below the FP32 image error)
+    LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->band.d_bank, bank->band.d_scales, bank->band.n, Nx, B,
                                 (float*)T, 0, c->tw, st));
     if (c->timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));

@@ -156,7 +156,7 @@ int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double
  * point-symmetric source gives such kernels) has a real field on a real
  * mask, so two of them share one transform as sqrt(la) Ka + i sqrt(lb) Kb;
  * the other kernels keep their own entries.  litho_socs_tmpimg (a given
- * spectrum) and the nextPow2 path use the set as given. */
+ * spectrum) uses the set as given. */
 int litho_bank_create(litho_ctx* ctx, const float* krn_r, const float* krn_i,
                       const float* scales, int nk, int Nx, int Ny, litho_bank** bank);
 int litho_bank_destroy(litho_bank* bank);
