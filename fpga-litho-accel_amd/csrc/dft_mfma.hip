@@ -25,6 +25,7 @@
 // the D fragment holds rows 4*(l/16) + i, column l%16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -211,14 +212,20 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
 }
 
 // ---------------------------------------------------------------------------
-// U [B][nb][N] (float2, FP32) -> image [B][N][N] (f32 or fp16).  NKS k-steps of
-// 32 cover 8 kx values each: (re, im) x (hi, lo) for two kx per lane group.
+// U [B][nb][N] (float2, FP32) -> image [B][N][N] (f32 or fp16).  The first KS
+// k-steps of 32 cover 8 kx values each, (re, im) x (hi, lo) for two kx per
+// lane group (kx < 8 KS, at least kx <= nb / 2); the other NKS - KS steps cover
+// 16 kx each as (re, im) hi parts only.  The intensity spectrum above half the
+// band is small (it comes only from products of band-edge kernel components):
+// rounding it to fp16 alone adds below 1e-7 of the image at 1024^2 (measured on
+// the FP64 images of Manhattan and random grey-level masks), where rounding all
+// of U would add 3-5e-4.
 // The product is formed transposed (table as the A operand, U as B): a lane's
 // D fragment is 4 consecutive pixels of one image row.  Fragments are staged
 // per wave in LDS until every row has a full 128-byte segment, which is then
 // stored 16 bytes per lane (8 rows per instruction), so HBM sees whole lines.
 // ---------------------------------------------------------------------------
-template <int NKS, bool OUT16>
+template <int NKS, int KS, bool OUT16>
 __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* __restrict__ U,
                                                                  const u4v* __restrict__ Hpk,
                                                                  void* __restrict__ img_, int N,
@@ -248,17 +255,27 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
 #pragma unroll
         for (int s = 0; s < NKS; s++) {
             unsigned q[4];
+            if (s < KS) {
 #pragma unroll
-            for (int e = 0; e < 2; e++) {
-                const int kx = 8 * s + 2 * g + e;
-                // clamped index and a select, not a branch: every load is in
-                // flight at once
-                float2 v = Ub[(long)min(kx, nb - 1) * N + y];
-                if (kx >= nb) v = make_float2(0.f, 0.f);
-                const _Float16 hr = (_Float16)v.x, hi = (_Float16)v.y;
-                const float lr = (v.x - (float)hr) * 4096.f, li = (v.y - (float)hi) * 4096.f;
-                q[2 * e] = __builtin_bit_cast(unsigned, (h2v){hr, hi});
-                q[2 * e + 1] = f2_to_h2(lr, li);
+                for (int e = 0; e < 2; e++) {
+                    const int kx = 8 * s + 2 * g + e;
+                    // clamped index and a select, not a branch: every load is in
+                    // flight at once
+                    float2 v = Ub[(long)min(kx, nb - 1) * N + y];
+                    if (kx >= nb) v = make_float2(0.f, 0.f);
+                    const _Float16 hr = (_Float16)v.x, hi = (_Float16)v.y;
+                    const float lr = (v.x - (float)hr) * 4096.f, li = (v.y - (float)hi) * 4096.f;
+                    q[2 * e] = __builtin_bit_cast(unsigned, (h2v){hr, hi});
+                    q[2 * e + 1] = f2_to_h2(lr, li);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int kx = 8 * KS + 16 * (s - KS) + 4 * g + e;
+                    float2 v = Ub[(long)min(kx, nb - 1) * N + y];
+                    if (kx >= nb) v = make_float2(0.f, 0.f);
+                    q[e] = f2_to_h2(v.x, v.y);
+                }
             }
             a[rt][s] = (u4v){q[0], q[1], q[2], q[3]};
         }
@@ -361,8 +378,16 @@ std::vector<unsigned short> mask_rows_table(int N, int kmin, int count) {
     return t;
 }
 
+static void image_rows_steps(int nb, int* nks, int* ks) {
+    const int split = std::min(nb, nb / 2 + 1);  // kx <= nb / 2 keep their lo parts
+    *ks = (split + 7) / 8;
+    const int rest = std::max(0, nb - 8 * *ks);
+    *nks = *ks + (rest + 15) / 16;
+}
+
 std::vector<unsigned short> image_rows_table(int N, int nb) {
-    const int NKS = (nb + 7) / 8;
+    int NKS, KS;
+    image_rows_steps(nb, &NKS, &KS);
     const int nch = N / XCH;
     std::vector<unsigned short> t((size_t)nch * 4 * NKS * 64 * 8);
     for (int c = 0; c < nch; c++)
@@ -371,8 +396,11 @@ std::vector<unsigned short> image_rows_table(int N, int nb) {
                 for (int l = 0; l < 64; l++)
                     for (int i = 0; i < 8; i++) {
                         const int x = c * XCH + 16 * tt + (l & 15);
-                        const int kx = 8 * s + 2 * (l >> 4) + (i >> 2);
-                        const int part = i & 3;
+                        // split steps: (re, im) x (hi, lo) of two kx per lane
+                        // group; hi steps: (re, im) hi of four kx
+                        const int kx = s < KS ? 8 * s + 2 * (l >> 4) + (i >> 2)
+                                              : 8 * KS + 16 * (s - KS) + 4 * (l >> 4) + (i >> 1);
+                        const int part = s < KS ? (i & 3) : (i & 1);
                         double v = 0.0;
                         if (kx < nb) {
                             const double wk = kx ? 2.0 : 1.0;
@@ -425,28 +453,30 @@ int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, vo
                            bool out16, hipStream_t st) {
     if (N % ROWS_PER_BLOCK || N % XCH || B <= 0 || B > 65535 || nb <= 0 || nb > 72)
         return LITHO_EARG;
-    const int NKS = (nb + 7) / 8;
+    int NKS, KS;
+    image_rows_steps(nb, &NKS, &KS);
     const dim3 grid(N / ROWS_PER_BLOCK, B);
     const u4v* H = (const u4v*)table;
-#define LITHO_IR(nks)                                                                        \
-    if (NKS == nks) {                                                                        \
-        if (out16)                                                                           \
-            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, true>), grid, dim3(256), 0, st, U, \
-                               H, img, N, nb);                                               \
-        else                                                                                 \
-            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, false>), grid, dim3(256), 0, st,   \
-                               U, H, img, N, nb);                                            \
-        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                      \
+#define LITHO_IR(nks, ks)                                                                     \
+    if (NKS == nks && KS == ks) {                                                             \
+        if (out16)                                                                            \
+            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, true>), grid, dim3(256), 0, st, \
+                               U, H, img, N, nb);                                             \
+        else                                                                                  \
+            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, false>), grid, dim3(256), 0, st, \
+                               U, H, img, N, nb);                                             \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                       \
     }
-    LITHO_IR(1)
-    LITHO_IR(2)
-    LITHO_IR(3)
-    LITHO_IR(4)
-    LITHO_IR(5)
-    LITHO_IR(6)
-    LITHO_IR(7)
-    LITHO_IR(8)
-    LITHO_IR(9)
+    // every (steps, split steps) pair of nb <= 72
+    LITHO_IR(1, 1)
+    LITHO_IR(2, 1)
+    LITHO_IR(2, 2)
+    LITHO_IR(3, 2)
+    LITHO_IR(4, 3)
+    LITHO_IR(5, 3)
+    LITHO_IR(5, 4)
+    LITHO_IR(6, 4)
+    LITHO_IR(7, 5)
 #undef LITHO_IR
     return LITHO_EARG;
 }
