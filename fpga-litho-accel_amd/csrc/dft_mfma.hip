@@ -442,7 +442,7 @@ int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, v
     }
     LITHO_MR(1, 4)
     LITHO_MR(2, 4)
-    LITHO_MR(3, 4)
+    LITHO_MR(3, 2)  // 32 rows per wave: 114 VGPRs, 16 waves per CU (7 % faster than 64 rows)
     LITHO_MR(4, 2)
     LITHO_MR(5, 2)
 #undef LITHO_MR
