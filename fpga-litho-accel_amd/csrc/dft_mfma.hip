@@ -14,10 +14,13 @@
 // Numerics.  The mask is exact in fp16 (u8 levels 0..255, or 0/1 floats); the
 // mask-side table is one fp16 rounding of cos/sin.  On the image side U is
 // split on the fly into hi + lo fp16 parts (lo scaled by 2^12 so it stays in
-// fp16's normal range); the table rows for the lo parts carry the 2^-12.
+// fp16's normal range) for kx up to half the band, hi alone above; the table
+// rows for the lo parts carry the 2^-12.
 //
-// Tiling.  A block is 4 waves; each wave owns 64 rows (4 row tiles of 16) so
-// that one 16x32 B fragment read from LDS feeds four MFMAs.  The B operand
+// Tiling.  A wave owns RT row tiles of 16 rows, so that one 16x32 B fragment
+// read from LDS feeds RT MFMAs: on the mask side a 256-row block is 4 waves of
+// 4 tiles for windows of up to 32 columns and 8 waves of 2 tiles for wider
+// ones; on the image side a 128-row block is 4 waves of 2 tiles.  The B operand
 // (the cos/sin table) is pre-arranged on the host in fragment order: lane l
 // reads 16 contiguous bytes, so LDS reads are conflict-free and the global ->
 // LDS copy is a straight uint4 copy.  Within one k-step, lane l of the MFMA
@@ -65,7 +68,7 @@ __device__ __forceinline__ unsigned f2_to_h2(float a, float b) {
     return __builtin_bit_cast(unsigned, v);
 }
 
-constexpr int ROWS_PER_BLOCK = 256;  // 4 waves x 64 rows
+constexpr int ROWS_PER_BLOCK = 256;  // mask side: rows per block
 constexpr int KCH = 128;             // mask-side K (pixel) chunk staged in LDS
 constexpr int XCH = 64;              // image-side output columns per LDS chunk
 // image side: row tiles per wave (32 rows: 135 VGPRs, 46 KB of LDS, three waves
@@ -445,7 +448,7 @@ int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, v
     }
     LITHO_MR(1, 4)
     LITHO_MR(2, 4)
-    LITHO_MR(3, 2)  // 32 rows per wave: 114 VGPRs, 16 waves per CU (7 % faster than 64 rows)
+    LITHO_MR(3, 2)  // 32 rows per wave: 114 VGPRs, two 8-wave blocks per CU (7 % faster than 64 rows)
     LITHO_MR(4, 2)
     LITHO_MR(5, 2)
 #undef LITHO_MR
