@@ -19,7 +19,7 @@
 //
 // Tiling.  A wave owns RT row tiles of 16 rows, so that one 16x32 B fragment
 // read from LDS feeds RT MFMAs: on the mask side a 256-row block is 4 waves of
-// 4 tiles for windows of up to 32 columns and 8 waves of 2 tiles for wider
+// 4 tiles for windows of up to 16 columns and 8 waves of 2 tiles for wider
 // ones; on the image side a 128-row block is 4 waves of 2 tiles.  The B operand
 // (the cos/sin table) is pre-arranged on the host in fragment order: lane l
 // reads 16 contiguous bytes, so LDS reads are conflict-free and the global ->
@@ -447,8 +447,10 @@ int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, v
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     LITHO_MR(1, 4)
-    LITHO_MR(2, 4)
-    LITHO_MR(3, 2)  // 32 rows per wave: 114 VGPRs, two 8-wave blocks per CU (7 % faster than 64 rows)
+    // 32 rows per wave from two frequency groups on (two 8-wave blocks per CU):
+    // 7 % faster than 64 rows at 768 and 1024 points, the same at 512
+    LITHO_MR(2, 2)
+    LITHO_MR(3, 2)
     LITHO_MR(4, 2)
     LITHO_MR(5, 2)
 #undef LITHO_MR
