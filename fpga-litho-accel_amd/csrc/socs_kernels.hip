@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict
     fill_twiddles<N>(twG, tid, NTH);
     __syncthreads();
     // an idle transform reads column 0 and writes nothing
-    const float2* Cb = C + b * (long)(2 * band + 1) * nb + (act ? col : 0);
+    const float2* Cb = C + (b * (long)nb + (act ? col : 0)) * (2 * band + 1);
     float2 v[S1][16];
 #pragma unroll
     for (int s = 0; s < S1; s++)
@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict
             const int n = Q * n1 + t + 16 * s;
             float2 z = make_float2(0.f, 0.f);
             if (n1 < NZ && n <= 2 * band) {
-                z = Cb[(long)n * nb];
+                z = Cb[n];
                 if ((n - band) & 1) z = cscale(z, -1.f);
             }
             v[s][n1] = z;
@@ -1160,7 +1160,7 @@ __global__ __launch_bounds__(P * SLOTS) void socs_band_kernel(const float2* __re
         __syncthreads();
         fft2_stage2<Geo, false>(t, xb, [&](int, int kk, float2 val) {
             const int ky = kk <= G / 2 ? kk : kk - G;
-            if (act && ky >= -band && ky <= band) Cb[(ky + band) * W + col] = cscale(val, inv);
+            if (act && ky >= -band && ky <= band) Cb[(long)col * (2 * band + 1) + ky + band] = cscale(val, inv);
         });
         __syncthreads();
     }
@@ -1238,11 +1238,11 @@ __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict_
     __syncthreads();
     float2* xb = xbuf + f * Geo::XBUF;
     // an idle transform reads column 0 and writes nothing
-    const float2* Cb = C + b * (long)(2 * band + 1) * nb + (act ? col : 0);
+    const float2* Cb = C + (b * (long)nb + (act ? col : 0)) * (2 * band + 1);
     fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
         float2 v = make_float2(0.f, 0.f);
         if (n <= 2 * band) {
-            v = Cb[(long)n * nb];
+            v = Cb[n];
             if ((n - band) & 1) v = cscale(v, -1.f);
         }
         return v;
@@ -1329,14 +1329,14 @@ __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __r
     fill_twiddles<N>(twG, tid, NTH);
     __syncthreads();
     // an idle transform reads column 0 and writes nothing
-    const float2* Cb = C + b * (long)(2 * band + 1) * nb + (act ? col : 0);
+    const float2* Cb = C + (b * (long)nb + (act ? col : 0)) * (2 * band + 1);
     float2 v[32];
 #pragma unroll
     for (int n1 = 0; n1 < 32; n1++) {
         const int n = 32 * n1 + t;
         float2 z = make_float2(0.f, 0.f);
         if (n1 < NZ && n <= 2 * band) {
-            z = Cb[(long)n * nb];
+            z = Cb[n];
             if ((n - band) & 1) z = cscale(z, -1.f);
         }
         v[n1] = z;
@@ -1903,7 +1903,7 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
         if (act)
             fft2_stage2<Geo, false>(t, xb, [&](int, int kk, float2 val) {
                 const int ky = kk <= G / 2 ? kk : kk - G;
-                if (ky >= -band && ky <= band) Cb[(ky + band) * W + col] = cscale(val, inv);
+                if (ky >= -band && ky <= band) Cb[(long)col * (2 * band + 1) + ky + band] = cscale(val, inv);
             });
         wave_sync();
     }

@@ -39,7 +39,8 @@ int launch_rows_dft_t(int L, const void* in, void* out, long nrows, int kmin, in
 // through M(-ky, -kx) = conj(M(ky, kx))
 int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin, int cy, bool h16,
                        bool lean, hipStream_t st, int mw = 0);
-// M x bank -> band spectrum C [B][4Nx+1][2Nx+1] of the intensity, /G^2.
+// M x bank -> band spectrum C [B][2Nx+1][4Nx+1] of the intensity (kx-major: column kx
+// holds ky = -2Nx..2Nx contiguously), /G^2.
 // bank_fmt 0: float2; 1: bf16 (re, im) pairs; 2: fp16 pairs; kscale[k] scales the real (x)
 // and imaginary (y) parts of entry k.
 // shape 0: 48-slot blocks, 1: 24-slot (lean) blocks at G = 144, 2: wave-synced

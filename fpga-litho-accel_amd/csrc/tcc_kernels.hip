@@ -210,7 +210,7 @@ int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n
 
 // ---------------------------------------------------------------------------
 // TCC-direct image spectrum (calcImage, litho_thread.cpp L539-583), FP64:
-//   C[ny2 + 2Ny][nx2] = sum_{n1} M(n2+n1) conj(M(n1)) TCC[n1][n2+n1]
+//   C[nx2][ny2 + 2Ny] = sum_{n1} M(n2+n1) conj(M(n1)) TCC[n1][n2+n1]
 // for ny2 in [-2Ny, 2Ny], nx2 in [0, 2Nx], n1 and n2+n1 inside the window.
 // M is the (2Ny+1) x (2Nx+1) centred mask spectrum window (float2); one
 // block per output, the n1 sum reduced across the block.
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void tcc_image_spectrum_kernel(const float2* _
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) C[(long)blockIdx.y * W + nx2] = make_float2((float)sr[0], (float)si[0]);
+    if (threadIdx.x == 0) C[(long)nx2 * (4 * Ny + 1) + blockIdx.y] = make_float2((float)sr[0], (float)si[0]);
 }
 
 int launch_tcc_image_spectrum(const float2* M, const double2* T, int Nx, int Ny, float2* C,

@@ -29,7 +29,7 @@ int herm_gemm_bf16s_slices(int n, int m);
 int launch_herm_gemm_bf16s(const double2* X, const double* sx, const double2* Y, int n, int m,
                            int has_imag, float2* part, int nks, double2* C, hipStream_t st);
 
-// TCC-direct image band spectrum C [(4Ny+1)][(2Nx+1)] from the mask window M
+// TCC-direct image band spectrum C [(2Nx+1)][(4Ny+1)] (kx-major) from the mask window M
 // [(2Ny+1)][(2Nx+1)] and the n x n TCC (calcImage)
 int launch_tcc_image_spectrum(const float2* M, const double2* T, int Nx, int Ny, float2* C,
                               hipStream_t st);
