@@ -338,7 +338,7 @@ def main():
     bf16_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
     chosen = timed_form(L.LITHO_PREC_COMPACT, d_mask8, bank16)
     log(f"graph FP32 {pipe_fp32:.1f} images/s; fp16 storage {mixed:.1f}; bf16 bank {bf16_bank:.1f}; "
-        f"chosen form (split-BF16 kernels, u8 masks, matrix-core DFTs, fp16 bank and images) "
+        f"chosen form (split-BF16 kernels, u8 masks, matrix-core DFTs, fp16 window, bank and images) "
         f"{chosen:.1f} images/s = {chosen / images_per_s:.2f}x plain")
     chosen_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
 
@@ -411,8 +411,9 @@ def main():
             "form": "kernels from the split-BF16 Gram contraction (FP32 accumulate) and FP64 "
                     "eigensolve; batched mixed precision: masks as 8-bit levels, mask-row and "
                     "image-row band DFTs as fp16 matrix products with FP32 accumulate (image side "
-                    "split hi+lo), fp16 kernel bank with per-kernel FP32 scale, FP32 column FFTs "
-                    "and SOCS, fp16 images; the step captured as one hipGraph",
+                    "split hi+lo below half the band), fp16 row-DFT window, fp16 kernel bank with "
+                    "FP32 scales, FP32 column FFTs and SOCS, fp16 images; the step captured as "
+                    "one hipGraph",
             "precision_flags": L.LITHO_PREC_COMPACT,
             "value": round(chosen, 2), "unit": "images/s",
             "ratio_vs_plain": round(chosen / images_per_s, 3),

@@ -51,8 +51,8 @@ LITHO_PREC_MASK_U8 = 8
 LITHO_PREC_IMAGE_FP16 = 16
 LITHO_PREC_DFT_MFMA = 32
 LITHO_PREC_MIXED = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16
-LITHO_PREC_COMPACT = (LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8 | LITHO_PREC_DFT_MFMA
-                      | LITHO_PREC_IMAGE_FP16)
+LITHO_PREC_COMPACT = (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8
+                      | LITHO_PREC_DFT_MFMA | LITHO_PREC_IMAGE_FP16)
 LITHO_PATH_AUTO = 0
 LITHO_PATH_POW2 = 1
 

@@ -193,9 +193,10 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
  * graph (per shape and buffers) and replayed.
  * `precision` is a set of flags:
  *  - INTER_FP16 stores the row-DFT window intermediate as fp16 (and the FI
- *    column output too, unless DFT_MFMA is set);
+ *    column output too, unless DFT_MFMA is set: the image-side product splits
+ *    that FP32 output itself);
  *  - BANK_BF16 / BANK_FP16 read the kernel bank as 16-bit pairs with one FP32
- *    scale per kernel;
+ *    scale per entry for the real and one for the imaginary parts;
  *  - DFT_MFMA computes the mask-row and image-row band DFTs as fp16 matrix
  *    products with FP32 accumulation on the matrix cores (the image side
  *    splits its input into hi + lo fp16 parts); the other stages stay FP32;
@@ -203,8 +204,8 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
  *    (the PNG -> BIN convention); IMAGE_FP16 (requires DFT_MFMA): d_image
  *    receives fp16 values.
  * LITHO_PREC_MIXED is the storage-only set that meets 1e-3 max relative error
- * against FP64; LITHO_PREC_COMPACT the matrix-core set with 8-bit masks and
- * fp16 images (about 6e-4 at 1024^2, K=24).
+ * against FP64; LITHO_PREC_COMPACT the matrix-core set with 8-bit masks, an
+ * fp16 row-DFT window and fp16 images (6-7.5e-4 at 1024^2, K=24).
  * Uses the band conv grid (N in {256, 512, 1024}, 4Nx+1 <= 144). */
 #define LITHO_PREC_FP32 0
 #define LITHO_PREC_INTER_FP16 1
@@ -214,8 +215,9 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
 #define LITHO_PREC_IMAGE_FP16 16
 #define LITHO_PREC_DFT_MFMA 32
 #define LITHO_PREC_MIXED (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16)
-#define LITHO_PREC_COMPACT \
-    (LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8 | LITHO_PREC_DFT_MFMA | LITHO_PREC_IMAGE_FP16)
+#define LITHO_PREC_COMPACT                                                              \
+    (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8 | LITHO_PREC_DFT_MFMA | \
+     LITHO_PREC_IMAGE_FP16)
 int litho_socs_image_batch_pipelined(litho_ctx* ctx, const litho_bank* bank, const void* d_mask,
                                      int B, int N, void* d_image, int precision, int chunks,
                                      int use_graph);

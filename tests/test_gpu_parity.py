@@ -318,7 +318,8 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
 
 @pytest.mark.parametrize("N,prec", [(1024, 32), (1024, 8 | 32 | 1 | 4), (1024, 8 | 32 | 4 | 16),
                                     (512, 8 | 32 | 1 | 4), (256, 8 | 32), (1024, 8 | 4 | 16),
-                                    (768, 8 | 16), (512, 8 | 1 | 4 | 16)])
+                                    (768, 8 | 16), (512, 8 | 1 | 4 | 16),
+                                    (1024, 8 | 32 | 1 | 4 | 16), (512, 8 | 32 | 1 | 4 | 16)])
 def test_matrix_core_forms(gpu, orc, base_source, N, prec):
     """u8 / f32 masks and f32 / fp16 images, with the band DFTs on the fp16
     matrix cores (flag 32) or as FP32 FFTs, stay within 1e-3 of FP64 on binary
