@@ -20,7 +20,7 @@
 // Tiling.  A wave owns RT row tiles of 16 rows, so that one 16x32 B fragment
 // read from LDS feeds RT MFMAs: on the mask side a 256-row block is 4 waves of
 // 4 tiles for windows of up to 16 columns and 8 waves of 2 tiles for wider
-// ones; on the image side a 128-row block is 4 waves of 2 tiles.  The B operand
+// ones; on the image side a 256-row block is 8 waves of 2 tiles.  The B operand
 // (the cos/sin table) is pre-arranged on the host in fragment order: lane l
 // reads 16 contiguous bytes, so LDS reads are conflict-free and the global ->
 // LDS copy is a straight uint4 copy.  Within one k-step, lane l of the MFMA
@@ -71,9 +71,14 @@ __device__ __forceinline__ unsigned f2_to_h2(float a, float b) {
 constexpr int ROWS_PER_BLOCK = 256;  // mask side: rows per block
 constexpr int KCH = 128;             // mask-side K (pixel) chunk staged in LDS
 constexpr int XCH = 64;              // image-side output columns per LDS chunk
-// image side: row tiles per wave (32 rows: 135 VGPRs, 46 KB of LDS, three waves
-// per SIMD; 64 rows measured 2 % slower with fp16 images, 13 % with f32)
+// image side: row tiles per wave (32 rows; 64 rows measured 2 % slower with
+// fp16 images, 13 % with f32)
 constexpr int IMAGE_RT = 2;
+// image side: threads per block.  8 waves share one table chunk, so each lane
+// prefetches 4 table fragments instead of 7: 126 VGPRs and 64 KB of LDS, two
+// blocks and four waves per SIMD (4-wave blocks: 135 VGPRs, three waves per
+// SIMD, 3 % slower per step)
+constexpr int IMAGE_NTH = 512;
 
 // ---------------------------------------------------------------------------
 // Mask rows -> transposed row-DFT window R [B][count][N] (same contract as the
@@ -230,29 +235,33 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
 // per wave in LDS until every row has a full 128-byte segment, which is then
 // stored 16 bytes per lane (8 rows per instruction), so HBM sees whole lines.
 // ---------------------------------------------------------------------------
-template <int NKS, int KS, bool OUT16, int RT>
-__global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* __restrict__ U,
+template <int NKS, int KS, bool OUT16, int RT, int NTH>
+__global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const float2* __restrict__ U,
                                                                  const u4v* __restrict__ Hpk,
                                                                  void* __restrict__ img_, int N,
                                                                  int nb) {
+    constexpr int NW = NTH / 64;      // waves per block, sharing one table chunk
     constexpr int CH = 4 * NKS * 64;  // u4v per XCH-column chunk (4 column tiles)
-    constexpr int PER = CH / 256;
-    static_assert(CH % 256 == 0, "chunk copy");
+    constexpr int PER = (CH + NTH - 1) / NTH;
+    // CH is a multiple of 256, so the last copy round is whole waves
+    static_assert(CH % 256 == 0 && NTH % 256 == 0, "chunk copy");
+    auto copy_lane = [](int tid, int i) { return CH % NTH == 0 || tid + NTH * i < CH; };
     constexpr int OB = OUT16 ? 2 : 4;         // bytes per pixel
     constexpr int TPS = OUT16 ? 4 : 2;        // column tiles per 128-byte row segment
     constexpr int SROW = 16 * TPS * OB + 16;  // staged row pitch in bytes (padded)
     __shared__ u4v fs[CH];
     constexpr int RW = 16 * RT;  // rows per wave
-    __shared__ __attribute__((aligned(16))) unsigned char stg[4][RW * SROW];
+    __shared__ __attribute__((aligned(16))) unsigned char stg[NW][RW * SROW];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int b = blockIdx.y;
-    const int ybase = blockIdx.x * 4 * RW + w * RW;
+    const int ybase = blockIdx.x * NW * RW + w * RW;
     const float2* Ub = U + (long)b * nb * N;
     u4v pre[PER];
 #pragma unroll
-    for (int i = 0; i < PER; i++) pre[i] = Hpk[tid + 256 * i];
+    for (int i = 0; i < PER; i++)
+        if (copy_lane(tid, i)) pre[i] = Hpk[tid + NTH * i];
     // U fragments for the whole K extent stay in registers
     u4v a[RT][NKS];
 #pragma unroll
@@ -287,7 +296,8 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
         }
     }
 #pragma unroll
-    for (int i = 0; i < PER; i++) fs[tid + 256 * i] = pre[i];
+    for (int i = 0; i < PER; i++)
+        if (copy_lane(tid, i)) fs[tid + NTH * i] = pre[i];
     __syncthreads();
     unsigned char* sw = stg[w];
     unsigned char* out = (unsigned char*)img_ + ((long)b * N * N + (long)ybase * N) * OB;
@@ -296,7 +306,8 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
         const bool more = c + 1 < nch;
         if (more) {
 #pragma unroll
-            for (int i = 0; i < PER; i++) pre[i] = Hpk[(long)(c + 1) * CH + tid + 256 * i];
+            for (int i = 0; i < PER; i++)
+                if (copy_lane(tid, i)) pre[i] = Hpk[(long)(c + 1) * CH + tid + NTH * i];
         }
         u4v tnext = fs[lane];
 #pragma unroll
@@ -347,7 +358,8 @@ __global__ __launch_bounds__(256, 2) void image_rows_mfma_kernel(const float2* _
         __syncthreads();  // every wave is done with this table chunk
         if (more) {
 #pragma unroll
-            for (int i = 0; i < PER; i++) fs[tid + 256 * i] = pre[i];
+            for (int i = 0; i < PER; i++)
+                if (copy_lane(tid, i)) fs[tid + NTH * i] = pre[i];
         }
         __syncthreads();
     }
@@ -464,15 +476,17 @@ int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, vo
     int NKS, KS;
     image_rows_steps(nb, &NKS, &KS);
     constexpr int IRT = IMAGE_RT;  // row tiles per wave
-    const dim3 grid(N / (64 * IRT), B);
+    constexpr int INTH = IMAGE_NTH;
+    static_assert(ROWS_PER_BLOCK % (INTH / 4 * IRT) == 0, "image rows per block divide N");
+    const dim3 grid(N / (INTH / 4 * IRT), B);
     const u4v* H = (const u4v*)table;
 #define LITHO_IR(nks, ks)                                                                      \
     if (NKS == nks && KS == ks) {                                                              \
         if (out16)                                                                             \
-            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, true, IRT>), grid, dim3(256), 0, \
+            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, true, IRT, INTH>), grid, dim3(INTH), 0, \
                                st, U, H, img, N, nb);                                          \
         else                                                                                   \
-            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, false, IRT>), grid, dim3(256), 0, \
+            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, false, IRT, INTH>), grid, dim3(INTH), 0, \
                                st, U, H, img, N, nb);                                          \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
