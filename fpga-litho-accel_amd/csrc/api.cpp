@@ -703,6 +703,7 @@ int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx
     Dev H;
     LITHO_TRY(H.alloc(nn * sizeof(double2)));
     LITHO_HIP(hipMemcpyAsync(H.p, h.data(), nn * sizeof(double2), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipStreamSynchronize(st));  // h is released on every return path below
     return kernels_from_device_tcc(c, H.as<double2>(), n, Nx, Ny, nk, scales, krn_r, krn_i, flags);
 }
 
@@ -810,6 +811,8 @@ static int upload_set(litho_ctx* c, const std::vector<float2>& h, const std::vec
     LITHO_HIP(hipMemcpyAsync(s->d_bankh, hh.data(), hh.size() * sizeof(__half2),
                              hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(s->d_kscale, ks.data(), n * sizeof(float2), hipMemcpyHostToDevice, st));
+    // the host copies are locals of this call: the copies finish before they go
+    LITHO_HIP(hipStreamSynchronize(st));
     return LITHO_OK;
 }
 
