@@ -582,10 +582,11 @@ __global__ __launch_bounds__(NTH) void rows_band1024_kernel(const void* __restri
     const long npairs = nrows / 2;
     const long p = (long)blockIdx.x * FPB + f;
     const bool act = p < npairs;
-    fill_twiddles<L>(twG, tid, NTH);
-    __syncthreads();
+    // the row loads go out first: the twiddle table is computed while they fly
     float2 v[32];
     rows_band1024_load<IN8>(in_, act ? p : 0, t, v);
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
     rows_band1024_fft<H16>(tid, t, twG, xplane + f * Geo::XBUF, v, act, p, out_, count);
 }
 
@@ -676,8 +677,6 @@ __global__ __launch_bounds__(NTH) void image_band1024_kernel(const void* __restr
     const long npairs = nrows / 2;
     const long p = (long)blockIdx.x * FPB + f;
     const bool act = p < npairs;
-    fill_twiddles<L>(twG, tid, NTH);
-    __syncthreads();
     float* xb = xplane + f * Geo::XBUF;
     const typename S::T* U = (const typename S::T*)U_;
     const long r0 = 2 * (act ? p : 0);
@@ -703,6 +702,9 @@ __global__ __launch_bounds__(NTH) void image_band1024_kernel(const void* __restr
         }
         v[n1] = z;
     }
+    // the loads above go out before the twiddle table is computed
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
     dft_reg<32, true>(v);
 #pragma unroll
     for (int k1 = 1; k1 < 32; k1++) {
@@ -786,8 +788,6 @@ __global__ __launch_bounds__(NTH) void rows_band16_kernel(const void* __restrict
     const long npairs = nrows / 2;
     const long p = (long)blockIdx.x * FPB + f;
     const bool act = p < npairs;
-    fill_twiddles<L>(twG, tid, NTH);
-    __syncthreads();
     float* xb = xplane + f * 16 * LDQ;
     // an idle transform reads pair 0 (in bounds) and writes nothing
     const long roff = 2 * (act ? p : 0) * (long)L;
@@ -807,6 +807,9 @@ __global__ __launch_bounds__(NTH) void rows_band16_kernel(const void* __restrict
             else
                 v[s][n1] = make_float2(r0[n], r0[L + n]);
         }
+    // the loads above go out before the twiddle table is computed
+    fill_twiddles<L>(twG, tid, NTH);
+    __syncthreads();
 #pragma unroll
     for (int s = 0; s < S1; s++) {
         dft_reg<16, false>(v[s]);
@@ -960,8 +963,6 @@ __global__ __launch_bounds__(NTH) void mask_cols16_kernel(const void* __restrict
     const long b = blockIdx.y;
     const int col = blockIdx.x * FPB + f;
     const bool act = col < cx;
-    fill_twiddles<N>(twG, tid, NTH);
-    __syncthreads();
     using S = Store<H16>;
     // an idle transform reads column 0 and writes nothing
     const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N + t;
@@ -970,6 +971,9 @@ __global__ __launch_bounds__(NTH) void mask_cols16_kernel(const void* __restrict
     for (int n1 = 0; n1 < 16; n1++)
 #pragma unroll
         for (int s = 0; s < S1; s++) v[s][n1] = S::ld(Rc[Q * n1 + 16 * s]);
+    // the loads above go out before the twiddle table is computed
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
 #pragma unroll
     for (int s = 0; s < S1; s++) {
         dft_reg<16, false>(v[s]);
@@ -1016,8 +1020,6 @@ __global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict
     const long b = blockIdx.y;
     const int col = blockIdx.x * FPB + f;
     const bool act = col < nb;
-    fill_twiddles<N>(twG, tid, NTH);
-    __syncthreads();
     // an idle transform reads column 0 and writes nothing
     const float2* Cb = C + (b * (long)nb + (act ? col : 0)) * (2 * band + 1);
     float2 v[S1][16];
@@ -1033,6 +1035,9 @@ __global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict
             }
             v[s][n1] = z;
         }
+    // the loads above go out before the twiddle table is computed
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
 #pragma unroll
     for (int s = 0; s < S1; s++) {
         dft_reg<16, true>(v[s]);
@@ -1274,13 +1279,14 @@ __global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __r
     const long b = blockIdx.y;
     const int col = blockIdx.x * FPB + f;
     const bool act = col < cx;
-    fill_twiddles<N>(twG, tid, NTH);
-    __syncthreads();
     using S = Store<H16>;
     const typename S::T* Rc = (const typename S::T*)R_ + (b * cx + (act ? col : 0)) * (long)N;
+    // the column loads go out first: the twiddle table is computed while they fly
     float2 v[32];
 #pragma unroll
     for (int n1 = 0; n1 < 32; n1++) v[n1] = S::ld(Rc[32 * n1 + t]);  // idle: column 0
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
     dft_reg<32, false>(v);
 #pragma unroll
     for (int k1 = 1; k1 < 32; k1++) v[k1] = cmul_w<false>(v[k1], twG[t * k1]);
@@ -1326,9 +1332,8 @@ __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __r
     const long b = blockIdx.y;
     const int col = blockIdx.x * FPB + f;
     const bool act = col < nb;
-    fill_twiddles<N>(twG, tid, NTH);
-    __syncthreads();
-    // an idle transform reads column 0 and writes nothing
+    // an idle transform reads column 0 and writes nothing; the band loads go
+    // out before the twiddle table is computed
     const float2* Cb = C + (b * (long)nb + (act ? col : 0)) * (2 * band + 1);
     float2 v[32];
 #pragma unroll
@@ -1341,6 +1346,8 @@ __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __r
         }
         v[n1] = z;
     }
+    fill_twiddles<N>(twG, tid, NTH);
+    __syncthreads();
     dft_reg<32, true>(v);
 #pragma unroll
     for (int k1 = 1; k1 < 32; k1++) {
