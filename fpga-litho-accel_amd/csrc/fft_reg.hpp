@@ -219,13 +219,27 @@ struct Fft2Geom {
 // transpose buffer xb[k1 * LDQ + n2].
 // W_G^m = e^{-2 pi i m / G} for m < G (G need not be a power of two), from
 // single-precision sincospi of the reduced argument 2m/G (the pi-scaled
-// argument keeps the reduction exact; results within ~1 ulp).
+// argument keeps the reduction exact; results within ~1 ulp).  For G a
+// multiple of 4 one sincospi gives four entries: W^(m + G/4) = -i W^m,
+// W^(m + G/2) = -W^m, W^(m + 3G/4) = i W^m are exact sign and part swaps.
 template <int G>
 __device__ __forceinline__ void fill_twiddles(float2* __restrict__ twG, int tid, int nth) {
-    for (int i = tid; i < G; i += nth) {
-        float sn, cs;
-        sincospif(2.0f * (float)i / (float)G, &sn, &cs);
-        twG[i] = make_float2(cs, -sn);
+    if constexpr (G % 4 == 0) {
+        constexpr int G4 = G / 4;
+        for (int i = tid; i < G4; i += nth) {
+            float sn, cs;
+            sincospif(2.0f * (float)i / (float)G, &sn, &cs);
+            twG[i] = make_float2(cs, -sn);
+            twG[i + G4] = make_float2(-sn, -cs);
+            twG[i + 2 * G4] = make_float2(-cs, sn);
+            twG[i + 3 * G4] = make_float2(sn, cs);
+        }
+    } else {
+        for (int i = tid; i < G; i += nth) {
+            float sn, cs;
+            sincospif(2.0f * (float)i / (float)G, &sn, &cs);
+            twG[i] = make_float2(cs, -sn);
+        }
     }
 }
 
