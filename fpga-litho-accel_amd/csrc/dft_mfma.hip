@@ -46,10 +46,11 @@ __device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
 }
 
 // two bytes of d (byte 2*hi, 2*hi+1) as exact fp16 values packed in a dword:
-// 0x6400 | b is the fp16 value 1024 + b, minus 1024 is exact.
+// 0x6400 | b is the fp16 value 1024 + b, minus 1024 is exact.  One byte
+// permute builds both halves (selector bytes 0-3 pick bytes of d, 4 picks
+// 0x64 from the constant word).
 __device__ __forceinline__ unsigned u8pair_to_h2(unsigned d, int hi) {
-    const unsigned s = hi ? d >> 16 : d;
-    const unsigned bits = (s & 0xffu) | ((s & 0xff00u) << 8) | 0x64006400u;
+    const unsigned bits = __builtin_amdgcn_perm(0x64646464u, d, hi ? 0x04030402u : 0x04010400u);
     h2v v = __builtin_bit_cast(h2v, bits);
     v = v - (h2v){(_Float16)1024.f, (_Float16)1024.f};
     return __builtin_bit_cast(unsigned, v);
