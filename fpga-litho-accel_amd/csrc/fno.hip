@@ -1,7 +1,7 @@
 // Fourier-neural-operator surrogate of the SOCS aerial image: forward and
 // backward kernels, Adam, and the litho_fno_* C-ABI.  The reference has no
-// learned model; this is the build's own surrogate beside the SOCS path,
-// which stays the judge of its error and the fallback.
+// learned model; this surrogate sits beside the SOCS path, which stays the
+// measure of its error and the fallback.
 //
 // Model (width C = 32, MX = 16 x-modes, MY = 32 y-modes = |ky| < 16 with the
 // negative half, PJ = 128 projection channels, 4 Fourier layers):

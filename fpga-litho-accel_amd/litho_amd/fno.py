@@ -1,6 +1,6 @@
 """Learned surrogate of the SOCS aerial image: a Fourier neural operator
 trained on SOCS-generated pairs by the GPU's own forward / backward kernels
-(litho_fno_* in the C-ABI), and a server that lets SOCS judge and back it up.
+(litho_fno_* in the C-ABI), and a server in which SOCS checks and backs it up.
 
 The reference has no learned model.  SOCS stays the ground truth: the
 training targets are SOCS images of seeded Manhattan masks, and in serving

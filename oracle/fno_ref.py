@@ -3,7 +3,7 @@ test infrastructure only: the GPU kernels (fpga-litho-accel_amd/csrc/fno.hip)
 are checked against it, and it is checked against finite differences and
 against numpy's rfft2 / irfft2 definition of a mode-truncated spectral
 convolution.  The reference repository has no learned model; this file
-restates this build's model, not reference code.
+restates the library's own model, not reference code.
 
 Model: width 32, 16 x-modes (kx = 0..15), 32 y-modes (ky = 0..15 and
 -16..-1), 4 Fourier layers, GELU, projection 32 -> 128 -> 1; parameters in one

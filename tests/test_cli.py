@@ -79,8 +79,8 @@ def test_full_flow_fixture_config(tmp_path, orc):
 
 
 @pytest.mark.gpu
-def test_full_flow_baseline_shape_source_route(tmp_path, orc):
-    out = tmp_path / "c1"
+def test_full_flow_512_source_route(tmp_path, orc):
+    out = tmp_path / "clip512"
     run(os.path.join(CFG, "manhattan_512_4nm.json"), str(out), "--quiet", "--no-tcc")
     assert not (out / "tcc_r.bin").exists()
     N, Nx, n = 512, 17, 35 * 35

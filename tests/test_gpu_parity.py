@@ -22,7 +22,7 @@ def case(msg):
 
 
 @pytest.fixture(scope="module")
-def c1(orc, base_source):
+def clip512(orc, base_source):
     """512^2 at 4 nm, Nx = 17, K = 10: oracle kernels via the Gram route."""
     src, s = base_source
     N = 512
@@ -51,10 +51,10 @@ def test_tcc_build_fixture_config(gpu, orc, fix_source, fixture_tcc):
     assert np.array_equal(T, T.conj().T)
 
 
-def test_tcc_build_baseline_c1(gpu, orc, c1):
+def test_tcc_build_baseline_c1(gpu, orc, clip512):
     case("tcc 41-source N=512 Nx=17 defocus 0")
-    T = gpu.calcTCC(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
-    To = orc.calc_tcc(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
+    T = gpu.calcTCC(clip512["src"], clip512["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, clip512["Nx"], clip512["Nx"])
+    To = orc.calc_tcc(clip512["src"], clip512["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, clip512["Nx"], clip512["Nx"])
     assert max_rel(T, To) < 1e-12
     assert not T.imag.any()
 
@@ -88,14 +88,14 @@ def test_eigenvalues_match_reference_runs(gpu, orc, fix_source, L, fname):
     assert np.all(np.abs(sc[big] - ref[:k][big]) / ref[:k][big] < 6e-6)
 
 
-def test_kernels_gram_route_c1(gpu, orc, c1):
+def test_kernels_gram_route_c1(gpu, orc, clip512):
     case("kernels_from_source 41-source N=512 nk=10 (Gram)")
-    sc, K, fl = gpu.kernels_from_source(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048,
-                                        c1["Nx"], c1["Nx"], 10)
+    sc, K, fl = gpu.kernels_from_source(clip512["src"], clip512["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048,
+                                        clip512["Nx"], clip512["Nx"], 10)
     assert fl & gpu.LITHO_FLAG_GRAM_ROUTE
     assert not fl & gpu.LITHO_FLAG_CLUSTER_SPLIT
-    assert np.abs(sc - c1["sc"]).max() / c1["sc"][0] < 1e-10
-    assert np.abs(K - c1["K"]).max() < 1e-6
+    assert np.abs(sc - clip512["sc"]).max() / clip512["sc"][0] < 1e-10
+    assert np.abs(K - clip512["K"]).max() < 1e-6
 
 
 # ------------------------------------------------------------ Module 3 / 7 ---
@@ -105,40 +105,40 @@ def test_mask_spectrum(gpu, orc):
     assert max_rel(gpu.FT_r2c_padded(m), orc.mask_spectrum(m)) < IMG_TOL
 
 
-def test_socs_tmpimg_hls_entry(gpu, orc, c1):
-    case("socs tmpimg C1")
-    m = gpu.manhattan_mask(c1["N"], 4)
+def test_socs_tmpimg_hls_entry(gpu, orc, clip512):
+    case("socs tmpimg 512^2 K=10")
+    m = gpu.manhattan_mask(clip512["N"], 4)
     mf = orc.mask_spectrum(m)
-    t = gpu.calc_socs_hls(mf, c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
-    to = orc.socs_tmpimg(mf, c1["K"], c1["sc"].astype(np.float32), c1["Nx"], c1["Nx"])
+    t = gpu.calc_socs_hls(mf, clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
+    to = orc.socs_tmpimg(mf, clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"], clip512["Nx"])
     assert t.shape == (128, 128)
     assert max_rel(t, to) < IMG_TOL
 
 
-def test_fi(gpu, orc, c1):
+def test_fi(gpu, orc, clip512):
     case("FI 128 -> 512")
-    m = gpu.manhattan_mask(c1["N"], 5)
-    to = orc.socs_tmpimg(orc.mask_spectrum(m), c1["K"], c1["sc"].astype(np.float32), c1["Nx"],
-                         c1["Nx"])
-    assert max_rel(gpu.FI(to, 512, 2 * c1["Nx"]), orc.fi(to, 512, 512)) < IMG_TOL
+    m = gpu.manhattan_mask(clip512["N"], 5)
+    to = orc.socs_tmpimg(orc.mask_spectrum(m), clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"],
+                         clip512["Nx"])
+    assert max_rel(gpu.FI(to, 512, 2 * clip512["Nx"]), orc.fi(to, 512, 512)) < IMG_TOL
 
 
-def test_calc_socs_from_spectrum(gpu, orc, c1):
-    case("calcSOCS from spectrum C1")
-    m = gpu.manhattan_mask(c1["N"], 6)
+def test_calc_socs_from_spectrum(gpu, orc, clip512):
+    case("calcSOCS from spectrum 512^2 K=10")
+    m = gpu.manhattan_mask(clip512["N"], 6)
     mf = orc.mask_spectrum(m)
-    img = gpu.calcSOCS(mf, c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
-    ref = orc.calc_socs(mf, c1["K"], c1["sc"].astype(np.float32), c1["Nx"], c1["Nx"])
+    img = gpu.calcSOCS(mf, clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
+    ref = orc.calc_socs(mf, clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"], clip512["Nx"])
     assert max_rel(img, ref) < IMG_TOL
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-def test_socs_image_c1(gpu, orc, c1, seed):
-    case(f"socs_image C1 seed {seed}")
-    m = gpu.manhattan_mask(c1["N"], seed)
-    img = gpu.socs_image(m, c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
-    ref = orc.calc_socs(orc.mask_spectrum(m), c1["K"], c1["sc"].astype(np.float32), c1["Nx"],
-                        c1["Nx"])
+def test_socs_image_c1(gpu, orc, clip512, seed):
+    case(f"socs_image 512^2 K=10 seed {seed}")
+    m = gpu.manhattan_mask(clip512["N"], seed)
+    img = gpu.socs_image(m, clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
+    ref = orc.calc_socs(orc.mask_spectrum(m), clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"],
+                        clip512["Nx"])
     assert max_rel(img, ref) < IMG_TOL
 
 
@@ -155,7 +155,7 @@ def test_socs_image_reference_fixture_full_rank(gpu, orc, fixture_tcc):
 
 
 def test_socs_image_c2_full_size(gpu, orc, base_source):
-    case("socs_image C2 N=1024 Nx=34 K=24")
+    case("socs_image N=1024 Nx=34 K=24")
     src, s = base_source
     N = 1024
     Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
@@ -174,20 +174,20 @@ def test_socs_image_c2_full_size(gpu, orc, base_source):
     assert np.abs(clear - expect).max() < 1e-5 * expect
 
 
-def test_batch_equals_single(gpu, orc, c1):
+def test_batch_equals_single(gpu, orc, clip512):
     case("batch of 3 vs single")
     ctx = gpu.Context()
     try:
-        N = c1["N"]
+        N = clip512["N"]
         masks = np.stack([gpu.manhattan_mask(N, 20 + i) for i in range(3)])
-        bank = ctx.bank(c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+        bank = ctx.bank(clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
         dm = ctx.alloc(masks.nbytes)
         di = ctx.alloc(masks.nbytes)
         ctx.h2d(dm, masks)
         ctx.socs_image_batch(bank, dm, 3, N, di)
         out = ctx.d2h(np.zeros_like(masks), di)
         for i in range(3):
-            single = gpu.socs_image(masks[i], c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+            single = gpu.socs_image(masks[i], clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
             assert np.array_equal(out[i], single)
         ctx.free(dm)
         ctx.free(di)
@@ -251,24 +251,24 @@ def test_other_clip_sizes(gpu, orc, base_source, N):
     assert max_rel(img, ref) < IMG_TOL
 
 
-def test_single_kernel_and_empty_mask(gpu, orc, c1):
+def test_single_kernel_and_empty_mask(gpu, orc, clip512):
     case("K=1 and empty mask")
-    m = gpu.manhattan_mask(c1["N"], 9)
-    img = gpu.socs_image(m, c1["K"][:1], c1["sc"][:1], c1["Nx"], c1["Nx"])
-    ref = orc.calc_socs(orc.mask_spectrum(m), c1["K"][:1], c1["sc"][:1].astype(np.float32),
-                        c1["Nx"], c1["Nx"])
+    m = gpu.manhattan_mask(clip512["N"], 9)
+    img = gpu.socs_image(m, clip512["K"][:1], clip512["sc"][:1], clip512["Nx"], clip512["Nx"])
+    ref = orc.calc_socs(orc.mask_spectrum(m), clip512["K"][:1], clip512["sc"][:1].astype(np.float32),
+                        clip512["Nx"], clip512["Nx"])
     assert max_rel(img, ref) < IMG_TOL
-    z = gpu.socs_image(np.zeros((c1["N"], c1["N"]), np.float32), c1["K"], c1["sc"], c1["Nx"],
-                       c1["Nx"])
+    z = gpu.socs_image(np.zeros((clip512["N"], clip512["N"]), np.float32), clip512["K"], clip512["sc"], clip512["Nx"],
+                       clip512["Nx"])
     assert not z.any()
 
 
-def test_invalid_requests_raise(gpu, c1):
+def test_invalid_requests_raise(gpu, clip512):
     case("invalid shapes")
     with pytest.raises(gpu.LithoError):
-        gpu.socs_image(np.zeros((100, 100), np.float32), c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+        gpu.socs_image(np.zeros((100, 100), np.float32), clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
     with pytest.raises(gpu.LithoError):
-        gpu.socs_image(np.zeros((64, 64), np.float32), c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+        gpu.socs_image(np.zeros((64, 64), np.float32), clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
 
 
 @pytest.mark.parametrize("precision,graph", [(0, False), (0, True), (5, True)])
@@ -371,11 +371,11 @@ def test_matrix_core_forms(gpu, orc, base_source, N, prec):
         ctx.close()
 
 
-def test_matrix_core_flag_rules(gpu, c1):
+def test_matrix_core_flag_rules(gpu, clip512):
     """Unknown flags and two bank formats at once fail."""
     ctx = gpu.Context()
     try:
-        bank = ctx.bank(c1["K"], c1["sc"], c1["Nx"], c1["Nx"])
+        bank = ctx.bank(clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
         d = ctx.alloc(4 * 512 * 512)
         for bad in (64, 128, 2 | 4, 2 | 4 | 32):
             with pytest.raises(gpu.LithoError):
@@ -405,9 +405,9 @@ def test_split_bf16_tcc(gpu, base_source, defocus):
 
 
 def test_split_bf16_kernels_and_image(gpu, orc, base_source):
-    """C2 precision form: split-BF16 Gram contraction + FP64 eigensolve + FP64
+    """1024^2, K=24 precision form: split-BF16 Gram contraction + FP64 eigensolve + FP64
     lift; eigenvalues, kernels and the FP32 SOCS image against FP64."""
-    case("split-bf16 Gram route C2")
+    case("split-bf16 Gram route 1024^2 K=24")
     src, s = base_source
     N, nk = 1024, 24
     Nx = gpu.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
@@ -422,7 +422,7 @@ def test_split_bf16_kernels_and_image(gpu, orc, base_source):
     ref = orc.calc_socs(orc.mask_spectrum(m), K64.astype(np.complex128), sc64.astype(np.float32),
                         Nx, Nx, nthreads=8)
     ierr = max_rel(img, ref)
-    print(f"[split-bf16 C2] eigenvalues {eig:.3e} kernels {kerr:.3e} image {ierr:.3e} (vs FP64)",
+    print(f"[split-bf16 1024^2 K=24] eigenvalues {eig:.3e} kernels {kerr:.3e} image {ierr:.3e} (vs FP64)",
           flush=True)
     assert eig < 1e-4
     assert kerr < 1e-3  # canonical basis in the mirror-degenerate pairs as well
@@ -440,19 +440,19 @@ def test_tcc_direct_image_reference_fixture(gpu, orc, fixture_tcc):
     assert max_rel(img, gold) < IMG_TOL
 
 
-def test_tcc_direct_image_c1_and_full_rank_socs(gpu, orc, c1):
+def test_tcc_direct_image_c1_and_full_rank_socs(gpu, orc, clip512):
     """TCC-direct image on a 512^2 production-optics clip vs the oracle, and the
     SOCS image with every kernel (K = n via the explicit route) converging to it."""
-    case("tcc-direct image C1")
-    T = orc.calc_tcc(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, c1["Nx"], c1["Nx"])
-    m = gpu.manhattan_mask(c1["N"], 77)
-    img = gpu.calcImage(m, T, c1["Nx"], c1["Nx"])
-    ref = orc.calc_image_tcc(orc.mask_spectrum(m), T, c1["Nx"], c1["Nx"])
+    case("tcc-direct image 512^2")
+    T = orc.calc_tcc(clip512["src"], clip512["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048, clip512["Nx"], clip512["Nx"])
+    m = gpu.manhattan_mask(clip512["N"], 77)
+    img = gpu.calcImage(m, T, clip512["Nx"], clip512["Nx"])
+    ref = orc.calc_image_tcc(orc.mask_spectrum(m), T, clip512["Nx"], clip512["Nx"])
     assert max_rel(img, ref) < IMG_TOL
     # rank of the TCC is 572 (< n = 1225): the top 572 kernels span it
-    sc, K, _ = gpu.kernels_from_source(c1["src"], c1["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048,
-                                       c1["Nx"], c1["Nx"], 572)
-    socs = gpu.socs_image(m, K, sc, c1["Nx"], c1["Nx"])
+    sc, K, _ = gpu.kernels_from_source(clip512["src"], clip512["s"], BASE_LAMBDA, BASE_NA, 0.0, 2048, 2048,
+                                       clip512["Nx"], clip512["Nx"], 572)
+    socs = gpu.socs_image(m, K, sc, clip512["Nx"], clip512["Nx"])
     assert max_rel(socs, img) < 1e-4
 
 
