@@ -1170,10 +1170,16 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->band.d_scales, bank->band.n, Nx, B,
                                (float2*)w.C, pl.socs_shape, st));
     if (mfma) {
-        // FP32 U: the image-side product splits it into hi + lo parts
-        LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st));
-        LITHO_TRY(launch_image_rows_mfma(N, (const float2*)w.U, band + 1, ti, d_image, B, out16,
-                                         st));
+        // U for the image-side product: at 1024 points the column pass writes
+        // it already split into fp16 hi + lo parts (its own form, fewer bytes);
+        // elsewhere FP32, split by the product on the fly
+        const bool usplit = N == 1024 && 2 * band + 1 <= 160;
+        if (usplit)
+            LITHO_TRY(launch_fi_cols_split(N, (float2*)w.C, w.U, B, band,
+                                           image_rows_split_cols(band + 1), st));
+        else
+            LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st));
+        LITHO_TRY(launch_image_rows_mfma(N, w.U, band + 1, ti, d_image, B, out16, usplit, st));
     } else {
         LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st));
         LITHO_TRY(launch_image_rows_t(N, w.U, d_image, (long)B * N, band, mixed, pl.lean_mem,

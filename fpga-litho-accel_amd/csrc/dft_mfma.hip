@@ -231,13 +231,16 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
 // rounding it to fp16 alone adds below 1e-7 of the image at 1024^2 (measured on
 // the FP64 images of Manhattan and random grey-level masks), where rounding all
 // of U would add 3-5e-4.
+// U arrives as FP32 [nb][N] and is split here, or (USPLIT) already in that
+// form from the column pass: [ns][N] (hi, lo) half2 pairs for the split
+// columns kx < ns = min(nb, 8 KS), then [nb - ns][N] hi half2, per image.
 // The product is formed transposed (table as the A operand, U as B): a lane's
 // D fragment is 4 consecutive pixels of one image row.  Fragments are staged
 // per wave in LDS until every row has a full 128-byte segment, which is then
 // stored 16 bytes per lane (8 rows per instruction), so HBM sees whole lines.
 // ---------------------------------------------------------------------------
-template <int NKS, int KS, bool OUT16, int RT, int NTH>
-__global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const float2* __restrict__ U,
+template <int NKS, int KS, bool OUT16, int RT, int NTH, bool USPLIT>
+__global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const void* __restrict__ U_,
                                                                  const u4v* __restrict__ Hpk,
                                                                  void* __restrict__ img_, int N,
                                                                  int nb) {
@@ -258,7 +261,14 @@ __global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const float2* _
     const int g = lane >> 4, r16 = lane & 15;
     const int b = blockIdx.y;
     const int ybase = blockIdx.x * NW * RW + w * RW;
-    const float2* Ub = U + (long)b * nb * N;
+    // FP32 U [nb][N], or the split form: [ns][N] (hi, lo) half2 pairs, then
+    // [nb - ns][N] hi half2
+    const int ns = USPLIT ? min(nb, 8 * KS) : 0;
+    const float2* Ub = (const float2*)U_ + (long)b * nb * N;
+    const unsigned char* Us =
+        (const unsigned char*)U_ + (long)b * ((long)ns * N * 8 + (long)(nb - ns) * N * 4);
+    const uint2* Usp = (const uint2*)Us;
+    const unsigned* Uhi = (const unsigned*)(Us + (long)ns * N * 8);
     u4v pre[PER];
 #pragma unroll
     for (int i = 0; i < PER; i++)
@@ -271,7 +281,24 @@ __global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const float2* _
 #pragma unroll
         for (int s = 0; s < NKS; s++) {
             unsigned q[4];
-            if (s < KS) {
+            if (USPLIT && s < KS) {
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int kx = 8 * s + 2 * g + e;  // < 8 KS: split while kx < nb
+                    uint2 v = Usp[(long)min(kx, ns - 1) * N + y];
+                    if (kx >= ns) v = make_uint2(0u, 0u);
+                    q[2 * e] = v.x;
+                    q[2 * e + 1] = v.y;
+                }
+            } else if (USPLIT) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int kx = 8 * KS + 16 * (s - KS) + 4 * g + e;  // >= ns here
+                    unsigned v = Uhi[(long)min(kx - ns, nb - ns - 1) * N + y];
+                    if (kx >= nb) v = 0u;
+                    q[e] = v;
+                }
+            } else if (s < KS) {
 #pragma unroll
                 for (int e = 0; e < 2; e++) {
                     const int kx = 8 * s + 2 * g + e;
@@ -470,8 +497,14 @@ int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, v
     return LITHO_EARG;
 }
 
-int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, void* img, int B,
-                           bool out16, hipStream_t st) {
+int image_rows_split_cols(int nb) {
+    int NKS, KS;
+    image_rows_steps(nb, &NKS, &KS);
+    return std::min(nb, 8 * KS);
+}
+
+int launch_image_rows_mfma(int N, const void* U, int nb, const void* table, void* img, int B,
+                           bool out16, bool usplit, hipStream_t st) {
     if (N % ROWS_PER_BLOCK || N % XCH || B <= 0 || B > 65535 || nb <= 0 || nb > 72)
         return LITHO_EARG;
     int NKS, KS;
@@ -481,14 +514,19 @@ int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, vo
     static_assert(ROWS_PER_BLOCK % (INTH / 4 * IRT) == 0, "image rows per block divide N");
     const dim3 grid(N / (INTH / 4 * IRT), B);
     const u4v* H = (const u4v*)table;
+#define LITHO_IR1(nks, ks, o16, us)                                                            \
+    hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, o16, IRT, INTH, us>), grid, dim3(INTH), \
+                       0, st, U, H, img, N, nb)
 #define LITHO_IR(nks, ks)                                                                      \
     if (NKS == nks && KS == ks) {                                                              \
-        if (out16)                                                                             \
-            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, true, IRT, INTH>), grid, dim3(INTH), 0, \
-                               st, U, H, img, N, nb);                                          \
+        if (out16 && usplit)                                                                   \
+            LITHO_IR1(nks, ks, true, true);                                                    \
+        else if (out16)                                                                        \
+            LITHO_IR1(nks, ks, true, false);                                                   \
+        else if (usplit)                                                                       \
+            LITHO_IR1(nks, ks, false, true);                                                   \
         else                                                                                   \
-            hipLaunchKernelGGL((image_rows_mfma_kernel<nks, ks, false, IRT, INTH>), grid, dim3(INTH), 0, \
-                               st, U, H, img, N, nb);                                          \
+            LITHO_IR1(nks, ks, false, false);                                                  \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     // every (steps, split steps) pair of nb <= 72
@@ -502,6 +540,7 @@ int launch_image_rows_mfma(int N, const float2* U, int nb, const void* table, vo
     LITHO_IR(6, 4)
     LITHO_IR(7, 5)
 #undef LITHO_IR
+#undef LITHO_IR1
     return LITHO_EARG;
 }
 

@@ -1319,9 +1319,13 @@ __global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __r
     }
 }
 
-template <bool H16, int NTH = 256>
+// SPLIT: U in the image-side product's fp16 form (columns kx < ns as (hi,
+// lo * 2^12) half2 pairs, the rest hi half2; per image [ns][N] 8-byte then
+// [nb - ns][N] 4-byte entries), the rounding the product would do itself.
+template <bool H16, int NTH = 256, bool SPLIT = false>
 __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __restrict__ C,
-                                                               void* __restrict__ U_, int band) {
+                                                               void* __restrict__ U_, int band,
+                                                               int ns) {
     using Geo = Fft2Geom<32, 32>;
     constexpr int N = 1024, T = 32, FPB = NTH / T, NZ = 5;
     __shared__ float2 twG[N];
@@ -1359,9 +1363,33 @@ __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __r
     transpose32_planes(t, xplane + f * Geo::XBUF, v, u);
     if (!act) return;
     dft_reg<32, true>(u);  // u[k2] = row y = t + 32 k2
+    const float sx = (col & 1) ? -1.f : 1.f;
+    if (SPLIT) {
+        unsigned char* Ui =
+            (unsigned char*)U_ + b * ((long)ns * N * 8 + (long)(nb - ns) * N * 4);
+        uint2* Usp = (uint2*)Ui + (long)(col < ns ? col : 0) * N;
+        unsigned* Uhi = (unsigned*)(Ui + (long)ns * N * 8) + (long)(col < ns ? 0 : col - ns) * N;
+#pragma unroll
+        for (int k2 = 0; k2 < 32; k2++) {
+            const int y = t + 32 * k2;
+            const float2 w = twG[(int)(((long)band * y) % N)];  // e^{-2 pi i band y / N}
+            const float2 z = cscale(cmul(u[k2], w), sx);
+            const _Float16 hr = (_Float16)z.x, hi = (_Float16)z.y;
+            const unsigned hb = (unsigned)__builtin_bit_cast(unsigned short, hr) |
+                                ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+            if (col < ns) {
+                const _Float16 lr = (_Float16)((z.x - (float)hr) * 4096.f);
+                const _Float16 li = (_Float16)((z.y - (float)hi) * 4096.f);
+                Usp[y] = make_uint2(hb, (unsigned)__builtin_bit_cast(unsigned short, lr) |
+                                            ((unsigned)__builtin_bit_cast(unsigned short, li) << 16));
+            } else {
+                Uhi[y] = hb;
+            }
+        }
+        return;
+    }
     using S = Store<H16>;
     typename S::T* Uc = (typename S::T*)U_ + (b * nb + col) * (long)N;
-    const float sx = (col & 1) ? -1.f : 1.f;
 #pragma unroll
     for (int k2 = 0; k2 < 32; k2++) {
         const int y = t + 32 * k2;
@@ -2130,6 +2158,16 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
     return LITHO_EARG;
 }
 
+int launch_fi_cols_split(int N, const float2* C, void* U, int B, int band, int ns,
+                         hipStream_t st) {
+    if (B <= 0 || B > 65535 || N != 1024 || 2 * band + 1 > 160 || ns < 1 || ns > band + 1)
+        return LITHO_EARG;
+    const dim3 grid(cdiv(band + 1, 8), B);
+    hipLaunchKernelGGL((fi_cols_band1024_kernel<false, 256, true>), grid, dim3(256), 0, st, C, U,
+                       band, ns);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
                      hipStream_t st) {
     if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
@@ -2151,9 +2189,9 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
     if (N == 1024 && 2 * band + 1 <= 160) {  // every plan: one kernel, the same bits
         const dim3 grid(cdiv(band + 1, 8), B);
         if (h16)
-            hipLaunchKernelGGL((fi_cols_band1024_kernel<true>), grid, dim3(256), 0, st, C, U, band);
+            hipLaunchKernelGGL((fi_cols_band1024_kernel<true>), grid, dim3(256), 0, st, C, U, band, 0);
         else
-            hipLaunchKernelGGL((fi_cols_band1024_kernel<false>), grid, dim3(256), 0, st, C, U, band);
+            hipLaunchKernelGGL((fi_cols_band1024_kernel<false>), grid, dim3(256), 0, st, C, U, band, 0);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
 #define LITHO_FC(p, q, nz, nth)                                                                 \
