@@ -231,9 +231,9 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
 // rounding it to fp16 alone adds below 1e-7 of the image at 1024^2 (measured on
 // the FP64 images of Manhattan and random grey-level masks), where rounding all
 // of U would add 3-5e-4.
-// U arrives as FP32 [nb][N] and is split here, or (USPLIT) already in that
-// form from the column pass: [ns][N] (hi, lo) half2 pairs for the split
-// columns kx < ns = min(nb, 8 KS), then [nb - ns][N] hi half2, per image.
+// U arrives as FP32 [nb][N], or (USPLIT) in a compact form from the column
+// pass: per image [ns][N] FP32 for the split columns kx < ns = min(nb, 8 KS),
+// then [nb - ns][N] fp16 for the rest (rounded as the product would).
 // The product is formed transposed (table as the A operand, U as B): a lane's
 // D fragment is 4 consecutive pixels of one image row.  Fragments are staged
 // per wave in LDS until every row has a full 128-byte segment, which is then
@@ -261,13 +261,12 @@ __global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const void* __r
     const int g = lane >> 4, r16 = lane & 15;
     const int b = blockIdx.y;
     const int ybase = blockIdx.x * NW * RW + w * RW;
-    // FP32 U [nb][N], or the split form: [ns][N] (hi, lo) half2 pairs, then
-    // [nb - ns][N] hi half2
+    // FP32 U [nb][N], or the compact form: [ns][N] FP32 for the columns the
+    // product splits, then [nb - ns][N] fp16 (its hi parts, already rounded)
     const int ns = USPLIT ? min(nb, 8 * KS) : 0;
-    const float2* Ub = (const float2*)U_ + (long)b * nb * N;
     const unsigned char* Us =
         (const unsigned char*)U_ + (long)b * ((long)ns * N * 8 + (long)(nb - ns) * N * 4);
-    const uint2* Usp = (const uint2*)Us;
+    const float2* Ub = USPLIT ? (const float2*)Us : (const float2*)U_ + (long)b * nb * N;
     const unsigned* Uhi = (const unsigned*)(Us + (long)ns * N * 8);
     u4v pre[PER];
 #pragma unroll
@@ -281,16 +280,7 @@ __global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const void* __r
 #pragma unroll
         for (int s = 0; s < NKS; s++) {
             unsigned q[4];
-            if (USPLIT && s < KS) {
-#pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    const int kx = 8 * s + 2 * g + e;  // < 8 KS: split while kx < nb
-                    uint2 v = Usp[(long)min(kx, ns - 1) * N + y];
-                    if (kx >= ns) v = make_uint2(0u, 0u);
-                    q[2 * e] = v.x;
-                    q[2 * e + 1] = v.y;
-                }
-            } else if (USPLIT) {
+            if (USPLIT && s >= KS) {
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const int kx = 8 * KS + 16 * (s - KS) + 4 * g + e;  // >= ns here
@@ -301,7 +291,7 @@ __global__ __launch_bounds__(NTH, 2) void image_rows_mfma_kernel(const void* __r
             } else if (s < KS) {
 #pragma unroll
                 for (int e = 0; e < 2; e++) {
-                    const int kx = 8 * s + 2 * g + e;
+                    const int kx = 8 * s + 2 * g + e;  // < ns in the compact form
                     // clamped index and a select, not a branch: every load is in
                     // flight at once
                     float2 v = Ub[(long)min(kx, nb - 1) * N + y];

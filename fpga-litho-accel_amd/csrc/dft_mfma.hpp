@@ -19,9 +19,9 @@ int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, v
                           int count, bool h16, hipStream_t st);
 // U [B][nb][N] float2 (kx = 0..nb-1 of each image row) -> image [B][N][N]
 // (fp16 if out16, else f32), the c2r sum with the DC imaginary part dropped.
-// usplit: U already in the product's fp16 form, per image the first
-// ns = image_rows_split_cols(nb) columns as [ns][N] of (hi, lo * 2^12) half2
-// pairs (8 bytes), then [nb - ns][N] hi half2 (4 bytes).
+// usplit: U in a compact form, per image the first ns =
+// image_rows_split_cols(nb) columns as [ns][N] float2 (split into hi + lo here),
+// then [nb - ns][N] half2 (the fp16 values the product uses for them).
 int launch_image_rows_mfma(int N, const void* U, int nb, const void* table, void* img, int B,
                            bool out16, bool usplit, hipStream_t st);
 // number of U columns (kx = 0..ns-1) the image-side product keeps as hi + lo

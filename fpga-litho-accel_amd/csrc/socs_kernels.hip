@@ -1319,9 +1319,10 @@ __global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __r
     }
 }
 
-// SPLIT: U in the image-side product's fp16 form (columns kx < ns as (hi,
-// lo * 2^12) half2 pairs, the rest hi half2; per image [ns][N] 8-byte then
-// [nb - ns][N] 4-byte entries), the rounding the product would do itself.
+// SPLIT: U in the compact form of the image-side matrix-core product: per
+// image [ns][N] FP32 for the columns kx < ns that the product splits into fp16
+// hi + lo parts itself, then [nb - ns][N] half2 for the rest, which it uses
+// as fp16 values alone (rounded here as it would round them).
 template <bool H16, int NTH = 256, bool SPLIT = false>
 __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __restrict__ C,
                                                                void* __restrict__ U_, int band,
@@ -1367,24 +1368,18 @@ __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __r
     if (SPLIT) {
         unsigned char* Ui =
             (unsigned char*)U_ + b * ((long)ns * N * 8 + (long)(nb - ns) * N * 4);
-        uint2* Usp = (uint2*)Ui + (long)(col < ns ? col : 0) * N;
-        unsigned* Uhi = (unsigned*)(Ui + (long)ns * N * 8) + (long)(col < ns ? 0 : col - ns) * N;
+        const bool lo = col < ns;
+        float2* Uf = (float2*)Ui + (long)(lo ? col : 0) * N;
+        __half2* Uh = (__half2*)(Ui + (long)ns * N * 8) + (long)(lo ? 0 : col - ns) * N;
 #pragma unroll
         for (int k2 = 0; k2 < 32; k2++) {
             const int y = t + 32 * k2;
             const float2 w = twG[(int)(((long)band * y) % N)];  // e^{-2 pi i band y / N}
             const float2 z = cscale(cmul(u[k2], w), sx);
-            const _Float16 hr = (_Float16)z.x, hi = (_Float16)z.y;
-            const unsigned hb = (unsigned)__builtin_bit_cast(unsigned short, hr) |
-                                ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
-            if (col < ns) {
-                const _Float16 lr = (_Float16)((z.x - (float)hr) * 4096.f);
-                const _Float16 li = (_Float16)((z.y - (float)hi) * 4096.f);
-                Usp[y] = make_uint2(hb, (unsigned)__builtin_bit_cast(unsigned short, lr) |
-                                            ((unsigned)__builtin_bit_cast(unsigned short, li) << 16));
-            } else {
-                Uhi[y] = hb;
-            }
+            if (lo)
+                Uf[y] = z;
+            else
+                Uh[y] = __floats2half2_rn(z.x, z.y);
         }
         return;
     }

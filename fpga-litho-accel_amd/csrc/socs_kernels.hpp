@@ -51,8 +51,9 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float2* ksc
 // C -> U [B][2Nx+1][N]
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
                      hipStream_t st);
-// C -> U in the split fp16 form of launch_image_rows_mfma (usplit), ns split
-// columns; N = 1024 with 2 band + 1 <= 160 only (else LITHO_EARG)
+// C -> U in the compact form of launch_image_rows_mfma (usplit): FP32 for
+// the ns split columns, fp16 above; N = 1024 with 2 band + 1 <= 160 only
+// (else LITHO_EARG)
 int launch_fi_cols_split(int N, const float2* C, void* U, int B, int band, int ns,
                          hipStream_t st);
 // U [B][band+1][N] -> image [B*N][N]
