@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <tuple>
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "dft_mfma.hpp"
 #include "eigen_kernels.hpp"
@@ -96,11 +98,18 @@ struct KernelSet {
 // field IFFT(K M) that is real up to the phase ramp common to every field of
 // the clip; two such kernels a, b then share one transform: the entry
 // sqrt(la) Ka + i sqrt(lb) Kb, weight 1, has |IFFT|^2 = la |Ea|^2 + lb |Eb|^2.
+// Kernels with Nx != Ny are stored zero-padded into the square window of
+// half-width max(Nx, Ny) (a kernel is zero outside its window, so every
+// product K M and every field is unchanged); Nx and Ny below are that square
+// half-width, nx0 / ny0 the kernels' own.  serial is unique per bank for the
+// life of the process: cached graphs are keyed by it, never by the address.
 struct litho_bank {
     litho_ctx* ctx = nullptr;
     KernelSet full, band;  // band aliases full when no two kernels pair
     int nk = 0, Nx = 0, Ny = 0;
+    int nx0 = 0, ny0 = 0;
     int pairs = 0;
+    long serial = 0;
 };
 
 static int ctx_ws(litho_ctx* c, int slot, size_t bytes, void** out) {
@@ -606,7 +615,10 @@ int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int
     LITHO_HIP(hipMemcpyAsync(ho.data(), order.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
     std::vector<double> wdesc(n);
-    for (int r = 0; r < n; r++) wdesc[r] = hw[ho[r]];
+    for (int r = 0; r < n; r++) {
+        if (ho[r] < 0 || ho[r] >= n || !std::isfinite(hw[ho[r]])) return LITHO_ENOCONV;
+        wdesc[r] = hw[ho[r]];
+    }
     int mtop, split;
     std::vector<int2> clusters;
     plan_clusters(wdesc, nk, &mtop, &split, clusters, rtol);
@@ -694,12 +706,16 @@ int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx
     if (!tcc_re || !tcc_im || !scales || !krn_r || !krn_i || nk <= 0 || nk > n ||
         n != (2 * Nx + 1) * (2 * Ny + 1))
         return LITHO_EARG;
+    const size_t nn = (size_t)n * n;
+    std::vector<double2> h(nn);
+    for (size_t i = 0; i < nn; i++) {
+        // a NaN or infinity would leave the eigensolver's ranking undefined
+        if (!std::isfinite(tcc_re[i]) || !std::isfinite(tcc_im[i])) return LITHO_EARG;
+        h[i] = make_double2(tcc_re[i], tcc_im[i]);
+    }
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
     hipStream_t st = c->stream;
-    const size_t nn = (size_t)n * n;
-    std::vector<double2> h(nn);
-    for (size_t i = 0; i < nn; i++) h[i] = make_double2(tcc_re[i], tcc_im[i]);
     Dev H;
     LITHO_TRY(H.alloc(nn * sizeof(double2)));
     LITHO_HIP(hipMemcpyAsync(H.p, h.data(), nn * sizeof(double2), hipMemcpyHostToDevice, st));
@@ -749,7 +765,10 @@ static int kernels_from_source_impl(const LithoOptics* op, const float* src, int
     LITHO_HIP(hipMemcpyAsync(ho.data(), order.p, m * sizeof(int), hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
     std::vector<double> wdesc(m);
-    for (int r = 0; r < m; r++) wdesc[r] = hw[ho[r]];
+    for (int r = 0; r < m; r++) {
+        if (ho[r] < 0 || ho[r] >= m || !std::isfinite(hw[ho[r]])) return LITHO_ENOCONV;
+        wdesc[r] = hw[ho[r]];
+    }
     int mtop, split;
     std::vector<int2> clusters;
     plan_clusters(wdesc, nk, &mtop, &split, clusters, rtol);
@@ -895,18 +914,30 @@ extern "C" {
 // ---------------------------------------------------------------------------
 int litho_bank_create(litho_ctx* c, const float* krn_r, const float* krn_i, const float* scales,
                       int nk, int Nx, int Ny, litho_bank** out) {
-    if (!c || !krn_r || !krn_i || !scales || !out || nk <= 0 || Nx < 0 || Ny != Nx)
+    if (!c || !krn_r || !krn_i || !scales || !out || nk <= 0 || Nx < 0 || Ny < 0 || Nx > 1023 ||
+        Ny > 1023)
         return LITHO_EARG;
     *out = nullptr;
-    const int W = 2 * Nx + 1;
+    static std::atomic<long> next_serial{1};
+    const int m = std::max(Nx, Ny);
+    const int W = 2 * m + 1, Wx = 2 * Nx + 1, Wy = 2 * Ny + 1;
     const size_t per = (size_t)W * W;
-    std::vector<float2> h((size_t)nk * per);
-    for (size_t i = 0; i < h.size(); i++) h[i] = make_float2(krn_r[i], krn_i[i]);
+    std::vector<float2> h((size_t)nk * per, make_float2(0.f, 0.f));
+    for (int k = 0; k < nk; k++)
+        for (int y = 0; y < Wy; y++)
+            for (int x = 0; x < Wx; x++) {
+                const size_t i = (size_t)k * Wx * Wy + (size_t)y * Wx + x;
+                h[(size_t)k * per + (size_t)(y + m - Ny) * W + (x + m - Nx)] =
+                    make_float2(krn_r[i], krn_i[i]);
+            }
     std::unique_ptr<litho_bank, int (*)(litho_bank*)> b(new litho_bank(), litho_bank_destroy);
     b->ctx = c;
     b->nk = nk;
-    b->Nx = Nx;
-    b->Ny = Ny;
+    b->Nx = m;
+    b->Ny = m;
+    b->nx0 = Nx;
+    b->ny0 = Ny;
+    b->serial = next_serial.fetch_add(1);
     LITHO_HIP(hipSetDevice(c->device));
     LITHO_TRY(upload_set(c, h, std::vector<float>(scales, scales + nk), per, &b->full));
     std::vector<float2> hp;
@@ -951,72 +982,117 @@ int litho_ctx_set_path(litho_ctx* c, int path) {
     return LITHO_OK;
 }
 
-int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
-                           int N, float* d_image) {
-    if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
+}  // extern "C"
+
+namespace litho {
+
+// Workspaces of litho_socs_image_batch for B clips of N^2 under the context's
+// path choice.  Band pipeline: row-DFT window R, mask window M, intensity
+// band C, FI column output U.  nextPow2 path: R, M, conv-grid intensity T,
+// its row-DFT window P, U.  Every kernel overwrites its workspace in full
+// before the next one reads it, so none has to start at zero.
+int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[5], int* band_path) {
+    if (!bank || B <= 0 || B > 65535) return LITHO_EARG;
     const int Nx = bank->Nx;
-    const int W = 2 * Nx + 1;
-    const int band = 2 * Nx;
-    if ((!is_pow2(N) && N != 768) || N < 32 || N > 4096 || W > N) return LITHO_EARG;
-    hipStream_t st = c->stream;
+    const size_t W = 2 * Nx + 1, band = 2 * Nx;
+    if ((!is_pow2(N) && N != 768) || N < 32 || N > 4096 || (int)W > N) return LITHO_EARG;
     const int Gb = band_grid(Nx);
-    if (c->path == LITHO_PATH_AUTO && Gb && Gb <= N && N >= 256 && N <= 1024) {
-        // band pipeline: 5 launches
-        void *R, *M, *C, *U;
-        LITHO_TRY(ctx_ws(c, 0, (size_t)B * (Nx + 1) * N * sizeof(float2), &R));
-        LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
-        LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
-        LITHO_TRY(ctx_ws(c, 4, (size_t)B * (band + 1) * N * sizeof(float2), &U));
-        static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
-                                      "image_rows"};
-        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
-        // a real mask: kx = 0..Nx of the row spectra, the rest by symmetry
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, 0, Nx + 1, false, false, false, st));
-        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
-        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
-        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
-        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->band.d_bank, nullptr, 0, bank->band.d_scales,
-                                   bank->band.n, Nx, B, (float2*)C, 2, st));
-        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
-        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
-        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
-        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, false, st));
-        if (c->timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
-        c->nstage = 5;
-        for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
-        c->timed = c->timing;
+    if (path == LITHO_PATH_AUTO && Gb && Gb <= N && N >= 256 && N <= 1024) {
+        *band_path = 1;
+        sz[0] = (size_t)B * (Nx + 1) * N * sizeof(float2);
+        sz[1] = (size_t)B * W * W * sizeof(float2);
+        sz[2] = (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2);
+        sz[3] = 0;
+        sz[4] = (size_t)B * (band + 1) * N * sizeof(float2);
         return LITHO_OK;
     }
     const int G = next_pow2(4 * Nx + 1);
     if (!is_pow2(N) || G > N || G > 1024) return LITHO_EARG;
-    void *R, *M, *T, *P, *U;
-    LITHO_TRY(ctx_ws(c, 0, (size_t)B * N * W * sizeof(float2), &R));
-    LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
-    LITHO_TRY(ctx_ws(c, 2, (size_t)B * G * G * sizeof(float), &T));
-    LITHO_TRY(ctx_ws(c, 3, (size_t)B * G * (band + 1) * sizeof(float2), &P));
-    LITHO_TRY(ctx_ws(c, 4, (size_t)B * N * (band + 1) * sizeof(float2), &U));
+    *band_path = 0;
+    sz[0] = (size_t)B * N * W * sizeof(float2);
+    sz[1] = (size_t)B * W * W * sizeof(float2);
+    sz[2] = (size_t)B * G * G * sizeof(float);
+    sz[3] = (size_t)B * G * (band + 1) * sizeof(float2);
+    sz[4] = (size_t)B * N * (band + 1) * sizeof(float2);
+    return LITHO_OK;
+}
+
+// The launches of one batch on the context stream, into the workspaces ws
+// (sized by image_batch_sizes); per-stage events when timing.
+int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B, int N,
+                        float* d_image, int band_path, void* const ws[5], bool timing) {
+    const int Nx = bank->Nx;
+    const int W = 2 * Nx + 1;
+    const int band = 2 * Nx;
+    hipStream_t st = c->stream;
+    c->timed = false;
+    c->nstage = 0;
+    if (band_path) {
+        // band pipeline: 5 launches
+        const int Gb = band_grid(Nx);
+        void *R = ws[0], *M = ws[1], *C = ws[2], *U = ws[4];
+        static const char* names[] = {"mask_rows", "mask_cols", "socs_band", "fi_cols",
+                                      "image_rows"};
+        if (timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
+        // a real mask: kx = 0..Nx of the row spectra, the rest by symmetry
+        LITHO_TRY(launch_rows_dft_t(N, d_mask, R, (long)B * N, 0, Nx + 1, false, false, false, st));
+        if (timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
+        LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
+        if (timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
+        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->band.d_bank, nullptr, 0, bank->band.d_scales,
+                                   bank->band.n, Nx, B, (float2*)C, 2, st));
+        if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
+        if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
+        LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, false, st));
+        if (timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
+        c->nstage = 5;
+        for (int i = 0; i < 5; i++) c->stage_names[i] = names[i];
+        c->timed = timing;
+        return LITHO_OK;
+    }
+    const int G = next_pow2(4 * Nx + 1);
+    void *R = ws[0], *M = ws[1], *T = ws[2], *P = ws[3], *U = ws[4];
     static const char* names[] = {"mask_rows", "mask_cols", "socs_accum", "fi_rows", "fi_cols",
                                   "image_rows"};
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
     LITHO_TRY(launch_rows_dft(N, d_mask, (float2*)R, (long)B * N, -Nx, W, true, c->tw, st));
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
     LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
     // a real mask's window spectrum, conjugate-symmetric to FP32 round-off: the
     // paired set (cross-talk of that order, far below the FP32 image error)
     LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->band.d_bank, bank->band.d_scales, bank->band.n, Nx, B,
                                 (float*)T, 0, c->tw, st));
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
     LITHO_TRY(launch_fi_cols(G, N, (float2*)P, (float2*)U, B, band, 0, c->tw, st));
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
     LITHO_TRY(launch_image_rows(N, (float2*)U, d_image, (long)B * N, band, c->tw, st));
-    if (c->timing) LITHO_HIP(hipEventRecord(c->ev[6], st));
+    if (timing) LITHO_HIP(hipEventRecord(c->ev[6], st));
     c->nstage = 6;
     for (int i = 0; i < 6; i++) c->stage_names[i] = names[i];
-    c->timed = c->timing;
+    c->timed = timing;
     return LITHO_OK;
+}
+
+int ctx_path(litho_ctx* c) { return c->path; }
+
+}  // namespace litho
+
+extern "C" {
+
+int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
+                           int N, float* d_image) {
+    if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
+    size_t sz[5];
+    int band_path = 0;
+    LITHO_TRY(image_batch_sizes(c->path, bank, B, N, sz, &band_path));
+    void* ws[5] = {};
+    for (int i = 0; i < 5; i++)
+        if (sz[i]) LITHO_TRY(ctx_ws(c, i, sz[i], &ws[i]));
+    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, c->timing);
 }
 
 // fp16 DFT table of the matrix-core kernels: kind 0 mask rows (N, kmin, count),
@@ -1285,7 +1361,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         return LITHO_OK;
     };
     if (!use_graph) return enqueue();
-    std::vector<long> key = {1, (long)bank, (long)d_mask, (long)d_image, B, N, precision, chunks,
+    std::vector<long> key = {1, bank->serial, (long)d_mask, (long)d_image, B, N, precision, chunks,
                              (long)w[0].R, (long)w[0].U, (long)w[0].M, (long)w[0].C, tuned.G,
                              tuned.socs_shape, tuned.lean_mem};
     return run_graph(c, key, enqueue);
@@ -1377,7 +1453,7 @@ int litho_socs_image_buckets(litho_ctx* c, const LithoBucket* bk, int nb, int pr
     std::vector<long> key = {2, precision, nb};
     for (int i = 0; i < nb; i++) {
         const LithoBucket& b = bk[i];
-        for (long v : {(long)b.bank, (long)b.N, (long)b.count, (long)b.d_masks, (long)b.d_images,
+        for (long v : {b.bank->serial, (long)b.N, (long)b.count, (long)b.d_masks, (long)b.d_images,
                        (long)w[i].R, (long)w[i].U, (long)w[i].M, (long)w[i].C, (long)plans[i].G,
                        (long)plans[i].socs_shape, (long)plans[i].lean_mem})
             key.push_back(v);
@@ -1509,42 +1585,134 @@ int litho_socs_image(const float* mask, int N, const float* krn_r, const float* 
     return LITHO_OK;
 }
 
-int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn_r,
-                      const float* krn_i, const float* scales, float* tmpImg, float* output,
-                      int nk, int nx, int ny, int Lx, int Ly) {
-    if (!mskf_r || !mskf_i || !krn_r || !krn_i || !scales || nk <= 0 || nx < 0 || ny != nx ||
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Accelerator-shaped SOCS (the v18 top): tmpImgp, the fftshifted intensity
+// on a Gy x Gx conv grid, before FI.
+// ---------------------------------------------------------------------------
+namespace {
+
+// the v18 top's fixed 128-point grid (v18 .h L34-35, good for n <= 8, its
+// MAX_KERNEL_SIZE = 17 at L43), grown to the reference's nextPow2(4n+1)
+// (litho_thread.cpp L833-834) where a wider window needs more
+int tmpimg_grid(int n) { return std::max(128, next_pow2(4 * n + 1)); }
+
+// square grid the SOCS loop runs on for a window of half-width m
+int tmpimg_square(int m, int Gx, int Gy) {
+    return std::max(std::max(Gx, Gy), next_pow2(4 * m + 1));
+}
+
+bool tmpimg_grid_ok(int nx, int ny, int m, int Gx, int Gy) {
+    return is_pow2(Gx) && is_pow2(Gy) && Gx >= 2 * nx + 1 && Gy >= 2 * ny + 1 &&
+           tmpimg_square(m, Gx, Gy) <= 1024;
+}
+
+// bytes of the unshifted square-grid intensity, needed unless the output grid
+// is the square grid on both axes
+size_t tmpimg_scratch_bytes(int m, int Gx, int Gy) {
+    const int Gs = tmpimg_square(m, Gx, Gy);
+    return (Gx == Gs && Gy == Gs) ? 0 : (size_t)Gs * Gs * sizeof(float);
+}
+
+// window M (W x W, W = 2m+1) x kernels as given -> tmpImgp (Gy x Gx) in o1
+// and o2 (either may be null, they may alias).  The loop runs on the square
+// grid Gs >= 4m+1; a coarser output axis samples it (the field of a window
+// that fits the coarser grid takes the same values at the common points).
+int tmpimg_enqueue(litho_ctx* c, const KernelSet& ks, int m, const float2* dM, int Gx, int Gy,
+                   float* T, float* o1, float* o2, hipStream_t st) {
+    const int Gs = tmpimg_square(m, Gx, Gy);
+    if (o1 == o2) o2 = nullptr;
+    if (Gx == Gs && Gy == Gs) {
+        float* dst = o1 ? o1 : o2;
+        if (!dst) return LITHO_OK;
+        LITHO_TRY(launch_socs_accum(Gs, dM, ks.d_bank, ks.d_scales, ks.n, m, 1, dst, 1, c->tw, st));
+        if (o1 && o2)
+            LITHO_HIP(hipMemcpyAsync(o2, o1, (size_t)Gs * Gs * sizeof(float),
+                                     hipMemcpyDeviceToDevice, st));
+        return LITHO_OK;
+    }
+    if (!T) return LITHO_EARG;
+    LITHO_TRY(launch_socs_accum(Gs, dM, ks.d_bank, ks.d_scales, ks.n, m, 1, T, 0, c->tw, st));
+    return launch_grid_sample_shift(T, Gs, Gx, Gy, o1, o2, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int litho_tmpimg_grid(int n) {
+    if (n < 0 || n > 255) return LITHO_EARG;
+    return tmpimg_grid(n);
+}
+
+int litho_socs_tmpimg_on_grid(const float* mskf_r, const float* mskf_i, const float* krn_r,
+                              const float* krn_i, const float* scales, float* tmpImg,
+                              float* output, int nk, int nx, int ny, int Lx, int Ly, int Gx,
+                              int Gy) {
+    if (!mskf_r || !mskf_i || !krn_r || !krn_i || !scales || nk <= 0 || nx < 0 || ny < 0 ||
         Lx <= 0 || Ly <= 0)
         return LITHO_EARG;
-    const int W = 2 * nx + 1;
-    const int G = next_pow2(4 * nx + 1);
-    if (G > 1024) return LITHO_EARG;
+    const int m = std::max(nx, ny);
+    if (!tmpimg_grid_ok(nx, ny, m, Gx, Gy)) return LITHO_EARG;
+    const int W = 2 * m + 1;
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
     hipStream_t st = c->stream;
-    // window of the centred spectrum (the embed's bounds check, L909-936)
+    // window of the centred spectrum (the embed's bounds check, v18 .cpp
+    // L114-116, litho_thread.cpp L909-936); the padded kernels are zero
+    // outside their own (2ny+1) x (2nx+1) window
     std::vector<float2> win((size_t)W * W, make_float2(0.f, 0.f));
-    for (int y = -ny; y <= ny; y++)
-        for (int x = -nx; x <= nx; x++) {
+    for (int y = -m; y <= m; y++)
+        for (int x = -m; x <= m; x++) {
             int my = Ly / 2 + y, mx = Lx / 2 + x;
             if (my >= 0 && my < Ly && mx >= 0 && mx < Lx)
-                win[(size_t)(y + ny) * W + x + nx] =
+                win[(size_t)(y + m) * W + x + m] =
                     make_float2(mskf_r[(size_t)my * Lx + mx], mskf_i[(size_t)my * Lx + mx]);
         }
     litho_bank* bank = nullptr;
     LITHO_TRY(litho_bank_create(c, krn_r, krn_i, scales, nk, nx, ny, &bank));
     std::unique_ptr<litho_bank, int (*)(litho_bank*)> guard(bank, litho_bank_destroy);
-    Dev dM, dT;
+    Dev dM, dT, dO;
     LITHO_TRY(dM.alloc(win.size() * sizeof(float2)));
-    LITHO_TRY(dT.alloc((size_t)G * G * sizeof(float)));
+    const size_t tb = tmpimg_scratch_bytes(m, Gx, Gy);
+    if (tb) LITHO_TRY(dT.alloc(tb));
+    const size_t ob = (size_t)Gx * Gy * sizeof(float);
+    LITHO_TRY(dO.alloc(ob));
     LITHO_HIP(hipMemcpyAsync(dM.p, win.data(), win.size() * sizeof(float2), hipMemcpyHostToDevice, st));
-    LITHO_TRY(launch_socs_accum(G, dM.as<float2>(), bank->full.d_bank, bank->full.d_scales, nk, nx, 1,
-                                dT.as<float>(), 1, c->tw, st));
-    std::vector<float> h((size_t)G * G);
-    LITHO_HIP(hipMemcpyAsync(h.data(), dT.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_TRY(tmpimg_enqueue(c, bank->full, m, dM.as<float2>(), Gx, Gy, dT.as<float>(),
+                             dO.as<float>(), nullptr, st));
+    std::vector<float> h((size_t)Gx * Gy);
+    LITHO_HIP(hipMemcpyAsync(h.data(), dO.p, ob, hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
-    if (tmpImg) std::memcpy(tmpImg, h.data(), h.size() * sizeof(float));
-    if (output) std::memcpy(output, h.data(), h.size() * sizeof(float));
+    if (tmpImg) std::memcpy(tmpImg, h.data(), ob);
+    if (output && output != tmpImg) std::memcpy(output, h.data(), ob);
     return LITHO_OK;
+}
+
+int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn_r,
+                      const float* krn_i, const float* scales, float* tmpImg, float* output,
+                      int nk, int nx, int ny, int Lx, int Ly) {
+    if (nx < 0 || ny < 0 || nx > 255 || ny > 255) return LITHO_EARG;
+    return litho_socs_tmpimg_on_grid(mskf_r, mskf_i, krn_r, krn_i, scales, tmpImg, output, nk, nx,
+                                     ny, Lx, Ly, tmpimg_grid(nx), tmpimg_grid(ny));
+}
+
+int litho_socs_tmpimg_dev(litho_ctx* c, const litho_bank* bank, const float* d_mskf_r,
+                          const float* d_mskf_i, int Lx, int Ly, int Gx, int Gy, float* d_tmpImg,
+                          float* d_output) {
+    if (!c || !bank || !d_mskf_r || !d_mskf_i || Lx <= 0 || Ly <= 0 || (!d_tmpImg && !d_output))
+        return LITHO_EARG;
+    const int m = bank->Nx;
+    if (!tmpimg_grid_ok(bank->nx0, bank->ny0, m, Gx, Gy)) return LITHO_EARG;
+    hipStream_t st = c->stream;
+    void *M, *T = nullptr;
+    LITHO_TRY(ctx_ws(c, 5, (size_t)(2 * m + 1) * (2 * m + 1) * sizeof(float2), &M));
+    const size_t tb = tmpimg_scratch_bytes(m, Gx, Gy);
+    if (tb) LITHO_TRY(ctx_ws(c, 6, tb, &T));
+    LITHO_TRY(launch_spectrum_window(d_mskf_r, d_mskf_i, Lx, Ly, m, (float2*)M, st));
+    return tmpimg_enqueue(c, bank->full, m, (const float2*)M, Gx, Gy, (float*)T, d_tmpImg,
+                          d_output, st);
 }
 
 int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double* tcc_im, int Nx,

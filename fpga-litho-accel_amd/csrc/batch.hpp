@@ -1,0 +1,18 @@
+// Internal interface of the mask -> image batch (api.cpp).
+#pragma once
+
+#include "common.hpp"
+
+namespace litho {
+
+// workspace bytes of litho_socs_image_batch (five slots, unused ones 0) and
+// whether the band pipeline runs; LITHO_EARG for a shape it does not take
+int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[5],
+                      int* band_path);
+// the batch's launches on the context stream into the given workspaces
+int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B, int N,
+                        float* d_image, int band_path, void* const ws[5], bool timing);
+int ctx_path(litho_ctx* c);
+int ctx_device(litho_ctx* c);
+
+}  // namespace litho

@@ -2269,4 +2269,64 @@ int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, b
     return LITHO_EARG;
 }
 
+// ---------------------------------------------------------------------------
+// tmpImgp helpers of the accelerator-shaped entry (v18 top, socs_2048_stream_
+// refactored_v18_hls_fft.cpp L58-131, L380-387).
+//
+// spectrum_window: M[(y+m)*W + x+m] = mskf[Ly/2+y][Lx/2+x] for |x|, |y| <= m
+// (W = 2m+1), zero where the index leaves the Lx x Ly planes (the embed's
+// bounds check, v18 .cpp L114-116).  One thread per window entry.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void spectrum_window_kernel(const float* __restrict__ re,
+                                                              const float* __restrict__ im,
+                                                              int Lx, int Ly, int m,
+                                                              float2* __restrict__ M) {
+    const int W = 2 * m + 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= W * W) return;
+    const int y = i / W - m, x = i % W - m;
+    const int my = Ly / 2 + y, mx = Lx / 2 + x;
+    float2 v = make_float2(0.f, 0.f);
+    if (my >= 0 && my < Ly && mx >= 0 && mx < Lx) {
+        const long j = (long)my * Lx + mx;
+        v = make_float2(re[j], im[j]);
+    }
+    M[i] = v;
+}
+
+// grid_sample_shift: T is the unshifted intensity on a G x G grid; the Gy x Gx
+// grid (G / Gx, G / Gy integers) samples the same band-limited intensity at
+// every (G / Gx)-th column and (G / Gy)-th row.  Writes it fftshifted
+// (v18 .cpp L199-216: out[y][x] = in[(y + Gy/2) % Gy][(x + Gx/2) % Gx]) to
+// o1 and o2 (either may be null).
+__global__ __launch_bounds__(256) void grid_sample_shift_kernel(const float* __restrict__ T,
+                                                                int G, int Gx, int Gy,
+                                                                float* __restrict__ o1,
+                                                                float* __restrict__ o2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Gx * Gy) return;
+    const int y = i / Gx, x = i - y * Gx;
+    const int sy = (y + Gy / 2) % Gy, sx = (x + Gx / 2) % Gx;
+    const float v = T[(long)sy * (G / Gy) * G + (long)sx * (G / Gx)];
+    if (o1) o1[i] = v;
+    if (o2) o2[i] = v;
+}
+
+int launch_spectrum_window(const float* re, const float* im, int Lx, int Ly, int m, float2* M,
+                           hipStream_t st) {
+    if (!re || !im || !M || Lx <= 0 || Ly <= 0 || m < 0) return LITHO_EARG;
+    const int W = 2 * m + 1;
+    hipLaunchKernelGGL(spectrum_window_kernel, dim3(cdiv((size_t)W * W, 256)), dim3(256), 0, st,
+                       re, im, Lx, Ly, m, M);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_grid_sample_shift(const float* T, int G, int Gx, int Gy, float* o1, float* o2,
+                             hipStream_t st) {
+    if (!T || Gx <= 0 || Gy <= 0 || G % Gx || G % Gy) return LITHO_EARG;
+    hipLaunchKernelGGL(grid_sample_shift_kernel, dim3(cdiv((size_t)Gx * Gy, 256)), dim3(256), 0,
+                       st, T, G, Gx, Gy, o1, o2);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 }  // namespace litho

@@ -61,4 +61,13 @@ int launch_fi_cols_split(int N, const float2* C, void* U, int B, int band, int n
 int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, bool h16,
                         bool lean, bool out16, hipStream_t st);
 
+// window [-m, m]^2 of a centred spectrum given as Lx x Ly planes -> M [W][W]
+// float2 (zero outside the planes)
+int launch_spectrum_window(const float* re, const float* im, int Lx, int Ly, int m, float2* M,
+                           hipStream_t st);
+// unshifted G x G intensity -> fftshifted Gy x Gx samples (G / Gx, G / Gy
+// integers) written to o1 and o2 (either may be null)
+int launch_grid_sample_shift(const float* T, int G, int Gx, int Gy, float* o1, float* o2,
+                             hipStream_t st);
+
 }  // namespace litho

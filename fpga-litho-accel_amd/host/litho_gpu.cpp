@@ -439,8 +439,8 @@ int main(int argc, char** argv) {
     {
         Timer t;
         std::vector<float> tmp((size_t)G * G);
-        CHECK(litho_socs_tmpimg(mr.data(), mi.data(), kr.data(), ki.data(), sc.data(), tmp.data(),
-                                nullptr, nk, Nx, Nx, N, N));
+        CHECK(litho_socs_tmpimg_on_grid(mr.data(), mi.data(), kr.data(), ki.data(), sc.data(),
+                                        tmp.data(), nullptr, nk, Nx, Nx, N, N, G, G));
         write_bin(out / ("tmpImgp_full_" + std::to_string(G) + ".bin"), tmp.data(), tmp.size());
         std::vector<float> img((size_t)N * N);
         CHECK(litho_socs_image(mask.data(), N, kr.data(), ki.data(), sc.data(), nk, Nx, Nx,

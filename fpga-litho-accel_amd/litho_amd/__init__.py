@@ -30,7 +30,8 @@ EXPORTS = [
     "litho_memcpy_d2h", "litho_memset_dev", "litho_memcpy_d2d", "litho_source_annular", "litho_source_quadrupole",
     "litho_source_dipole", "litho_source_point", "litho_outer_sigma", "litho_compute_nx",
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
-    "litho_kernels_from_source", "litho_socs_tmpimg", "litho_socs_image", "litho_fi",
+    "litho_kernels_from_source", "litho_tmpimg_grid", "litho_socs_tmpimg",
+    "litho_socs_tmpimg_on_grid", "litho_socs_tmpimg_dev", "litho_socs_image", "litho_fi",
     "litho_image_tcc",
     "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
     "litho_ctx_set_path", "litho_ctx_set_timing",
@@ -132,8 +133,12 @@ def lib():
         L.litho_tcc_build_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, _f64p, _f64p]
         L.litho_kernels_from_source_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, i,
                                                      _f64p, _f32p, _f32p, C.POINTER(i)]
+        L.litho_tmpimg_grid.argtypes = [i]
         L.litho_socs_tmpimg.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                         i, i, i, i, i]
+        L.litho_socs_tmpimg_on_grid.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                                _f32p, i, i, i, i, i, i, i]
+        L.litho_socs_tmpimg_dev.argtypes = [vp, vp, vp, vp, i, i, i, i, vp, vp]
         L.litho_socs_image.argtypes = [_f32p, i, _f32p, _f32p, _f32p, i, i, i, _f32p]
         L.litho_fi.argtypes = [_f32p, i, i, i, _f32p]
         L.litho_image_tcc.argtypes = [_f32p, i, _f64p, _f64p, i, i, _f32p]
@@ -266,21 +271,44 @@ def kernels_from_source(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny
 
 
 # ---- Module 7 ---------------------------------------------------------------
-def calc_socs_hls(mskf, krns, scales, nx, ny):
-    """HLS-v18-shaped SOCS (v18 .cpp L287-388): fftshifted G x G tmpImgp."""
+def tmpimg_grid(n):
+    """Conv grid of the accelerator-shaped entry along an axis of window
+    half-width n: v18's fixed 128 (v18 .h L34-35) up to n = 8, at least
+    nextPow2(4n+1) above (litho_thread.cpp L833-834)."""
+    g = lib().litho_tmpimg_grid(int(n))
+    _check(min(g, 0), "litho_tmpimg_grid")
+    return g
+
+
+def conv_grid(n):
+    """calcSOCS's conv grid nextPow2(4n+1) (litho_thread.cpp L833-834)."""
+    g = 1
+    while g < 4 * n + 1:
+        g *= 2
+    return g
+
+
+def calc_socs_hls(mskf, krns, scales, nx, ny, grid=None):
+    """HLS-v18-shaped SOCS (v18 .cpp L287-388): the fftshifted tmpImgp,
+    Gy x Gx = tmpimg_grid(ny) x tmpimg_grid(nx) (128 x 128 for nx, ny <= 8,
+    as v18), or on grid = (Gy, Gx) when given."""
     mskf = np.asarray(mskf)
     Ly, Lx = mskf.shape
     mr, mi = _planes(mskf)
     kr, ki = _planes(krns)
     sc = np.ascontiguousarray(scales, np.float32)
     nk = sc.size
-    G = 1
-    while G < 4 * nx + 1:
-        G *= 2
-    out = np.zeros(G * G, np.float32)
-    _check(lib().litho_socs_tmpimg(mr, mi, kr, ki, sc, out, out, nk, nx, ny, Lx, Ly),
-           "litho_socs_tmpimg")
-    return out.reshape(G, G)
+    if grid is None:
+        Gy, Gx = tmpimg_grid(ny), tmpimg_grid(nx)
+        out = np.zeros(Gx * Gy, np.float32)
+        _check(lib().litho_socs_tmpimg(mr, mi, kr, ki, sc, out, out, nk, nx, ny, Lx, Ly),
+               "litho_socs_tmpimg")
+    else:
+        Gy, Gx = grid
+        out = np.zeros(Gx * Gy, np.float32)
+        _check(lib().litho_socs_tmpimg_on_grid(mr, mi, kr, ki, sc, out, out, nk, nx, ny, Lx, Ly,
+                                               Gx, Gy), "litho_socs_tmpimg_on_grid")
+    return out.reshape(Gy, Gx)
 
 
 def FI(tmpImgp, N, band):
@@ -293,9 +321,11 @@ def FI(tmpImgp, N, band):
 
 
 def calcSOCS(mskf, krns, scales, Nx, Ny):
-    """calcSOCS (L813-1040) from a full centred spectrum: SOCS loop + FI."""
+    """calcSOCS (L813-1040) from a full centred spectrum: SOCS loop on the
+    nextPow2(4Nx+1) grid (L833) + FI."""
     mskf = np.asarray(mskf)
-    t = calc_socs_hls(mskf, krns, scales, Nx, Ny)
+    G = conv_grid(Nx)
+    t = calc_socs_hls(mskf, krns, scales, Nx, Ny, grid=(G, G))
     return FI(t, mskf.shape[0], 2 * Nx)
 
 

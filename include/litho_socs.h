@@ -127,14 +127,39 @@ int litho_kernels_from_source_prec(const LithoOptics* op, const float* src, int 
                                    int* flags);
 
 /* ---- Module 7: calcSOCS, litho_thread.cpp L813-1040 ------------------------ */
-/* HLS-v18-shaped entry (socs_2048_stream_refactored_v18_hls_fft.cpp L287-388):
- * full Lx x Ly spectrum planes, kernels nk x (2ny+1)(2nx+1) concatenated,
- * scales[nk]; writes the fftshifted G x G intensity tmpImgp (G =
- * nextPow2(4n+1) per axis, litho_thread.cpp L833-834) to tmpImg and to
- * output (either may be NULL).  Lx/Ly here are the spectrum sizes in pixels. */
+/* Conv grid of the accelerator-shaped entry along an axis of window
+ * half-width n: the v18 top's fixed 128 (socs_2048_stream_refactored_v18_hls_fft.h
+ * L34-35) for every n it accepts (n <= 8, MAX_KERNEL_SIZE = 17 at L43), and
+ * above that max(128, nextPow2(4n+1)) (litho_thread.cpp L833-834), where v18
+ * itself would drop the kernel rows and columns beyond 17.  n <= 255. */
+int litho_tmpimg_grid(int n);
+/* Drop-in for the HLS top calc_socs_2048_hls_stream_refactored_v18
+ * (socs_2048_stream_refactored_v18_hls_fft.h L86-97, .cpp L287-388), same
+ * pointer order and meaning: full Lx x Ly centred spectrum planes, kernels
+ * nk x (2ny+1)(2nx+1) concatenated ([ky][kx], row length 2nx+1), scales[nk];
+ * writes the fftshifted intensity tmpImgp (before FI) on the Gy x Gx grid,
+ * Gx = litho_tmpimg_grid(nx), Gy = litho_tmpimg_grid(ny) -- 128 x 128 for
+ * nx, ny <= 8, as v18 -- to tmpImg and to output (either may be NULL, they
+ * may alias).  Lx/Ly are the spectrum sizes in pixels.  nx != ny is accepted.
+ * FP32 arithmetic (v18 runs a block-floating-point IFFT). */
 int litho_socs_tmpimg(const float* mskf_r, const float* mskf_i, const float* krn_r,
                       const float* krn_i, const float* scales, float* tmpImg, float* output,
                       int nk, int nx, int ny, int Lx, int Ly);
+/* The same on a caller-chosen grid: Gx, Gy powers of two, Gx >= 2nx+1,
+ * Gy >= 2ny+1 (the intensity sampled there; litho_thread.cpp's calcSOCS uses
+ * nextPow2(4n+1) per axis, L833-834, which FI needs to be alias-free). */
+int litho_socs_tmpimg_on_grid(const float* mskf_r, const float* mskf_i, const float* krn_r,
+                              const float* krn_i, const float* scales, float* tmpImg,
+                              float* output, int nk, int nx, int ny, int Lx, int Ly, int Gx,
+                              int Gy);
+/* Device-resident form for repeated calls (no counterpart in the reference:
+ * the v18 testbench re-sends the kernels with every call): kernels uploaded
+ * once as a bank by litho_bank_create with nx, ny, spectrum planes and
+ * outputs in device memory, asynchronous on the context stream.  Gx, Gy as
+ * litho_socs_tmpimg_on_grid (litho_tmpimg_grid gives the v18 grid). */
+int litho_socs_tmpimg_dev(litho_ctx* ctx, const litho_bank* bank, const float* d_mskf_r,
+                          const float* d_mskf_i, int Lx, int Ly, int Gx, int Gy, float* d_tmpImg,
+                          float* d_output);
 /* Mask -> aerial image: mask spectrum, SOCS loop and FI (Modules 3 + 7). */
 int litho_socs_image(const float* mask, int N, const float* krn_r, const float* krn_i,
                      const float* scales, int nk, int Nx, int Ny, float* image);
@@ -150,7 +175,9 @@ int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double
                     int Ny, float* image);
 
 /* ---- device-resident batched path (the bench path) ------------------------- */
-/* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin).  The
+/* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin, each
+ * (2Ny+1) x (2Nx+1); Nx != Ny is stored zero-padded to the square window of
+ * half-width max(Nx, Ny), which leaves every product and field unchanged).  The
  * mask -> image paths below run a second form of the set: a kernel K with
  * K(-f) = e^{ib} conj(K(f)) about the window centre (zero defocus with a
  * point-symmetric source gives such kernels) has a real field on a real

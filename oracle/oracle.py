@@ -232,13 +232,17 @@ def conv_grid(Nx):
     return lib().orc_conv_grid(Nx)
 
 
-def socs_tmpimg(mskf, krn, scales, Nx, Ny, nthreads=1):
-    """calcSOCS accumulate + fftshift (L897-970): tmpImgp on the G grid (float64)."""
+def socs_tmpimg(mskf, krn, scales, Nx, Ny, nthreads=1, grid=None):
+    """calcSOCS accumulate + fftshift (L897-970): tmpImgp (float64) on the
+    nextPow2(4N+1) grid of L833-834, or on grid = (Gy, Gx) when given (the
+    v18 top's fixed 128 x 128, socs_2048_stream_refactored_v18_hls_fft.h L34-35)."""
     mskf = np.asarray(mskf, np.complex128)
     fy, fx = mskf.shape
     krn = np.asarray(krn, np.complex128)
     nk = krn.shape[0]
     Gx, Gy = conv_grid(Nx), conv_grid(Ny)
+    if grid is not None:
+        Gy, Gx = grid
     out = np.zeros(Gx * Gy)
     _check(lib().orc_socs_tmpimg(out, Gx, Gy, np.ascontiguousarray(mskf.real).ravel(),
                                  np.ascontiguousarray(mskf.imag).ravel(), fx, fy,

@@ -44,7 +44,29 @@ def test_arguments_rejected_before_device_work(gpu):
     assert L.litho_kernels_topk(t, t, 81, 4, 4, 100, sc, k, k, None) == -1  # nk > n
     assert L.litho_kernels_topk(t, t, 80, 4, 4, 5, sc, k, k, None) == -1  # n != W^2
     assert L.litho_fi(m, 48, 100, 3, m) == -1
+    tn = np.eye(9).ravel()
+    tn[10] = np.nan  # a non-finite TCC is refused before the eigensolver
+    assert L.litho_kernels_topk(tn, np.zeros(81), 9, 1, 1, 2, sc, k, k, None) == -1
     assert L.litho_socs_image_batch(None, None, None, 1, 512, None) == -1
+
+
+def test_tmpimg_grid_and_arguments(gpu):
+    """v18's fixed 128 grid up to n = 8 (v18 .h L34-35, L43), at least
+    nextPow2(4n+1) above (litho_thread.cpp L833-834); grids that cannot hold
+    the window are rejected before any device work."""
+    L = gpu.lib()
+    assert [L.litho_tmpimg_grid(n) for n in (0, 8, 17, 31, 32, 34, 137, 255)] == \
+        [128, 128, 128, 128, 256, 256, 1024, 1024]
+    assert L.litho_tmpimg_grid(-1) == -1 and L.litho_tmpimg_grid(256) == -1
+    x = np.zeros(17 * 17 * 4, np.float32)
+    sc = np.ones(1, np.float32)
+    args = (x, x, x, x, sc, x, x, 1, 8, 5, 32, 32)
+    assert L.litho_socs_tmpimg_on_grid(*args, 16, 128) == -1    # 16 < 2*8+1
+    assert L.litho_socs_tmpimg_on_grid(*args, 128, 8) == -1     # 8 < 2*5+1
+    assert L.litho_socs_tmpimg_on_grid(*args, 96, 128) == -1    # not a power of two
+    assert L.litho_socs_tmpimg_on_grid(*args, 2048, 128) == -1  # beyond 1024
+    assert L.litho_socs_tmpimg(x, x, x, x, sc, x, x, 1, 300, 5, 32, 32) == -1
+    assert L.litho_socs_tmpimg_dev(None, None, None, None, 32, 32, 128, 128, None, None) == -1
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")

@@ -110,7 +110,9 @@ def test_socs_tmpimg_hls_entry(gpu, orc, clip512):
     m = gpu.manhattan_mask(clip512["N"], 4)
     mf = orc.mask_spectrum(m)
     t = gpu.calc_socs_hls(mf, clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
-    to = orc.socs_tmpimg(mf, clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"], clip512["Nx"])
+    # Nx = 17 is above v18's MAX_KERNEL_SIZE range: grid max(128, nextPow2(69)) = 128
+    to = orc.socs_tmpimg(mf, clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"],
+                         clip512["Nx"], grid=(128, 128))
     assert t.shape == (128, 128)
     assert max_rel(t, to) < IMG_TOL
 
