@@ -144,6 +144,7 @@ const char* litho_status_string(int status) {
         case LITHO_ENOCONV: return "eigensolver did not converge";
         case LITHO_ENODEV: return "no usable GPU";
         case LITHO_ERANGE: return "pupil support leaves the kernel window";
+        case LITHO_ESCRATCH: return "batch exceeds the no-allocation scratch (LITHO_NOALLOC_SCRATCH_BYTES)";
         default: return "unknown status";
     }
 }
@@ -168,6 +169,9 @@ int litho_ctx_create(litho_ctx** out, int device) {
     LITHO_HIP(hipMalloc(&c->tw, sizeof(float2) * TW_MAX));
     LITHO_HIP(hipMemcpy(c->tw, tw.data(), sizeof(float2) * TW_MAX, hipMemcpyHostToDevice));
     for (int i = 0; i <= NSTAGE; i++) LITHO_HIP(hipEventCreate(&c->ev[i]));
+    // the no-allocation entry's scratch on this device; a failed lookup is
+    // kept and reported by that entry point alone
+    (void)noalloc_init(device);
     *out = c.release();
     return LITHO_OK;
 }

@@ -1,4 +1,5 @@
-// Internal interface of the mask -> image batch (api.cpp).
+// Internal interface of the mask -> image batch (api.cpp) shared with the
+// no-allocation entry point (noalloc.hip).
 #pragma once
 
 #include "common.hpp"
@@ -14,5 +15,8 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
                         float* d_image, int band_path, void* const ws[5], bool timing);
 int ctx_path(litho_ctx* c);
 int ctx_device(litho_ctx* c);
+// resolves this device's copy of the no-allocation scratch array (noalloc.hip);
+// called by litho_ctx_create with the device current
+int noalloc_init(int device);
 
 }  // namespace litho

@@ -34,6 +34,7 @@ EXPORTS = [
     "litho_socs_tmpimg_on_grid", "litho_socs_tmpimg_dev", "litho_socs_image", "litho_fi",
     "litho_image_tcc",
     "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
+    "litho_socs_image_batch_noalloc",
     "litho_ctx_set_path", "litho_ctx_set_timing",
     "litho_last_timing", "litho_stage_name", "litho_socs_image_batch_pipelined",
     "litho_tcc_build_prec", "litho_kernels_from_source_prec",
@@ -56,6 +57,8 @@ LITHO_PREC_COMPACT = (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_
                       | LITHO_PREC_DFT_MFMA | LITHO_PREC_IMAGE_FP16)
 LITHO_PATH_AUTO = 0
 LITHO_PATH_POW2 = 1
+LITHO_ESCRATCH = -7
+LITHO_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class LithoError(RuntimeError):
@@ -146,6 +149,7 @@ def lib():
         L.litho_bank_destroy.argtypes = [vp]
         L.litho_bank_info.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
         L.litho_socs_image_batch.argtypes = [vp, vp, vp, i, i, vp]
+        L.litho_socs_image_batch_noalloc.argtypes = [vp, vp, vp, i, i, vp]
         L.litho_last_timing.argtypes = [vp, C.POINTER(f), C.POINTER(f), i]
         L.litho_ctx_set_path.argtypes = [vp, i]
         L.litho_ctx_set_timing.argtypes = [vp, i]
@@ -402,6 +406,12 @@ class Context:
     def socs_image_batch(self, bank, d_mask, B, N, d_image):
         _check(lib().litho_socs_image_batch(self.h, bank, d_mask, B, N, d_image),
                "litho_socs_image_batch")
+
+    def socs_image_batch_noalloc(self, bank, d_mask, B, N, d_image):
+        """socs_image_batch with its workspaces in the library's fixed
+        per-GPU scratch (LITHO_NOALLOC_SCRATCH_BYTES): allocates nothing."""
+        _check(lib().litho_socs_image_batch_noalloc(self.h, bank, d_mask, B, N, d_image),
+               "litho_socs_image_batch_noalloc")
 
     def socs_image_batch_pipelined(self, bank, d_mask, B, N, d_image, precision=5, chunks=1,
                                    graph=True):

@@ -36,6 +36,7 @@ extern "C" {
 #define LITHO_ENOCONV -4   /* eigensolver did not converge */
 #define LITHO_ENODEV -5    /* no usable GPU */
 #define LITHO_ERANGE -6    /* pupil support leaves the (2Nx+1)x(2Ny+1) window */
+#define LITHO_ESCRATCH -7  /* request exceeds the no-allocation scratch */
 
 /* flag bits returned through `int* flags` */
 #define LITHO_FLAG_CLUSTER_SPLIT 1 /* nk cuts a degenerate eigenvalue cluster */
@@ -194,6 +195,19 @@ int litho_bank_info(const litho_bank* bank, int* entries, int* pairs);
  * context stream.  Workspaces are grown by the context as needed. */
 int litho_socs_image_batch(litho_ctx* ctx, const litho_bank* bank, const float* d_mask, int B,
                            int N, float* d_image);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  litho_socs_image_batch with its workspaces in a fixed
+ * device array of LITHO_NOALLOC_SCRATCH_BYTES (64 MiB) that the library holds
+ * once per GPU until the process ends: same launches, same images bit for
+ * bit, nothing allocated or freed.  LITHO_ESCRATCH when the batch's
+ * workspaces do not fit (about 0.92 MiB per clip at 1024^2, Nx = 34, so
+ * B <= 69; 60 KiB per clip at 256^2, Nx = 8).  All calls on one GPU share
+ * the array: calls that overlap in time on one GPU (two contexts or streams)
+ * race and must be serialised by the caller.  Records no per-stage timing. */
+#define LITHO_NOALLOC_SCRATCH_BYTES (64u << 20)
+int litho_socs_image_batch_noalloc(litho_ctx* ctx, const litho_bank* bank, const float* d_mask,
+                                   int B, int N, float* d_image);
 /* Conv-grid choice of the image path.  AUTO: smallest square grid of
  * 2^a 3^b points (36, 81, 144) >= 4Nx+1 with the intensity's band spectrum
  * taken inside the SOCS kernel, else the reference's nextPow2(4Nx+1) grid

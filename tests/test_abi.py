@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(gpu):
 def test_status_strings(gpu):
     L = gpu.lib()
     assert L.litho_status_string(0) == b"ok"
-    for code in (-1, -2, -4, -5, -6):
+    for code in (-1, -2, -4, -5, -6, -7):
         assert L.litho_status_string(code) != b"unknown status"
     assert L.litho_version() >= 100
 
@@ -67,6 +67,7 @@ def test_tmpimg_grid_and_arguments(gpu):
     assert L.litho_socs_tmpimg_on_grid(*args, 2048, 128) == -1  # beyond 1024
     assert L.litho_socs_tmpimg(x, x, x, x, sc, x, x, 1, 300, 5, 32, 32) == -1
     assert L.litho_socs_tmpimg_dev(None, None, None, None, 32, 32, 128, 128, None, None) == -1
+    assert L.litho_socs_image_batch_noalloc(None, None, None, 1, 512, None) == -1
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
