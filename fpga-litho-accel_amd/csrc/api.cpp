@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -26,6 +27,7 @@
 #include "fft_lds.hpp"
 #include "socs_kernels.hpp"
 #include "tcc_kernels.hpp"
+#include "topk_kernels.hpp"
 
 namespace litho {
 
@@ -602,11 +604,185 @@ int finish_kernels(litho_ctx* c, double2* Vt, int n, int mtop, int nk, int Nx, i
     return LITHO_OK;
 }
 
+// Setup timing of the calling thread's last Module 4 / 6 call (host wall
+// clock, ms): pupil, contraction, eigensolve; solver (0 Jacobi, 1 subspace);
+// subspace iterations.
+struct SetupTiming {
+    double t[5] = {0, 0, 0, 0, 0};
+};
+thread_local SetupTiming g_setup;
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// Top eigenpairs of device H (n x n, upper triangle valid) by block subspace
+// iteration (topk_kernels.hip).  Returns LITHO_OK with the kernels written,
+// 1 when the problem does not suit the block (H not PSD to working accuracy,
+// a cluster that does not fit, a rank-deficient block): the caller then runs
+// the full Jacobi.
+constexpr int SUBSPACE_MIN_N = 1024;
+// the source-space Gram eigenproblem (n_src x n_src) takes it from this size
+constexpr int SUBSPACE_MIN_GRAM = 256;
+
+// finish(X, theta, mtop, split, clusters) receives the converged Ritz
+// vectors (vector-major, descending) and values.
+template <class F>
+int subspace_topk(litho_ctx* c, double2* H, int n, int nk, double rtol, F&& finish) {
+    hipStream_t st = c->stream;
+    const int b = std::min(n, std::max(64, (2 * nk + 16 + 15) / 16 * 16));
+    if (b > 256 || b < nk + 8) return 1;
+    const size_t bn = (size_t)b * n, bb = (size_t)b * b;
+    const int nsplit = std::max(1, std::min(64, n / 128));
+    Dev Q, Y, X, HX, T, V, S, Rw, Cf, work, th, res, stat;
+    LITHO_TRY(Q.alloc(bn * sizeof(double2)));
+    LITHO_TRY(Y.alloc(bn * sizeof(double2)));
+    LITHO_TRY(X.alloc(bn * sizeof(double2)));
+    LITHO_TRY(HX.alloc(bn * sizeof(double2)));
+    LITHO_TRY(T.alloc(bb * sizeof(double2)));
+    LITHO_TRY(V.alloc(bb * sizeof(double2)));
+    LITHO_TRY(S.alloc(bb * sizeof(double2)));
+    LITHO_TRY(Rw.alloc(bb * sizeof(double2)));
+    LITHO_TRY(Cf.alloc(bb * sizeof(double2)));
+    LITHO_TRY(work.alloc((size_t)nsplit * bb * sizeof(double2)));
+    LITHO_TRY(th.alloc(b * sizeof(double)));
+    LITHO_TRY(res.alloc(b * sizeof(double)));
+    LITHO_TRY(stat.alloc(sizeof(int)));
+    LITHO_TRY(launch_herm_from_upper(H, n, st));
+    LITHO_HIP(hipMemsetAsync(stat.p, 0, sizeof(int), st));
+    // deterministic start block (64-bit LCG, Knuth's MMIX constants)
+    {
+        std::vector<double2> h0(bn);
+        uint64_t x = 0x9e3779b97f4a7c15ULL;
+        auto u = [&]() {
+            x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+            return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        };
+        for (auto& v : h0) {
+            const double re = u();
+            v = make_double2(re, u());
+        }
+        LITHO_HIP(hipMemcpyAsync(Q.p, h0.data(), bn * sizeof(double2), hipMemcpyHostToDevice, st));
+        LITHO_HIP(hipStreamSynchronize(st));  // h0 is a local
+    }
+    double2 *q = Q.as<double2>(), *y = Y.as<double2>();
+    // CholQR2: z <- z R^-1 twice, through tmp
+    auto orth = [&](double2* z, double2* tmp) -> int {
+        for (int pass = 0; pass < 2; pass++) {
+            double2* src = pass == 0 ? z : tmp;
+            double2* dst = pass == 0 ? tmp : z;
+            LITHO_TRY(launch_zgemm_nt(src, n, 1, src, n, 0, b, b, n, S.as<double2>(), b, nsplit,
+                                      work.as<double2>(), st));
+            LITHO_TRY(launch_chol_coeff(S.as<double2>(), b, Rw.as<double2>(), Cf.as<double2>(),
+                                        stat.as<int>(), st));
+            LITHO_TRY(launch_rotate_rows(Cf.as<double2>(), b, b, src, n, dst, st));
+        }
+        return LITHO_OK;
+    };
+    LITHO_TRY(orth(q, y));
+    std::vector<double> hT(2 * bb), hV(2 * bb), theta(b), hres(b);
+    std::vector<double2> hCf(bb);
+    std::vector<int> ord(b);
+    const int maxit = 600;
+    double best = 1e300;
+    int stall = 0, it = 0;
+    for (; it < maxit; it++) {
+        // y = H q (row i: (H q_i)^T)
+        LITHO_TRY(launch_zgemm_nt(q, n, 0, H, n, 0, b, n, n, y, n, 1, nullptr, st));
+        const bool rr = it >= 2 && it % 2 == 0;
+        if (!rr) {
+            LITHO_TRY(orth(y, X.as<double2>()));
+            std::swap(q, y);
+            continue;
+        }
+        // Rayleigh-Ritz: T = Q^H H Q
+        LITHO_TRY(launch_zgemm_nt(q, n, 1, y, n, 0, b, b, n, T.as<double2>(), b, nsplit,
+                                  work.as<double2>(), st));
+        LITHO_TRY(launch_jacobi_small(T.as<double2>(), V.as<double2>(), b, stat.as<int>(), st));
+        int hs = 0;
+        LITHO_HIP(hipMemcpyAsync(hT.data(), T.p, bb * sizeof(double2), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipMemcpyAsync(hV.data(), V.p, bb * sizeof(double2), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipMemcpyAsync(&hs, stat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipStreamSynchronize(st));
+        if (hs) return 1;
+        for (int i = 0; i < b; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int d) {
+            return hT[2 * ((size_t)a * b + a)] > hT[2 * ((size_t)d * b + d)];
+        });
+        for (int i = 0; i < b; i++) {
+            theta[i] = hT[2 * ((size_t)ord[i] * b + ord[i])];
+            if (!std::isfinite(theta[i])) return LITHO_ENOCONV;
+            // Ritz vector i = sum_m q_m V[m][ord[i]]
+            for (int m = 0; m < b; m++) {
+                const size_t vi = (size_t)m * b + ord[i];
+                hCf[(size_t)i * b + m] = make_double2(hV[2 * vi], hV[2 * vi + 1]);
+            }
+        }
+        // a negative eigenvalue of weight could hide top ones from the block
+        if (theta[b - 1] < -1e-8 * std::fabs(theta[0])) return 1;
+        LITHO_HIP(hipMemcpyAsync(Cf.p, hCf.data(), bb * sizeof(double2), hipMemcpyHostToDevice, st));
+        LITHO_HIP(hipMemcpyAsync(th.p, theta.data(), b * sizeof(double), hipMemcpyHostToDevice, st));
+        LITHO_TRY(launch_rotate_rows(Cf.as<double2>(), b, b, q, n, X.as<double2>(), st));
+        LITHO_TRY(launch_rotate_rows(Cf.as<double2>(), b, b, y, n, HX.as<double2>(), st));
+        LITHO_TRY(launch_ritz_residual(HX.as<double2>(), X.as<double2>(), th.as<double>(), b, n,
+                                       res.as<double>(), st));
+        LITHO_HIP(hipMemcpyAsync(hres.data(), res.p, b * sizeof(double), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipMemcpyAsync(&hs, stat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        LITHO_HIP(hipStreamSynchronize(st));  // hCf, theta are read by the copies above
+        if (hs) return 1;
+        int mtop, split;
+        std::vector<int2> clusters;
+        plan_clusters(theta, nk, &mtop, &split, clusters, rtol);
+        if (mtop > b - 8) return 1;
+        double worst = 0.0;
+        for (int i = 0; i < mtop; i++) worst = std::max(worst, hres[i]);
+        const double scale = std::fabs(theta[0]);
+        // converged at 1e-12 of the norm; a residual that no longer falls is
+        // accepted at 1e-10 (the FP64 floor of an n-term product)
+        if (worst < 0.5 * best) {
+            best = worst;
+            stall = 0;
+        } else {
+            stall++;
+        }
+        const bool done = worst <= 1e-12 * scale || (stall >= 6 && worst <= 1e-10 * scale);
+        if (done) {
+            g_setup.t[3] = 1;
+            g_setup.t[4] = it + 1;
+            return finish(X.as<double2>(), theta, mtop, split, clusters);
+        }
+        // next block: the Ritz vectors' images, ordered by Ritz value
+        LITHO_TRY(orth(HX.as<double2>(), y));
+        LITHO_HIP(hipMemcpyAsync(q, HX.p, bn * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    }
+    return LITHO_ENOCONV;
+}
+
 // Eigen of device H (n x n, upper triangle valid) -> top kernels.
 int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int nk,
                             double* scales, float* kr, float* ki, int* flags,
                             double rtol = 1e-10) {
     hipStream_t st = c->stream;
+    g_setup.t[3] = 0;
+    g_setup.t[4] = 0;
+    if (n >= SUBSPACE_MIN_N) {
+        // the subspace route leaves H Hermitian from its upper triangle, as
+        // the Jacobi route expects
+        const int r = subspace_topk(
+            c, H, n, nk, rtol,
+            [&](double2* X, const std::vector<double>& theta, int mtop, int split,
+                const std::vector<int2>& clusters) -> int {
+                LITHO_TRY(finish_kernels(c, X, n, mtop, nk, Nx, Ny, clusters, kr, ki, nullptr));
+                for (int k = 0; k < nk; k++) scales[k] = theta[k];
+                if (flags) *flags = split ? LITHO_FLAG_CLUSTER_SPLIT : 0;
+                return LITHO_OK;
+            });
+        if (r != 1) return r;
+        g_setup.t[3] = 0;
+        g_setup.t[4] = 0;
+    }
     Dev V, w, order, Vt, wd;
     LITHO_TRY(V.alloc((size_t)n * n * sizeof(double2)));
     LITHO_TRY(jacobi_solve(c, H, V.as<double2>(), n));
@@ -656,13 +832,20 @@ static int tcc_build_impl(const LithoOptics* op, const float* src, int srcSize, 
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
     hipStream_t st = c->stream;
+    g_setup = SetupTiming();
+    double t0 = now_ms();
     Pupil P;
     LITHO_TRY(build_pupil(c, op, src, srcSize, P));
+    double t1 = now_ms();
+    g_setup.t[0] = t1 - t0;
     const size_t nn = (size_t)P.n * P.n;
     Dev C;
     LITHO_TRY(C.alloc(nn * sizeof(double2)));
+    t1 = now_ms();
     LITHO_TRY(contract_herm(c, P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), P.n, P.m,
                             P.imag, prec, C.as<double2>()));
+    LITHO_HIP(hipStreamSynchronize(st));
+    g_setup.t[1] = now_ms() - t1;
     std::vector<double2> h(nn);
     LITHO_HIP(hipMemcpyAsync(h.data(), C.p, nn * sizeof(double2), hipMemcpyDeviceToHost, st));
     LITHO_HIP(hipStreamSynchronize(st));
@@ -720,11 +903,22 @@ int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
     hipStream_t st = c->stream;
+    g_setup = SetupTiming();
     Dev H;
     LITHO_TRY(H.alloc(nn * sizeof(double2)));
     LITHO_HIP(hipMemcpyAsync(H.p, h.data(), nn * sizeof(double2), hipMemcpyHostToDevice, st));
     LITHO_HIP(hipStreamSynchronize(st));  // h is released on every return path below
-    return kernels_from_device_tcc(c, H.as<double2>(), n, Nx, Ny, nk, scales, krn_r, krn_i, flags);
+    const double t0 = now_ms();
+    const int r =
+        kernels_from_device_tcc(c, H.as<double2>(), n, Nx, Ny, nk, scales, krn_r, krn_i, flags);
+    g_setup.t[2] = now_ms() - t0;
+    return r;
+}
+
+int litho_setup_timing(double* out, int n) {
+    if (!out || n < 0) return LITHO_EARG;
+    for (int i = 0; i < n && i < 5; i++) out[i] = g_setup.t[i];
+    return 5;
 }
 
 }  // extern "C"
@@ -738,27 +932,74 @@ static int kernels_from_source_impl(const LithoOptics* op, const float* src, int
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
     hipStream_t st = c->stream;
+    g_setup = SetupTiming();
+    double t0 = now_ms();
     Pupil P;
     LITHO_TRY(build_pupil(c, op, src, srcSize, P));
+    double t1 = now_ms();
+    g_setup.t[0] = t1 - t0;
     const int n = P.n, m = P.m;
     if (nk > n || nk > m) return LITHO_EARG;
     if (n <= m) {
         Dev H;
         LITHO_TRY(H.alloc((size_t)n * n * sizeof(double2)));
+        t1 = now_ms();
         LITHO_TRY(contract_herm(c, P.A.as<double2>(), P.S.as<double>(), P.A.as<double2>(), n, m,
                                 P.imag, prec, H.as<double2>()));
-        return kernels_from_device_tcc(c, H.as<double2>(), n, op->Nx, op->Ny, nk, scales, krn_r,
-                                       krn_i, flags, rtol);
+        LITHO_HIP(hipStreamSynchronize(st));
+        const double t2 = now_ms();
+        g_setup.t[1] = t2 - t1;
+        const int r = kernels_from_device_tcc(c, H.as<double2>(), n, op->Nx, op->Ny, nk, scales,
+                                              krn_r, krn_i, flags, rtol);
+        g_setup.t[2] = now_ms() - t2;
+        return r;
     }
     // Gram route: G = B^H B (m x m), B = A diag(sqrt S)
     Dev Bt, G, U, w, order, Vt, wd;
     LITHO_TRY(Bt.alloc((size_t)m * n * sizeof(double2)));
     LITHO_TRY(G.alloc((size_t)m * m * sizeof(double2)));
     LITHO_TRY(U.alloc((size_t)m * m * sizeof(double2)));
+    t1 = now_ms();
     LITHO_TRY(launch_conj_transpose_scale(P.A.as<double2>(), P.S.as<double>(), n, m,
                                           Bt.as<double2>(), st));
     LITHO_TRY(contract_herm(c, Bt.as<double2>(), nullptr, Bt.as<double2>(), m, n, P.imag, prec,
                             G.as<double2>()));
+    LITHO_HIP(hipStreamSynchronize(st));
+    const double t2 = now_ms();
+    g_setup.t[1] = t2 - t1;
+    if (m >= SUBSPACE_MIN_GRAM) {
+        // top eigenvectors u of the Gram by subspace iteration, lifted as
+        // v = A diag(sqrt S) u / sqrt(lambda) by one matrix-core product
+        const int r = subspace_topk(
+            c, G.as<double2>(), m, nk, rtol,
+            [&](double2* X, const std::vector<double>& theta, int mtop, int split,
+                const std::vector<int2>& clusters) -> int {
+                Dev Xs, Vt2, th;
+                LITHO_TRY(Xs.alloc((size_t)mtop * m * sizeof(double2)));
+                LITHO_TRY(Vt2.alloc((size_t)mtop * n * sizeof(double2)));
+                LITHO_TRY(th.alloc(mtop * sizeof(double)));
+                LITHO_HIP(hipMemcpyAsync(th.p, theta.data(), mtop * sizeof(double),
+                                         hipMemcpyHostToDevice, st));
+                LITHO_TRY(launch_lift_scale(X, P.S.as<double>(), th.as<double>(), mtop, m,
+                                            Xs.as<double2>(), st));
+                LITHO_TRY(launch_zgemm_nt(Xs.as<double2>(), m, 0, P.A.as<double2>(), m, 0, mtop,
+                                          n, m, Vt2.as<double2>(), n, 1, nullptr, st));
+                LITHO_TRY(finish_kernels(c, Vt2.as<double2>(), n, mtop, nk, op->Nx, op->Ny,
+                                         clusters, krn_r, krn_i, nullptr));
+                for (int k = 0; k < nk; k++) scales[k] = theta[k];
+                if (flags)
+                    *flags = (split ? LITHO_FLAG_CLUSTER_SPLIT : 0) | LITHO_FLAG_GRAM_ROUTE;
+                return LITHO_OK;
+            });
+        if (r != 1) {
+            g_setup.t[2] = now_ms() - t2;
+            return r;
+        }
+        // the subspace route leaves G as it was (Hermitian from its upper
+        // triangle): the Jacobi below starts from it
+        g_setup.t[3] = 0;
+        g_setup.t[4] = 0;
+    }
     LITHO_TRY(jacobi_solve(c, G.as<double2>(), U.as<double2>(), m));
     LITHO_TRY(w.alloc(m * sizeof(double)));
     LITHO_TRY(order.alloc(m * sizeof(int)));
@@ -784,6 +1025,7 @@ static int kernels_from_source_impl(const LithoOptics* op, const float* src, int
                              krn_i, nullptr));
     for (int k = 0; k < nk; k++) scales[k] = wdesc[k];
     if (flags) *flags = (split ? LITHO_FLAG_CLUSTER_SPLIT : 0) | LITHO_FLAG_GRAM_ROUTE;
+    g_setup.t[2] = now_ms() - t2;
     return LITHO_OK;
 }
 }  // namespace

@@ -507,6 +507,12 @@ int jacobi_sweep(double2* H, double2* V, int n, const double* tol, JacobiRot* ro
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
+int launch_herm_from_upper(double2* H, int n, hipStream_t st) {
+    hipLaunchKernelGGL(herm_from_upper_kernel, dim3(cdiv((long)n * n, 256)), dim3(256), 0, st, H,
+                       n);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 int launch_diag_rank(const double2* H, int n, double* w, int* order, hipStream_t st) {
     hipLaunchKernelGGL(diag_kernel, dim3(cdiv(n, 256)), dim3(256), 0, st, H, n, w);
     hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n, 256)), dim3(256), 0, st, w, n, order);

@@ -19,6 +19,8 @@ int jacobi_init(double2* H, double2* V, int n, double* partial, int npart, doubl
 // one cyclic sweep (npad-1 rounds); *count accumulates the rotations done
 int jacobi_sweep(double2* H, double2* V, int n, const double* tol, JacobiRot* rots, int* count,
                  hipStream_t st);
+// H <- Hermitian from its upper triangle (zheevr 'U'), real diagonal
+int launch_herm_from_upper(double2* H, int n, hipStream_t st);
 // w = diag(H); order[r] = index of the r-th largest eigenvalue
 int launch_diag_rank(const double2* H, int n, double* w, int* order, hipStream_t st);
 // Vt[r][:] = eigenvector order[r] (vector-major), wd[r] = w[order[r]]

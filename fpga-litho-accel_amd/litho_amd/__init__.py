@@ -30,7 +30,7 @@ EXPORTS = [
     "litho_memcpy_d2h", "litho_memset_dev", "litho_memcpy_d2d", "litho_source_annular", "litho_source_quadrupole",
     "litho_source_dipole", "litho_source_point", "litho_outer_sigma", "litho_compute_nx",
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
-    "litho_kernels_from_source", "litho_tmpimg_grid", "litho_socs_tmpimg",
+    "litho_kernels_from_source", "litho_setup_timing", "litho_tmpimg_grid", "litho_socs_tmpimg",
     "litho_socs_tmpimg_on_grid", "litho_socs_tmpimg_dev", "litho_socs_image", "litho_fi",
     "litho_image_tcc",
     "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
@@ -137,6 +137,7 @@ def lib():
         L.litho_kernels_from_source_prec.argtypes = [C.POINTER(LithoOptics), _f32p, i, i, i,
                                                      _f64p, _f32p, _f32p, C.POINTER(i)]
         L.litho_tmpimg_grid.argtypes = [i]
+        L.litho_setup_timing.argtypes = [_f64p, i]
         L.litho_socs_tmpimg.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                         i, i, i, i, i]
         L.litho_socs_tmpimg_on_grid.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
@@ -272,6 +273,16 @@ def kernels_from_source(src, outerSigma, wavelength, NA, defocus, Lx, Ly, Nx, Ny
                                                 tcc_prec, sc, kr, ki, C.byref(fl)),
            "litho_kernels_from_source_prec")
     return sc, (kr + 1j * ki).astype(np.complex64).reshape(nk, n), fl.value
+
+
+def setup_timing():
+    """Breakdown of this thread's last Module 4 / 6 call: pupil, contraction
+    and eigensolve (ms), the eigensolver (jacobi / subspace) and its
+    iterations."""
+    t = np.zeros(5)
+    _check(min(lib().litho_setup_timing(t, 5), 0), "litho_setup_timing")
+    return {"pupil_ms": float(t[0]), "contraction_ms": float(t[1]), "eigensolve_ms": float(t[2]),
+            "solver": "subspace" if t[3] == 1 else "jacobi", "iterations": int(t[4])}
 
 
 # ---- Module 7 ---------------------------------------------------------------

@@ -103,11 +103,20 @@ int litho_tcc_build(const LithoOptics* op, const float* src, int srcSize,
 
 /* ---- Module 6: calcKernels, litho_thread.cpp L597-663 ---------------------- */
 /* Top-nk eigenpairs of a Hermitian TCC (upper triangle read, as zheevr 'U'),
- * FP64 Jacobi on the GPU.  scales descending (FP64; the reference casts to
+ * FP64 on the GPU: Jacobi below n = 1024, block subspace iteration on the
+ * matrix cores with a Rayleigh-Ritz step above (Jacobi again if the block
+ * cannot hold the top clusters or H is not positive semi-definite).  scales descending (FP64; the reference casts to
  * f32 at L645), kernels flipped (nk x n f32 planes), degenerate clusters in
  * the canonical mirror basis, every vector phase-fixed.  flags may be NULL. */
 int litho_kernels_topk(const double* tcc_re, const double* tcc_im, int n, int Nx, int Ny,
                        int nk, double* scales, float* krn_r, float* krn_i, int* flags);
+/* Host wall-clock breakdown of the calling thread's last litho_tcc_build*,
+ * litho_kernels_from_source* or litho_kernels_topk call (no counterpart in
+ * the reference): out[0] pupil matrix ms, out[1] TCC / Gram contraction ms,
+ * out[2] eigensolve and kernel extraction ms, out[3] eigensolver (0: Jacobi,
+ * 1: block subspace iteration, used for n >= 1024), out[4] its iterations.
+ * Writes min(n, 5) values and returns 5. */
+int litho_setup_timing(double* out, int n);
 /* Source -> kernels without forming the TCC on the host: explicit TCC route
  * when n <= nsrc, source-space Gram route (nsrc x nsrc eigenproblem, lifted)
  * otherwise.  Same outputs as litho_kernels_topk(litho_tcc_build(...)). */
