@@ -87,6 +87,26 @@ def cpu_baseline(masks_host, K, sc, Nx, threads, budget_s, gpu_image_sets):
     return done / dt, done, dt, [max(e) for e in errs]
 
 
+def cpu_setup_baseline(src, sigma, Nx, nk, sc_gpu, K_gpu):
+    """Modules 4 + 6 once on the host: the oracle's pupil matrix (calcTCC's
+    float arithmetic, litho_thread.cpp L458-484) and its source-space Gram
+    route (572^2 FP64 Gram, cyclic Jacobi, lift), the fastest exact CPU form of
+    calcTCC + calcKernels here; also checks the GPU kernels against it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    period = PIX_NM * N
+    t0 = time.perf_counter()
+    A, S = O.pupil_matrix(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx, Nx)
+    t1 = time.perf_counter()
+    so, Ko, _ = O.calc_kernels_gram(A, S, nk, Nx, Nx)
+    t2 = time.perf_counter()
+    return {"pupil_s": round(t1 - t0, 3), "gram_eigen_lift_s": round(t2 - t1, 3),
+            "threads": "oracle (OpenMP where the C code has it)",
+            "gpu_eigenvalue_max_rel_err": float(np.abs(sc_gpu - so).max() / so[0]),
+            "gpu_kernel_max_abs_err": float(np.abs(K_gpu - Ko).max())}
+
+
 def mixed_size_bench(ctx, src, sigma, steps):
     """Batching form: 512 clips with sizes drawn from {256, 512, 768, 1024}^2
     at 4 nm, one K=16 kernel set per size (four TCC builds), all clips in one
@@ -223,6 +243,14 @@ def main():
                     help="sub-batches of the pipelined (chosen) form")
     ap.add_argument("--surrogate-s", type=float, default=60.0,
                     help="FNO training budget in seconds (0: skip the learned form)")
+    ap.add_argument("--no-explicit", action="store_true",
+                    help="skip the explicit n x n TCC + top-K eigensolve timing")
+    ap.add_argument("--setup-only", action="store_true",
+                    help="run the kernel extraction (Modules 4 + 6) timings only and print them")
+    ap.add_argument("--setup-n", type=int, default=1024, choices=[512, 1024, 4096],
+                    help="clip size of --setup-only (512: K=10, 4096: K=64, Gram route only)")
+    ap.add_argument("--no-cpu-setup", action="store_true",
+                    help="skip the CPU Modules 4 + 6 timing")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-size (batching form) measurement")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -241,6 +269,13 @@ def main():
               f"(--gpus {args.gpus} requested). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     B = args.batch
+    global N, NK
+    if args.setup_only and args.setup_n != N:
+        # Modules 4 + 6 at another clip of the same optics: 512^2 (Nx = 17,
+        # K = 10) or 4096^2 (Nx = 137, K = 64, implicit TCC only)
+        N, NK = args.setup_n, {512: 10, 4096: 64}.get(args.setup_n, NK)
+        if (2 * L.compute_nx(NA, PIX_NM * N, 0.9, LAMBDA) + 1) ** 2 > 20000:
+            args.no_explicit = True
 
     # ---- setup (one time): source, Nx, kernels on the GPU ----------------------
     src = L.createSource(41, "Annular", 0.6, 0.9)
@@ -254,17 +289,56 @@ def main():
     t0 = time.perf_counter()
     sc, K, flags = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx, Nx, NK)
     setup_s = time.perf_counter() - t0
-    log(f"kernels (TCC Gram route + FP64 Jacobi on GPU) {setup_s:.3f}s flags={flags}")
+    setup_gram = L.setup_timing()
+    log(f"kernels (Gram route, FP64 subspace eigensolve on GPU) {setup_s:.3f}s flags={flags} "
+        f"{setup_gram}")
     # precision form of the setup: split-BF16 Gram contraction, FP64 eigensolve
     t0 = time.perf_counter()
     sc16, K16, flags16 = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx,
                                                Nx, NK, tcc_prec=L.LITHO_TCC_SPLIT_BF16)
     setup16_s = time.perf_counter() - t0
+    setup_gram16 = L.setup_timing()
+    # the reference's own route (calcTCC L441-532 then calcKernels L597-663):
+    # the explicit n x n TCC, n = 4761, then its top-24 eigenpairs
+    explicit = None
+    if not args.no_explicit:
+        t0 = time.perf_counter()
+        T = L.calcTCC(src, sigma, LAMBDA, NA, DEFOCUS, period, period, Nx, Nx)
+        tcc_wall = time.perf_counter() - t0
+        tcc_t = L.setup_timing()
+        t0 = time.perf_counter()
+        scx, Kx, _ = L.calcKernels(T, NK, Nx, Nx)
+        eig_wall = time.perf_counter() - t0
+        eig_t = L.setup_timing()
+        del T
+        explicit = {"n": (2 * Nx + 1) ** 2,
+                    "pupil_ms": round(tcc_t["pupil_ms"], 3),
+                    "tcc_contraction_ms": round(tcc_t["contraction_ms"], 3),
+                    "tcc_call_s": round(tcc_wall, 3),
+                    "eigensolve_ms": round(eig_t["eigensolve_ms"], 2),
+                    "eigensolver": eig_t["solver"], "iterations": eig_t["iterations"],
+                    "kernels_call_s": round(eig_wall, 3),
+                    "tcc_contraction_gflop": round(8 * 572 * ((2 * Nx + 1) ** 2) ** 2 / 1e9, 1),
+                    "eigenvalue_max_rel_err_vs_gram_route": float(np.abs(scx - sc).max() / sc[0]),
+                    "kernel_max_abs_err_vs_gram_route": float(np.abs(Kx - K).max()),
+                    "reference_48_threads": "TCC 1.386 s, zheevr 111.8 s at n = 4225 "
+                                            "(output/Different_resolution_tests/TEST_REPORT.md "
+                                            "L55, L57)",
+                    "scope": "host wall clock inside the library calls; the call times add the "
+                             "host staging of the n x n complex128 matrix (363 MB each way)"}
+        log(f"explicit route n={explicit['n']}: TCC contraction {explicit['tcc_contraction_ms']} ms, "
+            f"top-{NK} eigensolve {explicit['eigensolve_ms']} ms ({eig_t['solver']}, "
+            f"{eig_t['iterations']} iterations)")
     eig_err = float(np.abs(sc16 - sc).max() / sc[0])
     krn_err = float(max(np.abs(K16[k] - K[k]).max() / np.abs(K[k]).max() for k in range(NK)))
     log(f"kernels (split-BF16 Gram contraction + FP64 Jacobi) {setup16_s:.3f}s; vs FP64 "
         f"eigenvalues {eig_err:.2e}, kernels {krn_err:.2e}")
 
+    if args.setup_only:
+        print(json.dumps({"N": N, "Nx": Nx, "K": NK, "gram_route_fp64": setup_gram,
+                          "gram_route_split_bf16": setup_gram16,
+                          "explicit_route_fp64": explicit}), flush=True)
+        return
     ctx = L.Context()
     ctx.set_path(L.LITHO_PATH_POW2 if args.path == "pow2" else L.LITHO_PATH_AUTO)
     bank = ctx.bank(K, sc, Nx, Nx)
@@ -336,6 +410,9 @@ def main():
     bf16_bank_prec = L.LITHO_PREC_COMPACT & ~L.LITHO_PREC_BANK_FP16 | L.LITHO_PREC_BANK_BF16
     bf16_bank = timed_form(bf16_bank_prec, d_mask8, bank16)
     bf16_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
+    bf16x2_prec = L.LITHO_PREC_COMPACT & ~L.LITHO_PREC_BANK_FP16 | L.LITHO_PREC_BANK_BF16X2
+    bf16x2_bank = timed_form(bf16x2_prec, d_mask8, bank16)
+    bf16x2_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
     chosen = timed_form(L.LITHO_PREC_COMPACT, d_mask8, bank16)
     log(f"graph FP32 {pipe_fp32:.1f} images/s; fp16 storage {mixed:.1f}; bf16 bank {bf16_bank:.1f}; "
         f"chosen form (split-BF16 kernels, u8 masks, matrix-core DFTs, fp16 window, bank and images) "
@@ -362,18 +439,28 @@ def main():
     chosen_err = None
     mixed_err = None
     bf16_err = None
+    bf16x2_err = None
     if not args.no_cpu:
+        # the box's CPU share for one GPU is 16 threads (OMP_NUM_THREADS there)
         threads = min(16, os.cpu_count() or 1)
         rate, n_done, dt, errs = cpu_baseline(masks[:8], K, sc, Nx, threads, args.cpu_budget,
-                                              [out_imgs, chosen_imgs, mixed_imgs, bf16_imgs])
-        max_err, chosen_err, mixed_err, bf16_err = errs
+                                              [out_imgs, chosen_imgs, mixed_imgs, bf16_imgs,
+                                               bf16x2_imgs])
+        max_err, chosen_err, mixed_err, bf16_err, bf16x2_err = errs
+        rate1, n1, dt1, _ = cpu_baseline(masks[:1], K, sc, Nx, 1, args.cpu_budget / 2, [])
         cpu = {"value": round(rate, 4), "unit": "images/s", "cores": threads, "kind": "port",
                "sample": f"{n_done} clip passes over 8 of the same 1024^2 clips, Module 3 + "
                          f"Module 7 (mask FFT, "
                          f"24-kernel SOCS on the 256^2 grid, FI) in FP64 with own FFT, "
-                         f"{dt:.1f}s"}
-        log(f"cpu baseline {rate:.3f} images/s on {threads} threads; GPU vs FP64 max rel err "
-            f"plain {max_err:.2e}, chosen {chosen_err:.2e}")
+                         f"{dt:.1f}s",
+               "host_nproc": os.cpu_count(),
+               "single_thread": {"value": round(rate1, 4), "unit": "images/s", "cores": 1,
+                                 "sample": f"{n1} passes over one clip, {dt1:.1f}s"}}
+        log(f"cpu baseline {rate:.3f} images/s on {threads} threads, {rate1:.3f} on 1; GPU vs "
+            f"FP64 max rel err plain {max_err:.2e}, chosen {chosen_err:.2e}")
+        if not args.no_cpu_setup:
+            cpu["setup_modules_4_6"] = cpu_setup_baseline(src, sigma, Nx, NK, sc, K)
+            log(f"cpu setup (Modules 4 + 6, Gram route): {cpu['setup_modules_4_6']}")
 
     line = {
         "metric": "1024^2 aerial images/s, plain vs chosen form at <=1e-3 max rel err; % HBM peak",
@@ -405,6 +492,11 @@ def main():
                               "traffic": path_traffic,
                               "scope": "whole step; algorithmic bytes = 8 N^2 per image (mask "
                                        "read + image write) / summed launch durations"}},
+        "path_roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                          "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": path_traffic,
+                          "scope": "whole step of the plain path: 8 N^2 bytes per image (mask "
+                                   "read + image write) / summed launch durations"},
         "kernels": kernels,
         "max_rel_err_vs_fp64": max_err,
         "chosen_form": {
@@ -428,8 +520,17 @@ def main():
                                "eigenvalue_max_rel_err": eig_err,
                                "kernel_max_rel_err": krn_err},
             "bf16_bank_value": round(bf16_bank, 2),
-            "bf16_bank_max_rel_err_vs_fp64": bf16_err},
+            "bf16_bank_max_rel_err_vs_fp64": bf16_err,
+            "bf16x2_bank_value": round(bf16x2_bank, 2),
+            "bf16x2_bank_max_rel_err_vs_fp64": bf16x2_err,
+            "bank_forms": "bf16: one bf16 per component x per-entry FP32 scale (8 significant "
+                          "bits); bf16x2: bf16 hi + lo per component, same scales (16 bits); "
+                          "fp16 (chosen): one fp16 per component (11 bits)"},
         "setup_s": round(setup_s, 3),
+        "setup": {"gram_route_fp64": setup_gram, "gram_route_split_bf16": setup_gram16,
+                  "explicit_route_fp64": explicit,
+                  "scope": "host wall clock (ms) inside the library: pupil matrix, contraction, "
+                           "eigensolve + kernel extraction"},
         "batching_form": mixed_sizes,
         "learned_form": surrogate,
         "cpu_baseline": cpu,

@@ -89,6 +89,7 @@ struct KernelSet {
     float* d_scales = nullptr;
     unsigned* d_bank16 = nullptr;  // bf16 (re | im << 16), scaled per entry
     unsigned* d_bankh = nullptr;   // fp16 (re, im) pairs, scaled per entry
+    uint2* d_bank16x2 = nullptr;   // bf16 hi + lo pairs, scaled per entry
     float2* d_kscale = nullptr;
     int n = 0;
 };
@@ -817,7 +818,14 @@ int kernels_from_device_tcc(litho_ctx* c, double2* H, int n, int Nx, int Ny, int
 static int contract_herm(litho_ctx* c, const double2* X, const double* sx, const double2* Y, int n,
                          int m, int imag, int prec, double2* C) {
     hipStream_t st = c->stream;
-    if (prec == LITHO_TCC_FP64) return launch_herm_gemm(X, sx, Y, n, m, imag, C, st);
+    if (prec == LITHO_TCC_FP64) {
+        const int ns = herm_gemm_slices(n, m);
+        Dev part;
+        if (ns > 1) LITHO_TRY(part.alloc((size_t)ns * n * n * sizeof(double2)));
+        LITHO_TRY(launch_herm_gemm(X, sx, Y, n, m, imag, C, part.as<double2>(), ns, st));
+        LITHO_HIP(hipStreamSynchronize(st));  // before the scratch is freed
+        return LITHO_OK;
+    }
     const int nks = herm_gemm_bf16s_slices(n, m);
     Dev part;
     LITHO_TRY(part.alloc((size_t)nks * n * n * sizeof(float2)));
@@ -967,7 +975,9 @@ static int kernels_from_source_impl(const LithoOptics* op, const float* src, int
     LITHO_HIP(hipStreamSynchronize(st));
     const double t2 = now_ms();
     g_setup.t[1] = t2 - t1;
-    if (m >= SUBSPACE_MIN_GRAM) {
+    // below about nine blocks of 64 vectors the Gram's own Jacobi is the
+    // faster of the two (572^2 with K = 64 at 4096^2: 85 against 290 ms)
+    if (m >= SUBSPACE_MIN_GRAM && 2 * nk + 16 <= 64) {
         // top eigenvectors u of the Gram by subspace iteration, lifted as
         // v = A diag(sqrt S) u / sqrt(lambda) by one matrix-core product
         const int r = subspace_topk(
@@ -1041,6 +1051,7 @@ static int upload_set(litho_ctx* c, const std::vector<float2>& h, const std::vec
     const int n = (int)w.size();
     s->n = n;
     std::vector<unsigned> h16(h.size());
+    std::vector<uint2> h16x2(h.size());
     std::vector<__half2> hh(h.size());
     std::vector<float2> ks(n);
     auto bf16 = [](float f) -> unsigned {
@@ -1056,9 +1067,17 @@ static int upload_set(litho_ctx* c, const std::vector<float2>& h, const std::vec
             my = std::max(my, std::fabs(h[k * per + j].y));
         }
         ks[k] = make_float2(mx > 0.f ? mx : 1.0f, my > 0.f ? my : 1.0f);
+        auto bf16f = [&](unsigned v) {
+            const unsigned u = v << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return f;
+        };
         for (size_t j = 0; j < per; j++) {
             const float re = h[k * per + j].x / ks[k].x, im = h[k * per + j].y / ks[k].y;
             h16[k * per + j] = bf16(re) | (bf16(im) << 16);
+            const unsigned rlo = bf16(re - bf16f(bf16(re))), ilo = bf16(im - bf16f(bf16(im)));
+            h16x2[k * per + j] = make_uint2(h16[k * per + j], rlo | (ilo << 16));
             hh[k * per + j] = __floats2half2_rn(re, im);
         }
     }
@@ -1066,6 +1085,7 @@ static int upload_set(litho_ctx* c, const std::vector<float2>& h, const std::vec
         hipMalloc(&s->d_scales, n * sizeof(float)) != hipSuccess ||
         hipMalloc(&s->d_bank16, h16.size() * sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&s->d_bankh, hh.size() * sizeof(__half2)) != hipSuccess ||
+        hipMalloc(&s->d_bank16x2, h16x2.size() * sizeof(uint2)) != hipSuccess ||
         hipMalloc(&s->d_kscale, n * sizeof(float2)) != hipSuccess)
         return LITHO_ENOMEM;
     hipStream_t st = c->stream;
@@ -1074,6 +1094,8 @@ static int upload_set(litho_ctx* c, const std::vector<float2>& h, const std::vec
     LITHO_HIP(hipMemcpyAsync(s->d_bank16, h16.data(), h16.size() * sizeof(unsigned),
                              hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(s->d_bankh, hh.data(), hh.size() * sizeof(__half2),
+                             hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(s->d_bank16x2, h16x2.data(), h16x2.size() * sizeof(uint2),
                              hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(s->d_kscale, ks.data(), n * sizeof(float2), hipMemcpyHostToDevice, st));
     // the host copies are locals of this call: the copies finish before they go
@@ -1203,6 +1225,7 @@ static void free_set(KernelSet& s) {
     if (s.d_scales) (void)hipFree(s.d_scales);
     if (s.d_bank16) (void)hipFree(s.d_bank16);
     if (s.d_bankh) (void)hipFree(s.d_bankh);
+    if (s.d_bank16x2) (void)hipFree(s.d_bank16x2);
     if (s.d_kscale) (void)hipFree(s.d_kscale);
     s = KernelSet();
 }
@@ -1403,9 +1426,11 @@ static BandPlan plan_of(litho_ctx* c, int N, int Nx, int nk, int prec) {
 
 static bool prec_ok(int prec) {
     const int known = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 |
-                      LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA;
+                      LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA |
+                      LITHO_PREC_BANK_BF16X2;
     if (prec & ~known) return false;
-    if ((prec & LITHO_PREC_BANK_BF16) && (prec & LITHO_PREC_BANK_FP16)) return false;
+    const int banks = prec & (LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_BANK_BF16X2);
+    if (banks & (banks - 1)) return false;  // at most one bank format
     return true;
 }
 
@@ -1470,9 +1495,14 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     const bool u8 = (prec & LITHO_PREC_MASK_U8) != 0;
     const bool out16 = (prec & LITHO_PREC_IMAGE_FP16) != 0;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
-    const int bfmt = (prec & LITHO_PREC_BANK_BF16) ? 1 : ((prec & LITHO_PREC_BANK_FP16) ? 2 : 0);
-    const void* kb = bfmt == 1 ? (const void*)bank->band.d_bank16
-                               : (bfmt == 2 ? (const void*)bank->band.d_bankh : (const void*)bank->band.d_bank);
+    const int bfmt = (prec & LITHO_PREC_BANK_BF16)     ? 1
+                     : (prec & LITHO_PREC_BANK_FP16)   ? 2
+                     : (prec & LITHO_PREC_BANK_BF16X2) ? 3
+                                                       : 0;
+    const void* kb = bfmt == 1   ? (const void*)bank->band.d_bank16
+                     : bfmt == 2 ? (const void*)bank->band.d_bankh
+                     : bfmt == 3 ? (const void*)bank->band.d_bank16x2
+                                 : (const void*)bank->band.d_bank;
     const float2* ks = bfmt ? bank->band.d_kscale : nullptr;
     const void *tm = nullptr, *ti = nullptr;
     if (mfma) {

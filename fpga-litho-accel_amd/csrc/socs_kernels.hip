@@ -113,6 +113,26 @@ struct Bank<2> {
     }
 };
 
+// BK = 3: bf16 hi + lo pairs per component, (re_hi | im_hi << 16, re_lo | im_lo << 16),
+// lo = bf16(x - hi): 16 significant bits, x kscale[k]
+template <>
+struct Bank<3> {
+    using R = uint2;
+    __device__ static __forceinline__ R raw(const void* b, long idx) {
+        return ((const uint2*)b)[idx];
+    }
+    __device__ static __forceinline__ float2 dec(R v, float2 s) {
+        const float re = bf16_to_f32((unsigned short)(v.x & 0xffffu)) +
+                         bf16_to_f32((unsigned short)(v.y & 0xffffu));
+        const float im = bf16_to_f32((unsigned short)(v.x >> 16)) +
+                         bf16_to_f32((unsigned short)(v.y >> 16));
+        return make_float2(re * s.x, im * s.y);
+    }
+    __device__ static __forceinline__ float2 ld(const void* b, const float2* ks, long idx, int k) {
+        return dec(raw(b, idx), ks[k]);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // rows_dft: in  [nrows][L] f32 (nrows even), out [nrows][count] float2
 // out[r][j] = sgn(k) * X_r[k mod L], k = kmin + j, X_r = DFT_L(row r) forward,
@@ -1970,11 +1990,12 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float2* ksc
                      const float* scales, int nk, int Nx, int B, float2* C, int shape,
                      hipStream_t st) {
     const int W = 2 * Nx + 1;
-    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 2 || shape < 0 ||
+    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 3 || shape < 0 ||
         shape > 2)
         return LITHO_EARG;
 #define LITHO_BAND(p, nz, slots)                                                                  \
     if (G == p * p && W <= nz * p) {                                                              \
+        if (bank_fmt == 3) return LITHO_EARG; /* hi + lo pairs: wave-synced blocks only */       \
         if (bank_fmt == 1)                                                                        \
             return launch_band<p, nz, slots, 1>(M, bank, kscale, scales, nk, Nx, B, C, st);       \
         if (bank_fmt == 2)                                                                        \
@@ -1987,6 +2008,8 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float2* ksc
             return launch_wave<p, nz, nw, 1>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
         if (bank_fmt == 2)                                                                        \
             return launch_wave<p, nz, nw, 2>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
+        if (bank_fmt == 3)                                                                        \
+            return launch_wave<p, nz, nw, 3>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
         return launch_wave<p, nz, nw, 0>(M, bank, kscale, scales, nk, Nx, B, C, st);             \
     }
     if (shape == 2) {

@@ -10,6 +10,8 @@
 //                    diag(sqrt S).
 //   conj_transpose_scale : Bt[s][i] = conj(A[i][s]) * sqrt(S[s]).
 #include "common.hpp"
+#include <algorithm>
+
 #include "tcc_kernels.hpp"
 
 namespace litho {
@@ -104,11 +106,15 @@ __global__ __launch_bounds__(256) void conj_transpose_scale_kernel(const double2
 constexpr int TG_T = 32;  // tile
 constexpr int TG_K = 16;  // K stage
 
+// blockIdx.y splits K into slices of kslice: with more than one slice, slice
+// z writes its upper-triangle tile sums to P[z] (n x n) and herm_sum_kernel
+// adds them up and mirrors.
 __global__ __launch_bounds__(256) void herm_gemm_mfma_kernel(const double2* __restrict__ X,
                                                              const double* __restrict__ sx,
                                                              const double2* __restrict__ Y, int n,
-                                                             int m, int has_imag,
-                                                             double2* __restrict__ C) {
+                                                             int m, int has_imag, int kslice,
+                                                             double2* __restrict__ C,
+                                                             double2* __restrict__ P) {
     const int nt = (n + TG_T - 1) / TG_T;
     // linear block index -> (ti, tj) with tj >= ti
     long bid = blockIdx.x;
@@ -130,12 +136,13 @@ __global__ __launch_bounds__(256) void herm_gemm_mfma_kernel(const double2* __re
     const int i0 = ti * TG_T, j0 = tj * TG_T;
     d4 cr = {0.0, 0.0, 0.0, 0.0};
     d4 ci = {0.0, 0.0, 0.0, 0.0};
-    for (int s0 = 0; s0 < m; s0 += TG_K) {
+    const int sbeg = blockIdx.y * kslice, send = min(m, sbeg + kslice);
+    for (int s0 = sbeg; s0 < send; s0 += TG_K) {
         for (int e = tid; e < TG_T * TG_K; e += 256) {
             int r = e / TG_K, c = e - r * TG_K;
             int s = s0 + c;
             double2 xv = make_double2(0.0, 0.0), yv = make_double2(0.0, 0.0);
-            if (s < m) {
+            if (s < send) {
                 if (i0 + r < n) {
                     xv = X[(long)(i0 + r) * m + s];
                     if (sx) {
@@ -171,6 +178,10 @@ __global__ __launch_bounds__(256) void herm_gemm_mfma_kernel(const double2* __re
         if (i >= n || j >= n) continue;
         if (ti == tj && j < i) continue;  // diagonal tile: upper part only
         double re = cr[v], im = ci[v];
+        if (P) {
+            P[(long)blockIdx.y * n * n + (long)i * n + j] = make_double2(re, im);
+            continue;
+        }
         if (i == j) {
             C[(long)i * n + j] = make_double2(re, 0.0);
         } else {
@@ -197,13 +208,49 @@ int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m,
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
+// C[i][j] (i <= j) = sum_z P[z][i][j], mirrored below the diagonal
+__global__ __launch_bounds__(256) void herm_sum_kernel(const double2* __restrict__ P, int ns,
+                                                       int n, double2* __restrict__ C) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (j < i) return;
+    double re = 0.0, im = 0.0;
+    for (int z = 0; z < ns; z++) {
+        const double2 v = P[(long)z * n * n + e];
+        re += v.x;
+        im += v.y;
+    }
+    if (i == j) {
+        C[e] = make_double2(re, 0.0);
+    } else {
+        C[e] = make_double2(re, im);
+        C[(long)j * n + i] = make_double2(re, -im);
+    }
+}
+
+int herm_gemm_slices(int n, int m) {
+    const long nt = (n + TG_T - 1) / TG_T;
+    const long nblocks = nt * (nt + 1) / 2;
+    // about 4096 blocks in flight, slices of at least 256 terms
+    long ns = (4096 + nblocks - 1) / nblocks;
+    ns = std::min(ns, (long)std::max(1, m / 256));
+    ns = std::min(ns, 64L);
+    return (int)std::max(1L, ns);
+}
+
 int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
-                     int has_imag, double2* C, hipStream_t st) {
-    if (n <= 0 || m <= 0) return LITHO_EARG;
+                     int has_imag, double2* C, double2* P, int nslices, hipStream_t st) {
+    if (n <= 0 || m <= 0 || nslices <= 0 || (nslices > 1 && !P)) return LITHO_EARG;
     long nt = (n + TG_T - 1) / TG_T;
     long nblocks = nt * (nt + 1) / 2;
-    hipLaunchKernelGGL(herm_gemm_mfma_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, X, sx, Y,
-                       n, m, has_imag, C);
+    const int kslice = (int)(((long)m + nslices - 1) / nslices + TG_K - 1) / TG_K * TG_K;
+    const int ns = (m + kslice - 1) / kslice;
+    hipLaunchKernelGGL(herm_gemm_mfma_kernel, dim3((unsigned)nblocks, ns), dim3(256), 0, st, X, sx,
+                       Y, n, m, has_imag, kslice, C, ns > 1 ? P : nullptr);
+    if (ns > 1)
+        hipLaunchKernelGGL(herm_sum_kernel, dim3(cdiv((long)n * n, 256)), dim3(256), 0, st, P, ns,
+                           n, C);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 

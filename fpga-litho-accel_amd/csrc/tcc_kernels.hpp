@@ -20,9 +20,12 @@ int launch_pupil(const int2* srcs, int nsrc, const PupilGeom& g, double2* A, int
                  hipStream_t st);
 int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m, double2* Bt,
                                 hipStream_t st);
-// C = (X diag(sx)) Y^H (sx may be null), Hermitian output, n x n
+// C = (X diag(sx)) Y^H (sx may be null), Hermitian output, n x n; the m terms
+// split into nslices K slices (herm_gemm_slices) whose partial sums go
+// through P (nslices * n * n complex doubles; unused for one slice)
+int herm_gemm_slices(int n, int m);
 int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
-                     int has_imag, double2* C, hipStream_t st);
+                     int has_imag, double2* C, double2* P, int nslices, hipStream_t st);
 // Same contraction in split BF16 on the matrix cores (tcc_bf16.hip): part is
 // an FP32 scratch of nks * n * n float2 (nks from herm_gemm_bf16s_slices).
 int herm_gemm_bf16s_slices(int n, int m);

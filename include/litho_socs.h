@@ -246,7 +246,9 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
  *    column output too, unless DFT_MFMA is set: the image-side product splits
  *    that FP32 output itself);
  *  - BANK_BF16 / BANK_FP16 read the kernel bank as 16-bit pairs with one FP32
- *    scale per entry for the real and one for the imaginary parts;
+ *    scale per entry for the real and one for the imaginary parts; BANK_BF16X2
+ *    as bf16 hi + lo pairs (hi = bf16(x), lo = bf16(x - hi)) with the same
+ *    scales, 16 significant bits (wave-synced SOCS blocks only);
  *  - DFT_MFMA computes the mask-row and image-row band DFTs as fp16 matrix
  *    products with FP32 accumulation on the matrix cores (the image side
  *    splits its input into hi + lo fp16 parts); the other stages stay FP32;
@@ -264,6 +266,7 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
 #define LITHO_PREC_MASK_U8 8
 #define LITHO_PREC_IMAGE_FP16 16
 #define LITHO_PREC_DFT_MFMA 32
+#define LITHO_PREC_BANK_BF16X2 64
 #define LITHO_PREC_MIXED (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16)
 #define LITHO_PREC_COMPACT                                                              \
     (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8 | LITHO_PREC_DFT_MFMA | \

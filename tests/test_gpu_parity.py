@@ -306,11 +306,14 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
         assert max_rel(got[4], ref) < tol
         if precision:
             # report the error of every storage flag on its own
-            for flag in (1, 2, 4):
+            for flag in (1, 2, 4, 64):
                 ctx.socs_image_batch_pipelined(bank, dm, B, N, d2, flag, 2, False)
                 ctx.sync()
                 e = max_rel(ctx.d2h(np.zeros_like(masks), d2)[4], ref)
                 print(f"[precision flag {flag}] max rel err vs FP64 {e:.3e}", flush=True)
+                if flag == 64:
+                    # bf16 hi + lo bank: 16 significant bits per kernel value
+                    assert e < 1e-4
         for p_ in (dm, d1, d2):
             ctx.free(p_)
         gpu.lib().litho_bank_destroy(bank)
@@ -321,7 +324,8 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
 @pytest.mark.parametrize("N,prec", [(1024, 32), (1024, 8 | 32 | 1 | 4), (1024, 8 | 32 | 4 | 16),
                                     (512, 8 | 32 | 1 | 4), (256, 8 | 32), (1024, 8 | 4 | 16),
                                     (768, 8 | 16), (512, 8 | 1 | 4 | 16),
-                                    (1024, 8 | 32 | 1 | 4 | 16), (512, 8 | 32 | 1 | 4 | 16)])
+                                    (1024, 8 | 32 | 1 | 4 | 16), (512, 8 | 32 | 1 | 4 | 16),
+                                    (1024, 8 | 32 | 1 | 64 | 16)])
 def test_matrix_core_forms(gpu, orc, base_source, N, prec):
     """u8 / f32 masks and f32 / fp16 images, with the band DFTs on the fp16
     matrix cores (flag 32) or as FP32 FFTs, stay within 1e-3 of FP64 on binary
@@ -379,7 +383,7 @@ def test_matrix_core_flag_rules(gpu, clip512):
     try:
         bank = ctx.bank(clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
         d = ctx.alloc(4 * 512 * 512)
-        for bad in (64, 128, 2 | 4, 2 | 4 | 32):
+        for bad in (128, 256, 2 | 4, 2 | 4 | 32, 4 | 64, 2 | 64):
             with pytest.raises(gpu.LithoError):
                 ctx.socs_image_batch_pipelined(bank, d, 1, 512, d, bad, 1, False)
         ctx.free(d)
