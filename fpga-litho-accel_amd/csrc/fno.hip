@@ -19,9 +19,12 @@
 // transpose, the adjoint of the truncated row DFT of real data is
 // Re(sum_kx d e^{+i theta}), and of the weighted c2r sum is the weighted
 // forward DFT.  Weight gradients over all pixels are outer-product
-// reductions split over blocks and added with FP32 atomics.
+// reductions split over pixel chunks: every chunk writes its partial sums,
+// and a second pass adds the chunks in a fixed order, so a gradient (and the
+// loss) is the same bit for bit from run to run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -384,14 +387,15 @@ __global__ __launch_bounds__(256) void proj_bwd_kernel(const float* __restrict__
     for (int i = 0; i < CW; i++) dx[(b * CW + i) * NN + p] = acc[i];
 }
 
-// G[j][c] += sum_{b,p} A[b][j][p] f(Bm[b][c][p]) (f = GELU when gelu_b), gb[j] += sum A;
-// one block per (pixel chunk, j group of 16); K split over blocks, atomics at the end.
+// Partials of G[j][c] = sum_{b,p} A[b][j][p] f(Bm[b][c][p]) (f = GELU when gelu_b) and
+// gb[j] = sum A over one pixel chunk: block (chunk x, j group y) writes
+// P[x][j * Cc + c] and, with has_gb, P[x][J * Cc + j]; outer_finish_kernel sums them.
 constexpr int OR_PIX = 64;
 __global__ __launch_bounds__(256) void outer_reduce_kernel(const float* __restrict__ A, int J,
                                                            const float* __restrict__ Bm, int Cc,
                                                            int B, long NN, long chunk, int gelu_b,
-                                                           float* __restrict__ G,
-                                                           float* __restrict__ gb) {
+                                                           int has_gb, float* __restrict__ P) {
+    float* Pc = P + (long)blockIdx.x * (J * Cc + J);
     __shared__ float As[16][OR_PIX];
     __shared__ float Bs[PJ][OR_PIX + 1];
     const int j0 = blockIdx.y * 16;
@@ -438,7 +442,7 @@ __global__ __launch_bounds__(256) void outer_reduce_kernel(const float* __restri
                 acc[k] += s;
             }
         }
-        if (gb && threadIdx.x < 16) {
+        if (has_gb && threadIdx.x < 16) {
             float s = 0.f;
             for (int pp = 0; pp < OR_PIX; pp++) s += As[threadIdx.x][pp];
             bacc += s;
@@ -449,16 +453,30 @@ __global__ __launch_bounds__(256) void outer_reduce_kernel(const float* __restri
         const int o = threadIdx.x + 256 * k;
         if (o < nout) {
             const int jj = o / Cc, c = o - jj * Cc;
-            if (j0 + jj < J) atomicAdd(&G[(size_t)(j0 + jj) * Cc + c], acc[k]);
+            if (j0 + jj < J) Pc[(j0 + jj) * Cc + c] = acc[k];
         }
     }
-    if (gb && threadIdx.x < 16 && j0 + threadIdx.x < J) atomicAdd(&gb[j0 + threadIdx.x], bacc);
+    if (has_gb && threadIdx.x < 16 && j0 + threadIdx.x < J) Pc[J * Cc + j0 + threadIdx.x] = bacc;
 }
 
-// dout = 2 (out - t) / n; loss += sum (out - t)^2 / n
+// G[e] += sum over chunks x of P[x][e] (e < J Cc), gb[j] += sum P[x][J Cc + j], chunks
+// added in order
+__global__ __launch_bounds__(256) void outer_finish_kernel(const float* __restrict__ P, int nch,
+                                                           int J, int Cc, float* __restrict__ G,
+                                                           float* __restrict__ gb) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int per = J * Cc + J;
+    if (e >= per || (e >= J * Cc && !gb)) return;
+    float s = 0.f;
+    for (int x = 0; x < nch; x++) s += P[(long)x * per + e];
+    if (e < J * Cc) G[e] += s;
+    else gb[e - J * Cc] += s;
+}
+
+// dout = 2 (out - t) / n; part[block] = this block's sum (out - t)^2 / n
 __global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ out,
                                                   const float* __restrict__ t,
-                                                  float* __restrict__ dout, float* __restrict__ loss,
+                                                  float* __restrict__ dout, float* __restrict__ part,
                                                   long n) {
     __shared__ float red[256];
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
@@ -474,7 +492,22 @@ __global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ out,
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)n);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] / (float)n;
+}
+
+// loss = sum of the np block partials, in a fixed order (one block)
+__global__ __launch_bounds__(256) void sum_parts_kernel(const float* __restrict__ part, int np,
+                                                        float* __restrict__ loss) {
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0];
 }
 
 __global__ void fill_kernel(float* __restrict__ p, long n, float v) {
@@ -530,6 +563,7 @@ struct litho_fno {
     float2* X2[NL] = {};       // retained spectra [B][C][NM]
     float2 *X1 = nullptr, *Y2 = nullptr;
     float *a1 = nullptr, *out = nullptr, *dout = nullptr, *dv = nullptr, *dx = nullptr;
+    float* part = nullptr;     // per-chunk partial sums of the gradient reductions
     std::vector<void*> owned;
 };
 
@@ -570,6 +604,11 @@ int fno_reserve(litho_fno* f, int B) {
     LITHO_TRY(fno_alloc(f, (void**)&f->dout, (size_t)B * NN * sizeof(float)));
     LITHO_TRY(fno_alloc(f, (void**)&f->dv, plane));
     LITHO_TRY(fno_alloc(f, (void**)&f->dx, plane));
+    // the largest reduction: J = PJ rows of CW columns plus PJ biases per
+    // 4096-pixel chunk; the loss takes one partial per 256 pixels
+    const size_t nch = (B * NN + 4095) / 4096;
+    LITHO_TRY(fno_alloc(f, (void**)&f->part,
+                        std::max(nch * (PJ * CW + PJ), (B * NN + 255) / 256) * sizeof(float)));
     f->cap = B;
     return LITHO_OK;
 }
@@ -612,9 +651,13 @@ int outer(litho_fno* f, const float* A, int J, const float* Bm, int Cc, int B, i
           size_t og, long gbo, hipStream_t st) {
     const long NN = (long)f->N * f->N, total = (long)B * NN;
     const long chunk = 4096;
-    const dim3 grid((unsigned)((total + chunk - 1) / chunk), (unsigned)((J + 15) / 16));
+    const int nch = (int)((total + chunk - 1) / chunk);
+    if ((long)J * Cc + J > (long)PJ * CW + PJ) return LITHO_EARG;  // part holds this size
+    const dim3 grid((unsigned)nch, (unsigned)((J + 15) / 16));
     hipLaunchKernelGGL(outer_reduce_kernel, grid, dim3(256), 0, st, A, J, Bm, Cc, B, NN, chunk,
-                       gelu_b, f->g + og, gbo >= 0 ? f->g + gbo : nullptr);
+                       gelu_b, gbo >= 0 ? 1 : 0, f->part);
+    hipLaunchKernelGGL(outer_finish_kernel, dim3(nb((long)J * Cc + J)), dim3(256), 0, st, f->part,
+                       nch, J, Cc, f->g + og, gbo >= 0 ? f->g + gbo : nullptr);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
@@ -749,10 +792,10 @@ int litho_fno_loss_grad(litho_fno* f, const float* d_mask, const float* d_target
     hipStream_t st = ctx_stream(f->ctx);
     const long P = (long)f->L.total, n = (long)B * f->N * f->N;
     hipLaunchKernelGGL(fill_kernel, dim3(nb(P)), dim3(256), 0, st, f->g, P, 0.f);
-    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(256), 0, st, f->loss, 1L, 0.f);
     LITHO_TRY(forward(f, d_mask, B, true, st));
-    hipLaunchKernelGGL(mse_kernel, dim3(nb(n)), dim3(256), 0, st, f->out, d_target, f->dout, f->loss,
+    hipLaunchKernelGGL(mse_kernel, dim3(nb(n)), dim3(256), 0, st, f->out, d_target, f->dout, f->part,
                        n);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, f->part, (int)nb(n), f->loss);
     LITHO_TRY(backward(f, d_mask, B, st));
     if (loss) {
         LITHO_HIP(hipMemcpyAsync(loss, f->loss, sizeof(float), hipMemcpyDeviceToHost, st));

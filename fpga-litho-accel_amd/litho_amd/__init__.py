@@ -370,11 +370,42 @@ def socs_image(mask, krns, scales, Nx, Ny):
 
 
 # ---- device-resident batch path ----------------------------------------------
+class Bank:
+    """One kernel set in HBM (litho_bank).  Passed to the C-ABI as its handle;
+    close() (also called by the owning Context's close and on garbage
+    collection) frees the device memory once."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @property
+    def _as_parameter_(self):
+        if not self.h:
+            raise LithoError(-1, "bank used after close()")
+        return self.h
+
+    @property
+    def value(self):
+        return self._as_parameter_.value
+
+    def close(self):
+        if self.h:
+            lib().litho_bank_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP stream + workspaces (litho_ctx)."""
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self._banks = []
         _check(lib().litho_ctx_create(C.byref(self.h), device), "litho_ctx_create")
 
     def alloc(self, nbytes):
@@ -405,7 +436,9 @@ class Context:
         b = C.c_void_p()
         _check(lib().litho_bank_create(self.h, kr, ki, sc, sc.size, Nx, Ny, C.byref(b)),
                "litho_bank_create")
-        return b
+        bank = Bank(b)
+        self._banks.append(bank)
+        return bank
 
     @staticmethod
     def bank_info(bank):
@@ -439,7 +472,8 @@ class Context:
         """[(bank, N, count, d_masks, d_images), ...] -> ctypes array."""
         arr = (LithoBucket * len(buckets))()
         for k, (bank, N, count, dm, di) in enumerate(buckets):
-            arr[k] = LithoBucket(bank, int(N), int(count), getattr(dm, "value", dm),
+            arr[k] = LithoBucket(getattr(bank, "value", bank), int(N), int(count),
+                                 getattr(dm, "value", dm),
                                  getattr(di, "value", di))
         return arr
 
@@ -492,6 +526,9 @@ class Context:
         return tot.value, dict(zip(names, list(st)[:n]))
 
     def close(self):
+        for b in self._banks:
+            b.close()
+        self._banks = []
         if self.h:
             lib().litho_ctx_destroy(self.h)
             self.h = None

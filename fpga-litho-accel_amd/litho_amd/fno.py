@@ -4,8 +4,8 @@ trained on SOCS-generated pairs by the GPU's own forward / backward kernels
 
 The reference has no learned model.  SOCS stays the ground truth: the
 training targets are SOCS images of seeded Manhattan masks, and in serving
-one clip in `verify_every` is recomputed by SOCS; a verified clip whose max
-relative error exceeds `tol` is served from SOCS instead.
+SOCS recomputes every clip (or one in `verify_every`); a verified clip whose
+max relative error exceeds `tol` is served from SOCS instead.
 """
 import ctypes as C
 import time
@@ -113,32 +113,59 @@ def max_rel_errors(ctx, model, d_masks, d_images, n, batch=64):
 
 
 class SurrogateServer:
-    """Serve images from the FNO; every `verify_every`-th clip is recomputed
-    by SOCS, and a verified clip above `tol` max relative error is served
-    from SOCS (the fallback)."""
+    """Serve images from the FNO with SOCS as judge and fallback.
 
-    def __init__(self, ctx, model, bank, tol=1e-2, verify_every=16):
+    Every `verify_every`-th clip of a batch is recomputed by SOCS and a
+    verified clip above `tol` max relative error is served from SOCS.  With
+    verify_every = 1 (the default) every served clip is within `tol` of SOCS;
+    SOCS at the surrogate's sizes costs a fraction of a percent of the FNO
+    forward pass, so the bound is nearly free.  With verify_every > 1 (one in
+    16 clips in the sampled mode) the other clips are served unverified: their
+    count is kept in `unverified`, and their error is only known
+    statistically (held-out error distribution)."""
+
+    def __init__(self, ctx, model, bank, tol=1e-2, verify_every=1):
+        if verify_every < 1:
+            raise ValueError("verify_every must be >= 1")
         self.ctx, self.model, self.bank = ctx, model, bank
         self.tol, self.every = tol, verify_every
-        self.verified = self.fallbacks = 0
+        self.served = self.verified = self.fallbacks = 0
         self.worst = 0.0
+
+    @property
+    def unverified(self):
+        return self.served - self.verified
+
+    def stats(self):
+        return {"served": self.served, "verified": self.verified,
+                "unverified": self.unverified, "fallbacks": self.fallbacks,
+                "unverified_fraction": self.unverified / max(1, self.served),
+                "worst_verified_before_fallback": self.worst}
 
     def serve(self, d_masks, B, d_images):
         N = self.model.N
         plane = N * N * 4
         self.model.forward(d_masks, B, d_images)
+        self.served += B
         idx = list(range(0, B, self.every))
         if not idx:
             return
-        d_m = self.ctx.alloc(len(idx) * plane)
         d_s = self.ctx.alloc(len(idx) * plane)
-        for j, i in enumerate(idx):
-            self.ctx.d2d(C.c_void_p(d_m.value + j * plane), C.c_void_p(d_masks.value + i * plane),
-                         plane)
-        self.ctx.socs_image_batch(self.bank, d_m, len(idx), N, d_s)
+        if self.every == 1:
+            self.ctx.socs_image_batch(self.bank, d_masks, B, N, d_s)
+        else:
+            d_m = self.ctx.alloc(len(idx) * plane)
+            for j, i in enumerate(idx):
+                self.ctx.d2d(C.c_void_p(d_m.value + j * plane),
+                             C.c_void_p(d_masks.value + i * plane), plane)
+            self.ctx.socs_image_batch(self.bank, d_m, len(idx), N, d_s)
+            self.ctx.free(d_m)
         ref = self.ctx.d2h(np.zeros((len(idx), N, N), np.float32), d_s)
-        got = np.stack([self.ctx.d2h(np.zeros((N, N), np.float32),
-                                     C.c_void_p(d_images.value + i * plane)) for i in idx])
+        if self.every == 1:
+            got = self.ctx.d2h(np.zeros((B, N, N), np.float32), d_images)
+        else:
+            got = np.stack([self.ctx.d2h(np.zeros((N, N), np.float32),
+                                         C.c_void_p(d_images.value + i * plane)) for i in idx])
         err = np.abs(got - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))
         self.verified += len(idx)
         self.worst = max(self.worst, float(err.max()))
@@ -147,5 +174,4 @@ class SurrogateServer:
                 self.fallbacks += 1
                 self.ctx.d2d(C.c_void_p(d_images.value + i * plane),
                              C.c_void_p(d_s.value + j * plane), plane)
-        self.ctx.free(d_m)
         self.ctx.free(d_s)

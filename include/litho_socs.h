@@ -17,8 +17,11 @@
  *
  * Errors: every function returns an int status, LITHO_OK (0) or a negative
  * LITHO_E* code; nothing exits or throws across the boundary.  Host-pointer
- * entry points are synchronous.  A litho_ctx owns one HIP stream and its
- * workspaces: use one context per host thread.
+ * entry points are synchronous and run on a default context of the calling
+ * thread, created on first use and kept (stream, workspaces) until the
+ * process ends: thread pools that come and go should use explicit contexts.
+ * A litho_ctx owns one HIP stream and its workspaces: use one context per
+ * host thread.
  */
 #ifndef LITHO_SOCS_H
 #define LITHO_SOCS_H

@@ -32,7 +32,7 @@ def test_window_widths_match_oracle(gpu, orc, base_source, N, pix):
         _, stages = ctx.last_timing()
         ctx.free(dm)
         ctx.free(di)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()
     assert "socs_band" in stages

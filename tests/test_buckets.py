@@ -38,7 +38,7 @@ def setup(gpu, orc, base_source):
     for p in dev:
         ctx.free(p)
     for b in banks:
-        gpu.lib().litho_bank_destroy(b)
+        b.close()
     ctx.close()
 
 
@@ -131,7 +131,7 @@ def test_bucket_argument_checks(gpu, setup):
     with pytest.raises(gpu.LithoError):
         ctx.socs_image_buckets([(b, N, -1, dm, di)], 0, False)
     with pytest.raises(gpu.LithoError):
-        ctx.socs_image_buckets([(b, N, c, dm, di)], 64, False)  # unknown precision flag
+        ctx.socs_image_buckets([(b, N, c, dm, di)], 128, False)  # unknown precision flag
     with pytest.raises(gpu.LithoError):  # two bank formats at once
         ctx.socs_image_buckets([(b, N, c, dm, di)],
                                gpu.LITHO_PREC_BANK_BF16 | gpu.LITHO_PREC_BANK_FP16, False)

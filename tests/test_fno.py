@@ -51,10 +51,12 @@ def test_oracle_gradient_against_finite_differences():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("N", [32, 64, 256])
 def test_gpu_fno_matches_oracle(gpu, N):
     """Forward output, loss and every gradient tensor of the GPU kernels
-    against the FP64 oracle at the same (bf16-rounded) weights."""
+    against the FP64 oracle at the same (bf16-rounded) weights, up to the
+    surrogate's 256^2 clip; a second backward pass gives the same gradient
+    bit for bit (the reductions add their partial sums in a fixed order)."""
     from litho_amd.fno import FNO
     print(f"[case] fno forward/backward N={N}", flush=True)
     rng = np.random.default_rng(N)
@@ -86,6 +88,9 @@ def test_gpu_fno_matches_oracle(gpu, N):
             assert e < 1e-4, (name, e)
         print(f"[fno N={N}] loss {loss:.6e} vs {ref_loss:.6e}; worst gradient rel err {worst:.2e}",
               flush=True)
+        loss2 = model.loss_grad(dm, dt, B)
+        assert loss2 == loss
+        assert np.array_equal(model.params(grads=True).astype(np.float64), g)
         for p in (dm, dt, do):
             ctx.free(p)
         model.close()
@@ -119,18 +124,28 @@ def test_gpu_training_beats_untrained(gpu, orc, base_source):
               f"median {np.median(e0):.3e} -> {np.median(e1):.3e}", flush=True)
         assert losses[-1] < losses[0] / 20
         assert np.median(e1) < np.median(e0) / 10
-        srv = fno.SurrogateServer(ctx, model, bank, tol=np.median(e1), verify_every=16)
         d_out = ctx.alloc(nte * N * N * 4)
-        srv.serve(vm, nte, d_out)
-        assert srv.verified == nte // 16
-        served = ctx.d2h(np.zeros((nte, N, N), np.float32), d_out)
         ref = ctx.d2h(np.zeros((nte, N, N), np.float32), vi)
-        for i in range(0, nte, 16):  # verified clips meet the tolerance (or are SOCS)
+        # the serving contract, 1e-2 max rel err: every clip verified (default)
+        srv = fno.SurrogateServer(ctx, model, bank, tol=1e-2)
+        srv.serve(vm, nte, d_out)
+        served = ctx.d2h(np.zeros((nte, N, N), np.float32), d_out)
+        err = np.abs(served - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))
+        assert srv.verified == nte and srv.unverified == 0
+        assert err.max() <= 1e-2 * (1 + 1e-6)
+        assert srv.fallbacks == int((e1 > 1e-2).sum())
+        # the sampled mode (1 in 16): verified clips meet 1e-2, the rest are
+        # counted as unverified
+        srv16 = fno.SurrogateServer(ctx, model, bank, tol=1e-2, verify_every=16)
+        srv16.serve(vm, nte, d_out)
+        assert srv16.verified == nte // 16 and srv16.unverified == nte - nte // 16
+        served = ctx.d2h(np.zeros((nte, N, N), np.float32), d_out)
+        for i in range(0, nte, 16):
             e = np.abs(served[i] - ref[i]).max() / np.abs(ref[i]).max()
-            assert e <= srv.tol * (1 + 1e-6)
+            assert e <= 1e-2 * (1 + 1e-6)
         for p in (tm, ti, vm, vi, d_out):
             ctx.free(p)
         model.close()
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()

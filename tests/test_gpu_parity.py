@@ -193,7 +193,7 @@ def test_batch_equals_single(gpu, orc, clip512):
             assert np.array_equal(out[i], single)
         ctx.free(dm)
         ctx.free(di)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()
 
@@ -230,7 +230,7 @@ def test_band_and_pow2_grids_agree(gpu, orc, base_source, N):
             ctx.set_timing(True)
         ctx.free(dm)
         ctx.free(di)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()
     ref = orc.calc_socs(orc.mask_spectrum(masks[1]), K.astype(np.complex128),
@@ -316,7 +316,7 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
                     assert e < 1e-4
         for p_ in (dm, d1, d2):
             ctx.free(p_)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()
 
@@ -372,7 +372,7 @@ def test_matrix_core_forms(gpu, orc, base_source, N, prec):
             assert e < 1e-3
         ctx.free(dm)
         ctx.free(di)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()
 
@@ -387,7 +387,7 @@ def test_matrix_core_flag_rules(gpu, clip512):
             with pytest.raises(gpu.LithoError):
                 ctx.socs_image_batch_pipelined(bank, d, 1, 512, d, bad, 1, False)
         ctx.free(d)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     finally:
         ctx.close()
 

@@ -153,7 +153,7 @@ def gpu_images(gpu, ctx, K, sc, Nx, masks):
         ctx.free(dm)
         ctx.free(di)
     finally:
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
     return info, img
 
 

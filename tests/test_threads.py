@@ -33,7 +33,7 @@ def run_batch(gpu, K, sc, Nx, masks, prec):
                 out.append(ctx.d2h(np.zeros((B, N, N), np.float16), di))
         ctx.free(dm)
         ctx.free(di)
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
         return out
     finally:
         ctx.close()

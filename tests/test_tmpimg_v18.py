@@ -158,5 +158,5 @@ def test_tmpimg_dev_matches_host_entry(gpu, orc, golden_8x5):
         for p in (dr, di, dt, do):
             ctx.free(p)
     finally:
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
         ctx.close()

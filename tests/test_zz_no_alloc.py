@@ -62,7 +62,7 @@ def test_noalloc_matches_batch(gpu, orc, N, nk, batches, path):
                                     sc.astype(np.float32), Nx, Nx)
                 assert max_rel(a[0], ref) < 1e-5
     finally:
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
         ctx.close()
 
 
@@ -79,5 +79,5 @@ def test_noalloc_rejects_batches_beyond_the_scratch(gpu):
         assert b"scratch" in gpu.lib().litho_status_string(rc)
         ctx.free(d)
     finally:
-        gpu.lib().litho_bank_destroy(bank)
+        bank.close()
         ctx.close()
