@@ -167,6 +167,59 @@ def mixed_size_bench(ctx, src, sigma, steps):
     return out
 
 
+def large_clip_bench(ctx, src, sigma, steps, with_oracle):
+    """Config 5: 4096^2 clips at 4 nm (period 16384 nm), Nx = 137, K = 64 from
+    the implicit TCC (source-space Gram route); the SOCS loop in FP32 on the
+    reference's nextPow2 grid (1024 points), kernels stored as given (f32) and
+    as bf16 with per-entry scales (config 5's storage), fp16, bf16 hi + lo.
+    Images/s over graph-replayed batches of 8 clips resident in HBM; error of
+    one clip against the FP64 oracle fed the same kernels."""
+    N, B, nk = 4096, 8, 64
+    Nx = L.compute_nx(NA, PIX_NM * N, sigma, LAMBDA)
+    t0 = time.perf_counter()
+    sc, K, flags = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, PIX_NM * N, PIX_NM * N,
+                                         Nx, Nx, nk)
+    setup = {"wall_s": round(time.perf_counter() - t0, 3), **L.setup_timing()}
+    bank = ctx.bank(K, sc, Nx, Nx)
+    masks = np.stack([L.manhattan_mask(N, 7000 + i, WMIN, WMAX, FILL) for i in range(B)])
+    dm, di = ctx.alloc(masks.nbytes), ctx.alloc(masks.nbytes)
+    ctx.h2d(dm, masks)
+    forms = {"fp32": L.LITHO_PREC_FP32, "bf16_bank": L.LITHO_PREC_BANK_BF16,
+             "fp16_bank": L.LITHO_PREC_BANK_FP16, "bf16x2_bank": L.LITHO_PREC_BANK_BF16X2}
+    out = {"N": N, "Nx": Nx, "K": nk, "batch": B, "conv_grid": 1024,
+           "entries": ctx.bank_info(bank)[0], "setup": setup, "forms": {}}
+    imgs = {}
+    for name, prec in forms.items():
+        for _ in range(2):
+            ctx.socs_image_batch_pipelined(bank, dm, B, N, di, prec, 1, True)
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            ctx.socs_image_batch_pipelined(bank, dm, B, N, di, prec, 1, True)
+        ctx.sync()
+        rate = B * steps / (time.perf_counter() - t)
+        imgs[name] = ctx.d2h(np.zeros((N, N), np.float32), di)
+        out["forms"][name] = {"value": round(rate, 2), "unit": "images/s",
+                              "path_frac": round(8 * N * N * rate / 1e9 / HBM_PEAK_GBPS, 4)}
+        log(f"config 5 (4096^2, K=64) {name}: {rate:.1f} images/s")
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t = time.perf_counter()
+        ref = O.calc_socs(O.mask_spectrum(masks[0]), K.astype(np.complex128), sc.astype(np.float32),
+                          Nx, Nx, nthreads=min(16, os.cpu_count() or 1))
+        for name, img in imgs.items():
+            out["forms"][name]["max_rel_err_vs_fp64"] = float(
+                np.abs(img.astype(np.float64) - ref).max() / np.abs(ref).max())
+        out["oracle_s"] = round(time.perf_counter() - t, 1)
+        log("config 5 errors vs FP64: " + ", ".join(
+            f"{n} {f['max_rel_err_vs_fp64']:.2e}" for n, f in out["forms"].items()))
+    ctx.free(dm)
+    ctx.free(di)
+    bank.close()
+    return out
+
+
 def surrogate_bench(ctx, src, sigma, train_s):
     """Learned form: the FNO trained on 20k SOCS pairs (256^2 at 4 nm, K=16)
     by its own GPU kernels for `train_s` seconds; held-out max relative error
@@ -255,6 +308,8 @@ def main():
                     help="clip size of --setup-only (512: K=10, 4096: K=64, Gram route only)")
     ap.add_argument("--no-cpu-setup", action="store_true",
                     help="skip the CPU Modules 4 + 6 timing")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 4096^2 (config 5) measurement")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-size (batching form) measurement")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -436,6 +491,7 @@ def main():
     out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
     mixed_sizes = None if args.no_mixed else mixed_size_bench(ctx, src, sigma,
                                                               max(3, args.steps // 4))
+    large = None if args.no_large else large_clip_bench(ctx, src, sigma, 4, not args.no_cpu)
     surrogate = surrogate_bench(ctx, src, sigma, args.surrogate_s) if args.surrogate_s > 0 else None
 
     cpu = None
@@ -536,6 +592,7 @@ def main():
                   "scope": "host wall clock (ms) inside the library: pupil matrix, contraction, "
                            "eigensolve + kernel extraction"},
         "batching_form": mixed_sizes,
+        "large_clip_form": large,
         "learned_form": surrogate,
         "cpu_baseline": cpu,
     }

@@ -1289,8 +1289,14 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
 // The launches of one batch on the context stream, into the workspaces ws
 // (sized by image_batch_sizes); per-stage events when timing.
 int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B, int N,
-                        float* d_image, int band_path, void* const ws[5], bool timing) {
+                        float* d_image, int band_path, void* const ws[5], bool timing,
+                        int bank_fmt) {
     const int Nx = bank->Nx;
+    const KernelSet& ks = bank->band;
+    const void* kb = bank_fmt == 1   ? (const void*)ks.d_bank16
+                     : bank_fmt == 2 ? (const void*)ks.d_bankh
+                     : bank_fmt == 3 ? (const void*)ks.d_bank16x2
+                                     : (const void*)ks.d_bank;
     const int W = 2 * Nx + 1;
     const int band = 2 * Nx;
     hipStream_t st = c->stream;
@@ -1308,8 +1314,8 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
         if (timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
         LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
-        LITHO_TRY(launch_socs_band(Gb, (float2*)M, bank->band.d_bank, nullptr, 0, bank->band.d_scales,
-                                   bank->band.n, Nx, B, (float2*)C, 2, st));
+        LITHO_TRY(launch_socs_band(Gb, (float2*)M, kb, ks.d_kscale, bank_fmt, ks.d_scales, ks.n,
+                                   Nx, B, (float2*)C, 2, st));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
         LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
@@ -1331,8 +1337,8 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
     if (timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
     // a real mask's window spectrum, conjugate-symmetric to FP32 round-off: the
     // paired set (cross-talk of that order, far below the FP32 image error)
-    LITHO_TRY(launch_socs_accum(G, (float2*)M, bank->band.d_bank, bank->band.d_scales, bank->band.n, Nx, B,
-                                (float*)T, 0, c->tw, st));
+    LITHO_TRY(launch_socs_accum(G, (float2*)M, kb, ks.d_kscale, bank_fmt, ks.d_scales, ks.n, Nx,
+                                B, (float*)T, 0, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
@@ -1361,7 +1367,7 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
     void* ws[5] = {};
     for (int i = 0; i < 5; i++)
         if (sz[i]) LITHO_TRY(ctx_ws(c, i, sz[i], &ws[i]));
-    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, c->timing);
+    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, c->timing, 0);
 }
 
 // fp16 DFT table of the matrix-core kernels: kind 0 mask rows (N, kmin, count),
@@ -1575,7 +1581,34 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         chunks > 64 || !prec_ok(precision))
         return LITHO_EARG;
     const int Nx = bank->Nx;
-    if (!band_shape_ok(N, Nx)) return LITHO_EARG;
+    if (!band_shape_ok(N, Nx)) {
+        // clips the band grids do not cover (above 1024^2, or 4Nx+1 > 144):
+        // the nextPow2 pipeline in one sub-batch, the kernel bank format the
+        // only storage choice
+        const int bankflags =
+            LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_BANK_BF16X2;
+        if (precision & ~bankflags) return LITHO_EARG;
+        const int bfmt = (precision & LITHO_PREC_BANK_BF16)   ? 1
+                         : (precision & LITHO_PREC_BANK_FP16) ? 2
+                         : (precision & LITHO_PREC_BANK_BF16X2) ? 3
+                                                                : 0;
+        size_t sz[5];
+        int band_path = 0;
+        LITHO_TRY(image_batch_sizes(LITHO_PATH_POW2, bank, B, N, sz, &band_path));
+        void* ws[5] = {};
+        for (int i = 0; i < 5; i++)
+            if (sz[i]) LITHO_TRY(ctx_ws(c, i, sz[i], &ws[i]));
+        c->nstage = 0;
+        c->timed = false;
+        auto enqueue = [&]() -> int {
+            return image_batch_enqueue(c, bank, (const float*)d_mask, B, N, (float*)d_image,
+                                       band_path, ws, false, bfmt);
+        };
+        if (!use_graph) return enqueue();
+        std::vector<long> key = {3, bank->serial, (long)d_mask, (long)d_image, B, N, precision};
+        for (int i = 0; i < 5; i++) key.push_back((long)ws[i]);
+        return run_graph(c, key, enqueue);
+    }
     const bool mfma = (precision & LITHO_PREC_DFT_MFMA) != 0;
     chunks = std::min(chunks, B);
     const int ns = std::min(chunks, 4);  // GPU_MAX_HW_QUEUES is 4 by default
@@ -1902,14 +1935,16 @@ int tmpimg_enqueue(litho_ctx* c, const KernelSet& ks, int m, const float2* dM, i
     if (Gx == Gs && Gy == Gs) {
         float* dst = o1 ? o1 : o2;
         if (!dst) return LITHO_OK;
-        LITHO_TRY(launch_socs_accum(Gs, dM, ks.d_bank, ks.d_scales, ks.n, m, 1, dst, 1, c->tw, st));
+        LITHO_TRY(launch_socs_accum(Gs, dM, ks.d_bank, nullptr, 0, ks.d_scales, ks.n, m, 1, dst, 1,
+                                    c->tw, st));
         if (o1 && o2)
             LITHO_HIP(hipMemcpyAsync(o2, o1, (size_t)Gs * Gs * sizeof(float),
                                      hipMemcpyDeviceToDevice, st));
         return LITHO_OK;
     }
     if (!T) return LITHO_EARG;
-    LITHO_TRY(launch_socs_accum(Gs, dM, ks.d_bank, ks.d_scales, ks.n, m, 1, T, 0, c->tw, st));
+    LITHO_TRY(launch_socs_accum(Gs, dM, ks.d_bank, nullptr, 0, ks.d_scales, ks.n, m, 1, T, 0,
+                                c->tw, st));
     return launch_grid_sample_shift(T, Gs, Gx, Gy, o1, o2, st);
 }
 

@@ -74,5 +74,5 @@ extern "C" int litho_socs_image_batch_noalloc(litho_ctx* c, const litho_bank* ba
     void* ws[5] = {};
     for (int i = 0; i < 5; i++)
         if (sz[i]) ws[i] = base + off[i];
-    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, false);
+    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, false, 0);
 }

@@ -322,9 +322,10 @@ __global__ __launch_bounds__(512) void socs_accum_kernel(const float2* __restric
 // so stage-1 inputs with n1 >= NZ = ceil(W/Q) are compile-time zeros.
 // LDS: W_G table | SLOTS transpose buffers | Y rows [GR][W].
 // ---------------------------------------------------------------------------
-template <int P, int Q, int GR, int NZ, int NTH>
+template <int P, int Q, int GR, int NZ, int NTH, int BK = 0>
 __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restrict__ M,
-                                                          const float2* __restrict__ bank,
+                                                          const void* __restrict__ bank,
+                                                          const float2* __restrict__ kscale,
                                                           const float* __restrict__ scales,
                                                           int nk, int Nx, float* __restrict__ out,
                                                           int shift,
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     __syncthreads();
 
     for (int k = 0; k < nk; k++) {
-        const float2* Kk = bank + (long)k * W * W;
+        const long kbase = (long)k * W * W;
         const float lam = scales[k];
         for (int c0 = 0; c0 < W; c0 += SLOTS) {
             const int col = c0 + slot;
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
                 float2 v = make_float2(0.f, 0.f);
                 if (act && n < W) {
                     const int j = n * W + col;
-                    v = cmul(Kk[j], Mb[j]);
+                    v = cmul(Bank<BK>::ld(bank, kscale, kbase + j, k), Mb[j]);
                 }
                 return v;
             });
@@ -1622,31 +1623,59 @@ int socs_rows_per_block(int G, int Nx) {
     return 0;
 }
 
-template <int P, int Q, int GR, int NZ, int NTH>
-static int launch_socs2(const float2* M, const float2* bank, const float* scales, int nk, int Nx,
-                        int B, float* out, int shift, const float2* tw, hipStream_t st) {
+template <int P, int Q, int GR, int NZ, int NTH, int BK>
+static int launch_socs2_bk(const float2* M, const void* bank, const float2* kscale,
+                           const float* scales, int nk, int Nx, int B, float* out, int shift,
+                           const float2* tw, hipStream_t st) {
     using Geo = Fft2Geom<P, Q>;
     constexpr int G = P * Q;
     constexpr int SLOTS = NTH / Geo::T;
     const int W = 2 * Nx + 1;
     size_t lds = (size_t)(G + SLOTS * Geo::XBUF + GR * W) * sizeof(float2);
     if (lds > 160 * 1024) return LITHO_EARG;
-    auto kern = socs_accum2_kernel<P, Q, GR, NZ, NTH>;
+    auto kern = socs_accum2_kernel<P, Q, GR, NZ, NTH, BK>;
     LITHO_TRY(allow_full_lds((const void*)kern));
     unsigned grid = (unsigned)((size_t)B * (G / GR));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(NTH), lds, st, M, bank, scales, nk, Nx, out, shift,
-                       tw);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NTH), lds, st, M, bank, kscale, scales, nk, Nx, out,
+                       shift, tw);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
-int launch_socs_accum(int G, const float2* M, const float2* bank, const float* scales, int nk,
-                      int Nx, int B, float* out, int shift, const float2* tw, hipStream_t st) {
-    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G) return LITHO_EARG;
+// the 16-bit bank formats are built for the large grids (clips above 1024^2),
+// the kernels as given everywhere
+template <int P, int Q, int GR, int NZ, int NTH>
+static int launch_socs2(const float2* M, const void* bank, const float2* kscale, int bank_fmt,
+                        const float* scales, int nk, int Nx, int B, float* out, int shift,
+                        const float2* tw, hipStream_t st) {
+    if (bank_fmt == 0)
+        return launch_socs2_bk<P, Q, GR, NZ, NTH, 0>(M, bank, kscale, scales, nk, Nx, B, out,
+                                                     shift, tw, st);
+    if constexpr (P * Q >= 512) {
+        if (bank_fmt == 1)
+            return launch_socs2_bk<P, Q, GR, NZ, NTH, 1>(M, bank, kscale, scales, nk, Nx, B, out,
+                                                         shift, tw, st);
+        if (bank_fmt == 2)
+            return launch_socs2_bk<P, Q, GR, NZ, NTH, 2>(M, bank, kscale, scales, nk, Nx, B, out,
+                                                         shift, tw, st);
+        if (bank_fmt == 3)
+            return launch_socs2_bk<P, Q, GR, NZ, NTH, 3>(M, bank, kscale, scales, nk, Nx, B, out,
+                                                         shift, tw, st);
+    }
+    return LITHO_EARG;
+}
+
+int launch_socs_accum(int G, const float2* M, const void* bank, const float2* kscale,
+                      int bank_fmt, const float* scales, int nk, int Nx, int B, float* out,
+                      int shift, const float2* tw, hipStream_t st) {
+    if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 3 ||
+        (bank_fmt && !kscale))
+        return LITHO_EARG;
     const int W = 2 * Nx + 1;
     // register-FFT variant: (P, Q, rows per block, nonzero stage-1 rows, threads)
 #define LITHO_S2(p, q, gr, nz, nth)                                                        \
     if (G == p * q && W <= nz * q)                                                         \
-        return launch_socs2<p, q, gr, nz, nth>(M, bank, scales, nk, Nx, B, out, shift, tw, st);
+        return launch_socs2<p, q, gr, nz, nth>(M, bank, kscale, bank_fmt, scales, nk, Nx, B, out, \
+                                               shift, tw, st);
     LITHO_S2(4, 4, 16, 4, 64)
     LITHO_S2(4, 8, 32, 4, 128)
     LITHO_S2(8, 8, 64, 3, 512)
@@ -1663,9 +1692,10 @@ int launch_socs_accum(int G, const float2* M, const float2* bank, const float* s
     LITHO_S2(32, 32, 16, 32, 256)
 #undef LITHO_S2
     int GR = socs_rows_per_block(G, Nx);
-    if (!GR) return LITHO_EARG;
+    if (!GR || bank_fmt) return LITHO_EARG;
+    const float2* bank2 = (const float2*)bank;
 #define LITHO_SOCS_CASE(g, gr) \
-    if (G == g && GR == gr) return launch_socs_GR<g, gr>(M, bank, scales, nk, Nx, B, out, shift, tw, st);
+    if (G == g && GR == gr) return launch_socs_GR<g, gr>(M, bank2, scales, nk, Nx, B, out, shift, tw, st);
     LITHO_SOCS_CASE(32, 32)
     LITHO_SOCS_CASE(64, 64)
     LITHO_SOCS_CASE(128, 128)

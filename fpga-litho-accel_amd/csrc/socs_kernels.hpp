@@ -14,9 +14,11 @@ int launch_mask_cols(int N, const float2* R, float2* M, int B, int cx, int kymin
                      const float2* tw, hipStream_t st);
 // conv-grid rows handled per block for a grid size / window
 int socs_rows_per_block(int G, int Nx);
-// window M [B][W][W] x bank [nk][W][W] -> intensity [B][G][G] (shifted if shift)
-int launch_socs_accum(int G, const float2* M, const float2* bank, const float* scales, int nk,
-                      int Nx, int B, float* out, int shift, const float2* tw, hipStream_t st);
+// window M [B][W][W] x bank [nk][W][W] -> intensity [B][G][G] (shifted if shift);
+// bank_fmt as launch_socs_band (16-bit formats on grids of 512 points and up)
+int launch_socs_accum(int G, const float2* M, const void* bank, const float2* kscale,
+                      int bank_fmt, const float* scales, int nk, int Nx, int B, float* out,
+                      int shift, const float2* tw, hipStream_t st);
 // FI second axis: P [B][G][band+1] -> U [B][N][band+1]; yalt: P rows came
 // from a centred (fftshifted) grid
 int launch_fi_cols(int G, int N, const float2* P, float2* U, int B, int band, int yalt,

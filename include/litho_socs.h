@@ -261,7 +261,10 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
  * LITHO_PREC_MIXED is the storage-only set that meets 1e-3 max relative error
  * against FP64; LITHO_PREC_COMPACT the matrix-core set with 8-bit masks, an
  * fp16 row-DFT window and fp16 images (6-7.5e-4 at 1024^2, K=24).
- * Uses the band conv grid (N in {256, 512, 1024}, 4Nx+1 <= 144). */
+ * Uses the band conv grid (N in {256, 512, 768, 1024}, 4Nx+1 <= 144); other
+ * clips (up to 4096^2) run the reference's nextPow2 grid in one sub-batch,
+ * where the bank formats (BANK_BF16 / BANK_FP16 / BANK_BF16X2, on conv grids
+ * of 512 points and up) are the only flags taken. */
 #define LITHO_PREC_FP32 0
 #define LITHO_PREC_INTER_FP16 1
 #define LITHO_PREC_BANK_BF16 2

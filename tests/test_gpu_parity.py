@@ -487,3 +487,30 @@ def test_large_clip_implicit_tcc(gpu, orc, base_source):
     ref = orc.calc_socs(orc.mask_spectrum(m), K.astype(np.complex128), sc.astype(np.float32), Nx,
                         Nx, nthreads=16)
     assert max_rel(img, ref) < IMG_TOL
+    # the same clip with 16-bit kernel storage (FP32 SOCS loop, the nextPow2
+    # grid of 1024 points): bf16 is config 5's storage, fp16 and bf16 hi + lo
+    # are reported beside it
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm, di = ctx.alloc(m.nbytes), ctx.alloc(m.nbytes)
+        ctx.h2d(dm, m)
+        for flag, tol in ((0, IMG_TOL), (gpu.LITHO_PREC_BANK_BF16, 1e-2),
+                          (gpu.LITHO_PREC_BANK_FP16, 2e-3), (gpu.LITHO_PREC_BANK_BF16X2, 1e-4)):
+            for graph in (False, True):
+                ctx.socs_image_batch_pipelined(bank, dm, 1, N, di, flag, 1, graph)
+                ctx.sync()
+                got = ctx.d2h(np.zeros_like(m), di)
+                e = max_rel(got, ref)
+                print(f"[C5 bank flag {flag} graph {graph}] max rel err vs FP64 {e:.3e}",
+                      flush=True)
+                assert e < tol, (flag, e)
+            if flag == 0:
+                assert np.array_equal(got, img)
+        with pytest.raises(gpu.LithoError):  # only bank formats on this path
+            ctx.socs_image_batch_pipelined(bank, dm, 1, N, di, gpu.LITHO_PREC_COMPACT, 1, False)
+        ctx.free(dm)
+        ctx.free(di)
+        bank.close()
+    finally:
+        ctx.close()
