@@ -115,6 +115,10 @@ struct litho_bank {
     long serial = 0;
 };
 
+// context workspace slot of batch workspace i (slots 5 and 6 belong to the
+// accelerator-shaped entry)
+static int batch_slot(int i) { return i < 5 ? i : 7; }
+
 static int ctx_ws(litho_ctx* c, int slot, size_t bytes, void** out) {
     DevBuf& b = c->ws[slot];
     if (b.bytes < bytes) {
@@ -1258,9 +1262,10 @@ namespace litho {
 // Workspaces of litho_socs_image_batch for B clips of N^2 under the context's
 // path choice.  Band pipeline: row-DFT window R, mask window M, intensity
 // band C, FI column output U.  nextPow2 path: R, M, conv-grid intensity T,
-// its row-DFT window P, U.  Every kernel overwrites its workspace in full
-// before the next one reads it, so none has to start at zero.
-int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[5], int* band_path) {
+// its row-DFT window P, U, and on grids of 512 points and up the SOCS column
+// pass Yt.  Every kernel overwrites its workspace in full before the next one
+// reads it, so none has to start at zero.
+int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[6], int* band_path) {
     if (!bank || B <= 0 || B > 65535) return LITHO_EARG;
     const int Nx = bank->Nx;
     const size_t W = 2 * Nx + 1, band = 2 * Nx;
@@ -1273,6 +1278,7 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
         sz[2] = (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2);
         sz[3] = 0;
         sz[4] = (size_t)B * (band + 1) * N * sizeof(float2);
+        sz[5] = 0;
         return LITHO_OK;
     }
     const int G = next_pow2(4 * Nx + 1);
@@ -1283,13 +1289,14 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
     sz[2] = (size_t)B * G * G * sizeof(float);
     sz[3] = (size_t)B * G * (band + 1) * sizeof(float2);
     sz[4] = (size_t)B * N * (band + 1) * sizeof(float2);
+    sz[5] = socs_pre_bytes(G, Nx, bank->band.n, B);  // column pass of the large grids
     return LITHO_OK;
 }
 
 // The launches of one batch on the context stream, into the workspaces ws
 // (sized by image_batch_sizes); per-stage events when timing.
 int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B, int N,
-                        float* d_image, int band_path, void* const ws[5], bool timing,
+                        float* d_image, int band_path, void* const ws[6], bool timing,
                         int bank_fmt) {
     const int Nx = bank->Nx;
     const KernelSet& ks = bank->band;
@@ -1338,7 +1345,7 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
     // a real mask's window spectrum, conjugate-symmetric to FP32 round-off: the
     // paired set (cross-talk of that order, far below the FP32 image error)
     LITHO_TRY(launch_socs_accum(G, (float2*)M, kb, ks.d_kscale, bank_fmt, ks.d_scales, ks.n, Nx,
-                                B, (float*)T, 0, c->tw, st));
+                                B, (float*)T, 0, c->tw, st, (float2*)ws[5]));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
@@ -1361,12 +1368,12 @@ extern "C" {
 int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B,
                            int N, float* d_image) {
     if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
-    size_t sz[5];
+    size_t sz[6];
     int band_path = 0;
     LITHO_TRY(image_batch_sizes(c->path, bank, B, N, sz, &band_path));
-    void* ws[5] = {};
-    for (int i = 0; i < 5; i++)
-        if (sz[i]) LITHO_TRY(ctx_ws(c, i, sz[i], &ws[i]));
+    void* ws[6] = {};
+    for (int i = 0; i < 6; i++)
+        if (sz[i]) LITHO_TRY(ctx_ws(c, batch_slot(i), sz[i], &ws[i]));
     return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, c->timing, 0);
 }
 
@@ -1592,12 +1599,12 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
                          : (precision & LITHO_PREC_BANK_FP16) ? 2
                          : (precision & LITHO_PREC_BANK_BF16X2) ? 3
                                                                 : 0;
-        size_t sz[5];
+        size_t sz[6];
         int band_path = 0;
         LITHO_TRY(image_batch_sizes(LITHO_PATH_POW2, bank, B, N, sz, &band_path));
-        void* ws[5] = {};
-        for (int i = 0; i < 5; i++)
-            if (sz[i]) LITHO_TRY(ctx_ws(c, i, sz[i], &ws[i]));
+        void* ws[6] = {};
+        for (int i = 0; i < 6; i++)
+            if (sz[i]) LITHO_TRY(ctx_ws(c, batch_slot(i), sz[i], &ws[i]));
         c->nstage = 0;
         c->timed = false;
         auto enqueue = [&]() -> int {
@@ -1606,7 +1613,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         };
         if (!use_graph) return enqueue();
         std::vector<long> key = {3, bank->serial, (long)d_mask, (long)d_image, B, N, precision};
-        for (int i = 0; i < 5; i++) key.push_back((long)ws[i]);
+        for (int i = 0; i < 6; i++) key.push_back((long)ws[i]);
         return run_graph(c, key, enqueue);
     }
     const bool mfma = (precision & LITHO_PREC_DFT_MFMA) != 0;

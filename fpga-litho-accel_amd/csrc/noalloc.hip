@@ -54,11 +54,11 @@ extern "C" int litho_socs_image_batch_noalloc(litho_ctx* c, const litho_bank* ba
                                               float* d_image) {
     using namespace litho;
     if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
-    size_t sz[5];
+    size_t sz[6];
     int band_path = 0;
     LITHO_TRY(image_batch_sizes(ctx_path(c), bank, B, N, sz, &band_path));
-    size_t off[5], total = 0;
-    for (int i = 0; i < 5; i++) {
+    size_t off[6], total = 0;
+    for (int i = 0; i < 6; i++) {
         off[i] = total;
         total += align256(sz[i]);
     }
@@ -71,8 +71,8 @@ extern "C" int litho_socs_image_batch_noalloc(litho_ctx* c, const litho_bank* ba
         if (g_state[dev] != 1) return g_state[dev] < 0 ? g_state[dev] : LITHO_EHIP;
         base = (char*)g_ptr[dev];
     }
-    void* ws[5] = {};
-    for (int i = 0; i < 5; i++)
+    void* ws[6] = {};
+    for (int i = 0; i < 6; i++)
         if (sz[i]) ws[i] = base + off[i];
     return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, false, 0);
 }

@@ -322,14 +322,19 @@ __global__ __launch_bounds__(512) void socs_accum_kernel(const float2* __restric
 // so stage-1 inputs with n1 >= NZ = ceil(W/Q) are compile-time zeros.
 // LDS: W_G table | SLOTS transpose buffers | Y rows [GR][W].
 // ---------------------------------------------------------------------------
-template <int P, int Q, int GR, int NZ, int NTH, int BK = 0>
+// PRE: the column pass was done by socs_cols_kernel into Yt [B][nk][W][G]
+// (column-major per entry); the block loads its GR rows of every entry from
+// there instead of recomputing every column (grids of 512 points and up,
+// where a block holds only GR of the G rows).
+template <int P, int Q, int GR, int NZ, int NTH, int BK = 0, bool PRE = false>
 __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restrict__ M,
                                                           const void* __restrict__ bank,
                                                           const float2* __restrict__ kscale,
                                                           const float* __restrict__ scales,
                                                           int nk, int Nx, float* __restrict__ out,
                                                           int shift,
-                                                          const float2* __restrict__ tw) {
+                                                          const float2* __restrict__ tw,
+                                                          const float2* __restrict__ Yt) {
     using Geo = Fft2Geom<P, Q>;
     constexpr int G = Geo::G;
     constexpr int T = Geo::T;
@@ -358,7 +363,16 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     for (int k = 0; k < nk; k++) {
         const long kbase = (long)k * W * W;
         const float lam = scales[k];
-        for (int c0 = 0; c0 < W; c0 += SLOTS) {
+        if constexpr (PRE) {
+            // rows g0 .. g0 + GR - 1 of entry k: GR consecutive values per column
+            const float2* yk = Yt + ((b * nk + k) * (long)W) * G + g0;
+            for (int i = tid; i < GR * W; i += NTH) {
+                const int col = i / GR, r = i - col * GR;
+                Y[r * W + col] = yk[(long)col * G + r];
+            }
+            __syncthreads();
+        }
+        for (int c0 = 0; c0 < (PRE ? 0 : W); c0 += SLOTS) {
             const int col = c0 + slot;
             const bool act = col < W;
             fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
@@ -1623,6 +1637,70 @@ int socs_rows_per_block(int G, int Nx) {
     return 0;
 }
 
+// Column pass of the large-grid SOCS: one transform per (image b, entry k,
+// window column col): the inverse G-point FFT over ky of K_k M (W nonzero
+// inputs at indices 0..W-1, as socs_accum2), written to Yt[b][k][col][0..G).
+template <int P, int Q, int NZ, int NTH, int BK>
+__global__ __launch_bounds__(NTH) void socs_cols_kernel(const float2* __restrict__ M,
+                                                        const void* __restrict__ bank,
+                                                        const float2* __restrict__ kscale, int nk,
+                                                        int Nx, float2* __restrict__ Yt) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int G = Geo::G;
+    constexpr int T = Geo::T;
+    constexpr int SLOTS = NTH / T;
+    extern __shared__ float2 lds[];
+    float2* twG = lds;
+    float2* xbuf = lds + G;
+    const int W = 2 * Nx + 1;
+    const int tid = threadIdx.x;
+    const int slot = tid / T, t = tid - slot * T;
+    const int col = blockIdx.x * SLOTS + slot;
+    const int k = blockIdx.y;
+    const long b = blockIdx.z;
+    const bool act = col < W;
+    const float2* Mb = M + b * (long)W * W;
+    const long kbase = (long)k * W * W;
+    float2* xb = xbuf + slot * Geo::XBUF;
+    fill_twiddles<G>(twG, tid, NTH);
+    __syncthreads();
+    fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+        float2 v = make_float2(0.f, 0.f);
+        if (act && n < W) {
+            const int j = n * W + col;
+            v = cmul(Bank<BK>::ld(bank, kscale, kbase + j, k), Mb[j]);
+        }
+        return v;
+    });
+    __syncthreads();
+    float2* yo = Yt + ((b * nk + k) * (long)W + (act ? col : 0)) * G;
+    fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) {
+        if (act) yo[g] = val;
+    });
+}
+
+template <int P, int Q, int GR, int NZ, int NTH, int BK>
+static int launch_socs2_pre(const float2* M, const void* bank, const float2* kscale,
+                            const float* scales, int nk, int Nx, int B, float* out, int shift,
+                            const float2* tw, float2* Yt, hipStream_t st) {
+    using Geo = Fft2Geom<P, Q>;
+    constexpr int G = P * Q;
+    constexpr int SLOTS = NTH / Geo::T;
+    const int W = 2 * Nx + 1;
+    const size_t lds_c = (size_t)(G + SLOTS * Geo::XBUF) * sizeof(float2);
+    const size_t lds_r = (size_t)(G + SLOTS * Geo::XBUF + GR * W) * sizeof(float2);
+    if (lds_c > 160 * 1024 || lds_r > 160 * 1024) return LITHO_EARG;
+    auto kc = socs_cols_kernel<P, Q, NZ, NTH, BK>;
+    auto kr = socs_accum2_kernel<P, Q, GR, NZ, NTH, BK, true>;
+    LITHO_TRY(allow_full_lds((const void*)kc));
+    LITHO_TRY(allow_full_lds((const void*)kr));
+    hipLaunchKernelGGL(kc, dim3(cdiv(W, SLOTS), nk, B), dim3(NTH), lds_c, st, M, bank, kscale, nk,
+                       Nx, Yt);
+    hipLaunchKernelGGL(kr, dim3((unsigned)((size_t)B * (G / GR))), dim3(NTH), lds_r, st, M, bank,
+                       kscale, scales, nk, Nx, out, shift, tw, (const float2*)Yt);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 template <int P, int Q, int GR, int NZ, int NTH, int BK>
 static int launch_socs2_bk(const float2* M, const void* bank, const float2* kscale,
                            const float* scales, int nk, int Nx, int B, float* out, int shift,
@@ -1637,7 +1715,7 @@ static int launch_socs2_bk(const float2* M, const void* bank, const float2* ksca
     LITHO_TRY(allow_full_lds((const void*)kern));
     unsigned grid = (unsigned)((size_t)B * (G / GR));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NTH), lds, st, M, bank, kscale, scales, nk, Nx, out,
-                       shift, tw);
+                       shift, tw, (const float2*)nullptr);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
@@ -1646,7 +1724,27 @@ static int launch_socs2_bk(const float2* M, const void* bank, const float2* ksca
 template <int P, int Q, int GR, int NZ, int NTH>
 static int launch_socs2(const float2* M, const void* bank, const float2* kscale, int bank_fmt,
                         const float* scales, int nk, int Nx, int B, float* out, int shift,
-                        const float2* tw, hipStream_t st) {
+                        const float2* tw, float2* Yt, hipStream_t st) {
+    if constexpr (P * Q >= 512) {
+        if (Yt) {
+            switch (bank_fmt) {
+                case 0:
+                    return launch_socs2_pre<P, Q, GR, NZ, NTH, 0>(M, bank, kscale, scales, nk, Nx,
+                                                                  B, out, shift, tw, Yt, st);
+                case 1:
+                    return launch_socs2_pre<P, Q, GR, NZ, NTH, 1>(M, bank, kscale, scales, nk, Nx,
+                                                                  B, out, shift, tw, Yt, st);
+                case 2:
+                    return launch_socs2_pre<P, Q, GR, NZ, NTH, 2>(M, bank, kscale, scales, nk, Nx,
+                                                                  B, out, shift, tw, Yt, st);
+                case 3:
+                    return launch_socs2_pre<P, Q, GR, NZ, NTH, 3>(M, bank, kscale, scales, nk, Nx,
+                                                                  B, out, shift, tw, Yt, st);
+                default:
+                    return LITHO_EARG;
+            }
+        }
+    }
     if (bank_fmt == 0)
         return launch_socs2_bk<P, Q, GR, NZ, NTH, 0>(M, bank, kscale, scales, nk, Nx, B, out,
                                                      shift, tw, st);
@@ -1664,9 +1762,13 @@ static int launch_socs2(const float2* M, const void* bank, const float2* kscale,
     return LITHO_EARG;
 }
 
+size_t socs_pre_bytes(int G, int Nx, int nk, int B) {
+    return G >= 512 ? (size_t)B * nk * (2 * Nx + 1) * G * sizeof(float2) : 0;
+}
+
 int launch_socs_accum(int G, const float2* M, const void* bank, const float2* kscale,
                       int bank_fmt, const float* scales, int nk, int Nx, int B, float* out,
-                      int shift, const float2* tw, hipStream_t st) {
+                      int shift, const float2* tw, hipStream_t st, float2* Yt) {
     if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 3 ||
         (bank_fmt && !kscale))
         return LITHO_EARG;
@@ -1675,7 +1777,7 @@ int launch_socs_accum(int G, const float2* M, const void* bank, const float2* ks
 #define LITHO_S2(p, q, gr, nz, nth)                                                        \
     if (G == p * q && W <= nz * q)                                                         \
         return launch_socs2<p, q, gr, nz, nth>(M, bank, kscale, bank_fmt, scales, nk, Nx, B, out, \
-                                               shift, tw, st);
+                                               shift, tw, G >= 512 ? Yt : nullptr, st);
     LITHO_S2(4, 4, 16, 4, 64)
     LITHO_S2(4, 8, 32, 4, 128)
     LITHO_S2(8, 8, 64, 3, 512)

@@ -16,9 +16,14 @@ int launch_mask_cols(int N, const float2* R, float2* M, int B, int cx, int kymin
 int socs_rows_per_block(int G, int Nx);
 // window M [B][W][W] x bank [nk][W][W] -> intensity [B][G][G] (shifted if shift);
 // bank_fmt as launch_socs_band (16-bit formats on grids of 512 points and up)
+// With Yt (socs_pre_bytes bytes, grids of 512 points and up) the column pass
+// runs once per window column into Yt and the row blocks read their rows from
+// there; without it every row block repeats the column pass.
 int launch_socs_accum(int G, const float2* M, const void* bank, const float2* kscale,
                       int bank_fmt, const float* scales, int nk, int Nx, int B, float* out,
-                      int shift, const float2* tw, hipStream_t st);
+                      int shift, const float2* tw, hipStream_t st, float2* Yt = nullptr);
+// bytes of the column-pass buffer of launch_socs_accum (0 below 512 points)
+size_t socs_pre_bytes(int G, int Nx, int nk, int B);
 // FI second axis: P [B][G][band+1] -> U [B][N][band+1]; yalt: P rows came
 // from a centred (fftshifted) grid
 int launch_fi_cols(int G, int N, const float2* P, float2* U, int B, int band, int yalt,
