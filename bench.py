@@ -108,11 +108,13 @@ def cpu_setup_baseline(src, sigma, Nx, nk, sc_gpu, K_gpu):
 
 
 def mixed_size_bench(ctx, src, sigma, steps):
-    """Batching form: 512 clips with sizes drawn from {256, 512, 768, 1024}^2
-    at 4 nm, one K=16 kernel set per size (four TCC builds), all clips in one
-    graph-captured step over four buckets; images/s and per-clip latency (a
-    clip's latency is the device time from the step's start to the end of its
-    bucket's last launch, all clips submitted together)."""
+    """Batching form: 512 distinct clips with sizes drawn from {256, 512, 768,
+    1024}^2 at 4 nm, one K=16 kernel set per size (four TCC builds), all clips
+    in one graph-captured step over four buckets, in FP32 and in the compact
+    form (1-bit masks, matrix-core band DFTs, fp16 storage and images);
+    images/s and per-clip latency (a clip's latency is the device time from the
+    step's start to the end of its bucket's last launch, all clips submitted
+    together)."""
     rng = np.random.default_rng(2024)
     sizes = rng.choice([256, 512, 768, 1024], size=512)
     out = {"clips": 512, "K": 16, "counts": {}, "plans": {}}
@@ -126,17 +128,19 @@ def mixed_size_bench(ctx, src, sigma, steps):
                                          Nx, Nx, 16)
         bank = ctx.bank(K, sc, Nx, Nx)
         banks.append(bank)
-        uniq = np.stack([L.manhattan_mask(N, 5000 + 100 * N + i, WMIN, WMAX, FILL)
-                         for i in range(8)])
-        masks = uniq[np.arange(cnt) % 8]
-        dm, dm8, di = ctx.alloc(masks.nbytes), ctx.alloc(masks.size), ctx.alloc(masks.nbytes)
+        masks = np.stack([L.manhattan_mask(N, 5000 + 10000 * N + i, WMIN, WMAX, FILL)
+                          for i in range(cnt)])  # every clip distinct
+        bits = np.packbits(masks.astype(bool), axis=-1, bitorder="little")
+        dm, dm1, di = ctx.alloc(masks.nbytes), ctx.alloc(bits.nbytes), ctx.alloc(masks.nbytes)
         ctx.h2d(dm, masks)
-        ctx.h2d(dm8, np.rint(masks * 255).astype(np.uint8))
-        dev += [dm, dm8, di]
+        ctx.h2d(dm1, bits)
+        dev += [dm, dm1, di]
         bl.append((bank, N, cnt, dm, di))
-        bl8.append((bank, N, cnt, dm8, di))
+        bl8.append((bank, N, cnt, dm1, di))
+        del masks
     out["tcc_builds_s"] = round(time.perf_counter() - t0, 3)
-    for name, prec, b in (("fp32", L.LITHO_PREC_FP32, bl), ("compact", L.LITHO_PREC_COMPACT, bl8)):
+    compact = (L.LITHO_PREC_COMPACT & ~L.LITHO_PREC_MASK_U8) | L.LITHO_PREC_MASK_1BIT
+    for name, prec, b in (("fp32", L.LITHO_PREC_FP32, bl), ("compact", compact, bl8)):
         t0 = time.perf_counter()
         ctx.autotune(b, prec, reps=3)
         tune_s = time.perf_counter() - t0
@@ -298,7 +302,7 @@ def main():
                     help="conv grid: auto (band grid 144 at Nx=34) or the reference nextPow2 grid")
     ap.add_argument("--chunks", type=int, default=1,
                     help="sub-batches of the pipelined (chosen) form")
-    ap.add_argument("--surrogate-s", type=float, default=60.0,
+    ap.add_argument("--surrogate-s", type=float, default=100.0,
                     help="FNO training budget in seconds (0: skip the learned form)")
     ap.add_argument("--no-explicit", action="store_true",
                     help="skip the explicit n x n TCC + top-K eigensolve timing")
@@ -447,8 +451,8 @@ def main():
     achieved = path_bytes * B / (step_dev_ms * 1e-3) / 1e9
     dom_gbps = alg[dom] / (stage_acc[dom] * 1e-3) / 1e9
 
-    # ---- chosen form: 8-bit masks, fp16 band DFTs on the matrix cores, fp16
-    # kernel bank, fp16 images; the step captured as one hipGraph -------------
+    # ---- chosen form: 1-bit (or 8-bit) masks, fp16 band DFTs on the matrix
+    # cores, fp16 kernel bank, fp16 images; the step captured as one hipGraph ---
     def timed_form(precision, dm, bk=bank, graph=True):
         for _ in range(max(2, args.warmup)):
             ctx.socs_image_batch_pipelined(bk, dm, B, N, d_img2, precision, args.chunks, graph)
@@ -472,11 +476,20 @@ def main():
     bf16x2_prec = L.LITHO_PREC_COMPACT & ~L.LITHO_PREC_BANK_FP16 | L.LITHO_PREC_BANK_BF16X2
     bf16x2_bank = timed_form(bf16x2_prec, d_mask8, bank16)
     bf16x2_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
-    chosen = timed_form(L.LITHO_PREC_COMPACT, d_mask8, bank16)
-    log(f"graph FP32 {pipe_fp32:.1f} images/s; fp16 storage {mixed:.1f}; bf16 bank {bf16_bank:.1f}; "
-        f"chosen form (split-BF16 kernels, u8 masks, matrix-core DFTs, fp16 window, bank and images) "
-        f"{chosen:.1f} images/s = {chosen / images_per_s:.2f}x plain")
+    compact_u8 = timed_form(L.LITHO_PREC_COMPACT, d_mask8, bank16)
+    u8_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
+    # chosen form: the same with the binary masks one bit per pixel (lossless
+    # for 0/1 masks, the workload's masks)
+    bits = np.packbits(masks.astype(bool), axis=-1, bitorder="little")
+    d_mask1 = ctx.alloc(bits.nbytes)
+    ctx.h2d(d_mask1, bits)
+    chosen_prec = (L.LITHO_PREC_COMPACT & ~L.LITHO_PREC_MASK_U8) | L.LITHO_PREC_MASK_1BIT
+    chosen = timed_form(chosen_prec, d_mask1, bank16)
     chosen_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float16), d_img2).astype(np.float32)
+    log(f"graph FP32 {pipe_fp32:.1f} images/s; fp16 storage {mixed:.1f}; bf16 bank {bf16_bank:.1f}; "
+        f"u8 masks {compact_u8:.1f}; chosen form (split-BF16 kernels, 1-bit masks, matrix-core "
+        f"DFTs, fp16 window, bank and images) {chosen:.1f} images/s = "
+        f"{chosen / images_per_s:.2f}x plain")
 
     # measured HBM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated against a
     # device copy) for this pipeline, scaled to the batch
@@ -500,13 +513,14 @@ def main():
     mixed_err = None
     bf16_err = None
     bf16x2_err = None
+    u8_err = None
     if not args.no_cpu:
         # the box's CPU share for one GPU is 16 threads (OMP_NUM_THREADS there)
         threads = min(16, os.cpu_count() or 1)
         rate, n_done, dt, errs = cpu_baseline(masks[:8], K, sc, Nx, threads, args.cpu_budget,
                                               [out_imgs, chosen_imgs, mixed_imgs, bf16_imgs,
-                                               bf16x2_imgs])
-        max_err, chosen_err, mixed_err, bf16_err, bf16x2_err = errs
+                                               bf16x2_imgs, u8_imgs])
+        max_err, chosen_err, mixed_err, bf16_err, bf16x2_err, u8_err = errs
         rate1, n1, dt1, _ = cpu_baseline(masks[:1], K, sc, Nx, 1, args.cpu_budget / 2, [])
         cpu = {"value": round(rate, 4), "unit": "images/s", "cores": threads, "kind": "port",
                "sample": f"{n_done} clip passes over 8 of the same 1024^2 clips, Module 3 + "
@@ -561,18 +575,21 @@ def main():
         "max_rel_err_vs_fp64": max_err,
         "chosen_form": {
             "form": "kernels from the split-BF16 Gram contraction (FP32 accumulate) and FP64 "
-                    "eigensolve; batched mixed precision: masks as 8-bit levels, mask-row and "
-                    "image-row band DFTs as fp16 matrix products with FP32 accumulate (image side "
-                    "split hi+lo below half the band), fp16 row-DFT window, fp16 kernel bank with "
-                    "FP32 scales, FP32 column FFTs and SOCS, fp16 images; the step captured as "
-                    "one hipGraph",
-            "precision_flags": L.LITHO_PREC_COMPACT,
+                    "eigensolve; batched mixed precision: binary masks one bit per pixel, "
+                    "mask-row and image-row band DFTs as fp16 matrix products with FP32 "
+                    "accumulate (image side split hi+lo below half the band), fp16 row-DFT "
+                    "window, fp16 kernel bank with FP32 scales, FP32 column FFTs and SOCS, fp16 "
+                    "images; the step captured as one hipGraph",
+            "precision_flags": chosen_prec,
             "value": round(chosen, 2), "unit": "images/s",
             "ratio_vs_plain": round(chosen / images_per_s, 3),
             "max_rel_err_vs_fp64": chosen_err,
-            "roofline_frac": round(3 * N * N * chosen / 1e9 / HBM_PEAK_GBPS, 4),
-            "roofline_scope": "algorithmic bytes of this form = 3 N^2 per image (u8 mask read + "
-                              "fp16 image write)",
+            "roofline_frac": round((N * N / 8 + 2 * N * N) * chosen / 1e9 / HBM_PEAK_GBPS, 4),
+            "roofline_scope": "algorithmic bytes of this form = N^2 / 8 + 2 N^2 per image "
+                              "(1-bit mask read + fp16 image write)",
+            "u8_masks_value": round(compact_u8, 2),
+            "u8_masks_max_rel_err_vs_fp64": u8_err,
+            "u8_masks_flags": L.LITHO_PREC_COMPACT,
             "graph_fp32_value": round(pipe_fp32, 2),
             "fp16_storage_value": round(mixed, 2),
             "fp16_storage_max_rel_err_vs_fp64": mixed_err,
@@ -603,6 +620,7 @@ def main():
     ctx.free(d_img)
     ctx.free(d_img2)
     ctx.free(d_mask8)
+    ctx.free(d_mask1)
     ctx.close()
 
 

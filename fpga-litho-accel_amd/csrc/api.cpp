@@ -1440,8 +1440,12 @@ static BandPlan plan_of(litho_ctx* c, int N, int Nx, int nk, int prec) {
 static bool prec_ok(int prec) {
     const int known = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 |
                       LITHO_PREC_MASK_U8 | LITHO_PREC_IMAGE_FP16 | LITHO_PREC_DFT_MFMA |
-                      LITHO_PREC_BANK_BF16X2;
+                      LITHO_PREC_BANK_BF16X2 | LITHO_PREC_MASK_1BIT;
     if (prec & ~known) return false;
+    // one input format; the bit-packed masks go through the matrix-core rows only
+    if ((prec & LITHO_PREC_MASK_1BIT) &&
+        ((prec & LITHO_PREC_MASK_U8) || !(prec & LITHO_PREC_DFT_MFMA)))
+        return false;
     const int banks = prec & (LITHO_PREC_BANK_BF16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_BANK_BF16X2);
     if (banks & (banks - 1)) return false;  // at most one bank format
     return true;
@@ -1506,6 +1510,7 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
     const bool mfma = (prec & LITHO_PREC_DFT_MFMA) != 0;
     const bool u8 = (prec & LITHO_PREC_MASK_U8) != 0;
+    const int mfmt = (prec & LITHO_PREC_MASK_1BIT) ? 2 : (u8 ? 1 : 0);
     const bool out16 = (prec & LITHO_PREC_IMAGE_FP16) != 0;
     const int Nx = bank->Nx, W = 2 * Nx + 1, band = 2 * Nx;
     const int bfmt = (prec & LITHO_PREC_BANK_BF16)     ? 1
@@ -1526,7 +1531,7 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
         ti = b->second.p;
     }
     if (mfma)
-        LITHO_TRY(launch_mask_rows_mfma(N, d_mask, u8, tm, w.R, B, Nx + 1, mixed, st));
+        LITHO_TRY(launch_mask_rows_mfma(N, d_mask, mfmt, tm, w.R, B, Nx + 1, mixed, st));
     else
         LITHO_TRY(launch_rows_dft_t(N, d_mask, w.R, (long)B * N, 0, Nx + 1, mixed, pl.lean_mem, u8,
                                     st));
@@ -1651,7 +1656,10 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
     const BandPlan tuned = plan_of(c, N, Nx, bank->nk, precision);
     auto enqueue = [&]() -> int {
         const size_t NN = (size_t)N * N;
-        const size_t mb = (precision & LITHO_PREC_MASK_U8) ? 1 : sizeof(float);
+        // mask bytes per 8 pixels
+        const size_t mb8 = (precision & LITHO_PREC_MASK_1BIT) ? 1
+                           : (precision & LITHO_PREC_MASK_U8) ? 8
+                                                              : 8 * sizeof(float);
         const size_t ib = (precision & LITHO_PREC_IMAGE_FP16) ? 2 : sizeof(float);
         const size_t W = 2 * Nx + 1, band = 2 * Nx;
         hipStream_t s0 = c->stream;
@@ -1667,7 +1675,7 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
             wk.C = (float2*)w[0].C + (size_t)b0 * (2 * band + 1) * (band + 1);
             BandPlan pl = tuned;
             if (chunks > 1) pl.lean_mem = true;
-            LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN * mb, nb, N,
+            LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN / 8 * mb8, nb, N,
                                    (char*)d_image + (size_t)b0 * NN * ib, precision, pl, wk, st));
         }
         for (int i = 1; i < ns; i++) {

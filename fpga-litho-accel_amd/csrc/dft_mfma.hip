@@ -70,6 +70,7 @@ __device__ __forceinline__ unsigned f2_to_h2(float a, float b) {
 }
 
 constexpr int ROWS_PER_BLOCK = 256;  // mask side: rows per block
+constexpr int BITS_MAXCH = 8;        // bit-packed masks: up to 1024 pixels per row
 constexpr int KCH = 128;             // mask-side K (pixel) chunk staged in LDS
 constexpr int XCH = 64;              // image-side output columns per LDS chunk
 // image side: row tiles per wave (32 rows; 64 rows measured 2 % slower with
@@ -86,10 +87,20 @@ constexpr int IMAGE_NTH = 512;
 // FFT row kernel's transposed output).  NG = ceil(count / 16) frequency groups,
 // each as a real-part and an imaginary-part column tile.
 // ---------------------------------------------------------------------------
-template <int NG, int RTW, bool U8, bool H16>
+// one bit per pixel, pixel 8j + i at bit i of byte j: the two pixels at
+// bits sh, sh + 1 of x as exact fp16 0 / 1 values packed in a dword
+// (t * 0x8001 puts bit 0 at 0 and bit 1 at 16; times 0x3C00 = fp16 1.0)
+__device__ __forceinline__ unsigned bitpair_to_h2(unsigned x, int sh) {
+    const unsigned t = (x >> sh) & 3u;
+    return ((t * 0x8001u) & 0x10001u) * 0x3C00u;
+}
+
+// MF: mask format, 0 f32, 1 u8 levels, 2 one bit per pixel
+template <int NG, int RTW, int MF, bool H16>
 __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_kernel(
     const void* __restrict__ mask_, const u4v* __restrict__ Fpk, void* __restrict__ R_, int N,
     int count, float oscale) {
+    constexpr bool U8 = MF == 1, BITS = MF == 2;
     constexpr int NT = 2 * NG;
     constexpr int NTH = ROWS_PER_BLOCK * 4 / RTW;  // 64 threads per 16 rows x RTW
     constexpr int RT = RTW;
@@ -122,9 +133,19 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
             raw[rt][1] = p[1];
         }
     };
-    // two register sets for the u8 fragments (ping-pong, no loop-carried
+    // one bit per pixel: the 32 pixels of lane group g in one dword per row
+    auto load_bits = [&](int c, unsigned (&raw)[RT]) {
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+            const long row = rbase + 16 * rt + r16;
+            raw[rt] = ((const unsigned*)mask_)[row * (N / 32) + c * (KCH / 32) + g];
+        }
+    };
+    // two register sets for the u8 / bit fragments (ping-pong, no loop-carried
     // copy): chunk c+1 loads into one set while chunk c computes from the other
     u4v rawA[RT][2], rawB[RT][2];
+    unsigned bitA[RT];
+    unsigned bitAll[BITS ? BITS_MAXCH : 1][RT];
     // table chunk c -> dst straight from global memory into LDS (16 bytes per
     // lane, the wave's 64 lanes contiguous), so no registers hold it
     auto load_table = [&](int c, u4v* dst) {
@@ -142,12 +163,14 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
     __builtin_assume(nch >= ROWS_PER_BLOCK / KCH);
     load_table(0, fsA);
     if (U8) load_u8(0, rawA);
+    if constexpr (BITS) load_bits(0, bitAll[0]);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    auto chunk = [&](int c, const u4v (&raw)[RT][2], const u4v* fb, u4v* fnext) {
+    auto chunk = [&](int c, const u4v (&raw)[RT][2], const unsigned (&bits)[RT], const u4v* fb,
+                     u4v* fnext) {
         if (c + 1 < nch) load_table(c + 1, fnext);
         u4v af[RT][4];
-        if (!U8) {
+        if (MF == 0) {
 #pragma unroll
             for (int rt = 0; rt < RT; rt++) {
                 const long row = rbase + 16 * rt + r16;
@@ -168,7 +191,11 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
             u4v as[RT];
 #pragma unroll
             for (int rt = 0; rt < RT; rt++)
-                as[rt] = U8 ? (u4v){u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1)], 0),
+                as[rt] = BITS ? (u4v){bitpair_to_h2(bits[rt], 8 * s),
+                                      bitpair_to_h2(bits[rt], 8 * s + 2),
+                                      bitpair_to_h2(bits[rt], 8 * s + 4),
+                                      bitpair_to_h2(bits[rt], 8 * s + 6)}
+                       : U8 ? (u4v){u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1)], 0),
                                     u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1)], 1),
                                     u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1) + 1], 0),
                                     u8pair_to_h2(raw[rt][s >> 1][2 * (s & 1) + 1], 1)}
@@ -189,12 +216,29 @@ __global__ __launch_bounds__(ROWS_PER_BLOCK * 4 / RTW, 2) void mask_rows_mfma_ke
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     };
-    for (int c = 0; c < nch; c += 2) {
-        if (U8 && c + 1 < nch) load_u8(c + 1, rawB);
-        chunk(c, rawA, fsA, fsB);
-        if (c + 1 >= nch) break;
-        if (U8 && c + 2 < nch) load_u8(c + 2, rawA);
-        chunk(c + 1, rawB, fsB, fsA);
+    if constexpr (BITS) {
+        // the whole row tile is N / 8 bytes: every chunk's bits are fetched
+        // here (chunk 0's already waited for), so the loop waits only on the
+        // table, which comes from L2
+#pragma unroll
+        for (int c = 1; c < BITS_MAXCH; c++)
+            if (c < nch) load_bits(c, bitAll[c]);
+#pragma unroll
+        for (int c = 0; c < BITS_MAXCH; c++) {
+            if (c >= nch) break;
+            if (c % 2 == 0)
+                chunk(c, rawA, bitAll[c], fsA, fsB);
+            else
+                chunk(c, rawA, bitAll[c], fsB, fsA);
+        }
+    } else {
+        for (int c = 0; c < nch; c += 2) {
+            if (U8 && c + 1 < nch) load_u8(c + 1, rawB);
+            chunk(c, rawA, bitA, fsA, fsB);
+            if (c + 1 >= nch) break;
+            if (U8 && c + 2 < nch) load_u8(c + 2, rawA);
+            chunk(c + 1, rawB, bitA, fsB, fsA);
+        }
     }
     // D fragment: rows 4g + i of the tile, column r16 = frequency 16 fg + r16
 #pragma unroll
@@ -451,30 +495,31 @@ std::vector<unsigned short> image_rows_table(int N, int nb) {
     return t;
 }
 
-int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, void* R, int B,
+int launch_mask_rows_mfma(int N, const void* mask, int mfmt, const void* table, void* R, int B,
                           int count, bool h16, hipStream_t st) {
-    if (N % ROWS_PER_BLOCK || N % KCH || B <= 0 || B > 65535 || count <= 0 || count > 80)
+    if (N % ROWS_PER_BLOCK || N % KCH || B <= 0 || B > 65535 || count <= 0 || count > 80 ||
+        mfmt < 0 || mfmt > 2 || (mfmt == 2 && N > BITS_MAXCH * KCH))
         return LITHO_EARG;
     const int NG = (count + 15) / 16;
     const dim3 grid(N / ROWS_PER_BLOCK, B);
-    const float osc = u8 ? 1.f / 255.f : 1.f;
+    const float osc = mfmt == 1 ? 1.f / 255.f : 1.f;
     const u4v* F = (const u4v*)table;
-#define LITHO_MR(ng, rtw)                                                                      \
-    if (NG == ng) {                                                                            \
-        constexpr int nth = ROWS_PER_BLOCK * 4 / rtw;                                          \
-        if (u8 && h16)                                                                         \
-            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, true, true>), grid, dim3(nth), 0, \
-                               st, mask, F, R, N, count, osc);                                 \
-        else if (u8)                                                                           \
-            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, true, false>), grid, dim3(nth), 0,\
-                               st, mask, F, R, N, count, osc);                                 \
-        else if (h16)                                                                          \
-            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, false, true>), grid, dim3(nth), 0,\
-                               st, mask, F, R, N, count, osc);                                 \
-        else                                                                                   \
-            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, false, false>), grid, dim3(nth),  \
-                               0, st, mask, F, R, N, count, osc);                              \
-        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
+#define LITHO_MR1(ng, rtw, mf)                                                                  \
+    if (mfmt == mf) {                                                                           \
+        if (h16)                                                                                \
+            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, mf, true>), grid, dim3(nth), 0, st, \
+                               mask, F, R, N, count, osc);                                      \
+        else                                                                                    \
+            hipLaunchKernelGGL((mask_rows_mfma_kernel<ng, rtw, mf, false>), grid, dim3(nth), 0,   \
+                               st, mask, F, R, N, count, osc);                                  \
+    }
+#define LITHO_MR(ng, rtw)                                                                       \
+    if (NG == ng) {                                                                             \
+        constexpr int nth = ROWS_PER_BLOCK * 4 / rtw;                                           \
+        LITHO_MR1(ng, rtw, 0)                                                                   \
+        LITHO_MR1(ng, rtw, 1)                                                                   \
+        LITHO_MR1(ng, rtw, 2)                                                                   \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                         \
     }
     LITHO_MR(1, 4)
     // 32 rows per wave from two frequency groups on (two 8-wave blocks per CU):
@@ -484,6 +529,7 @@ int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, v
     LITHO_MR(4, 2)
     LITHO_MR(5, 2)
 #undef LITHO_MR
+#undef LITHO_MR1
     return LITHO_EARG;
 }
 

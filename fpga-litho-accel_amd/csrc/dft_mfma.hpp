@@ -12,10 +12,11 @@ namespace litho {
 std::vector<unsigned short> mask_rows_table(int N, int kmin, int count);
 std::vector<unsigned short> image_rows_table(int N, int nb);
 
-// mask [B][N][N] (u8 levels / 255, or f32) -> R [B][count][N] (half2 if h16, else
-// float2): window kx in [kmin, kmin + count) of each row's DFT, (-1)^kx applied.
-// N a multiple of 256, count <= 80.
-int launch_mask_rows_mfma(int N, const void* mask, bool u8, const void* table, void* R, int B,
+// mask [B][N][N] -> R [B][count][N] (half2 if h16, else float2): window kx in
+// [kmin, kmin + count) of each row's DFT, (-1)^kx applied.  mfmt: 0 f32 masks,
+// 1 u8 levels (mask = level / 255), 2 one bit per pixel (pixel 8j + i at bit i
+// of byte j of the row, N / 8 bytes per row).  N a multiple of 256, count <= 80.
+int launch_mask_rows_mfma(int N, const void* mask, int mfmt, const void* table, void* R, int B,
                           int count, bool h16, hipStream_t st);
 // U [B][nb][N] float2 (kx = 0..nb-1 of each image row) -> image [B][N][N]
 // (fp16 if out16, else f32), the c2r sum with the DC imaginary part dropped.

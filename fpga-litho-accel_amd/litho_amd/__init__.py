@@ -53,6 +53,7 @@ LITHO_PREC_MASK_U8 = 8
 LITHO_PREC_IMAGE_FP16 = 16
 LITHO_PREC_DFT_MFMA = 32
 LITHO_PREC_BANK_BF16X2 = 64
+LITHO_PREC_MASK_1BIT = 128
 LITHO_PREC_MIXED = LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16
 LITHO_PREC_COMPACT = (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8
                       | LITHO_PREC_DFT_MFMA | LITHO_PREC_IMAGE_FP16)

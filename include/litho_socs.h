@@ -256,8 +256,10 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
  *    products with FP32 accumulation on the matrix cores (the image side
  *    splits its input into hi + lo fp16 parts); the other stages stay FP32;
  *  - MASK_U8 (requires DFT_MFMA): d_mask holds uint8 levels, mask = level/255
- *    (the PNG -> BIN convention); IMAGE_FP16 (requires DFT_MFMA): d_image
- *    receives fp16 values.
+ *    (the PNG -> BIN convention); MASK_1BIT (requires DFT_MFMA, not with
+ *    MASK_U8): d_mask holds binary masks one bit per pixel, N / 8 bytes per
+ *    row, pixel 8j + i at bit i of byte j (numpy packbits, bitorder little);
+ *    IMAGE_FP16 (requires DFT_MFMA): d_image receives fp16 values.
  * LITHO_PREC_MIXED is the storage-only set that meets 1e-3 max relative error
  * against FP64; LITHO_PREC_COMPACT the matrix-core set with 8-bit masks, an
  * fp16 row-DFT window and fp16 images (6-7.5e-4 at 1024^2, K=24).
@@ -273,6 +275,7 @@ const char* litho_stage_name(litho_ctx* ctx, int i);
 #define LITHO_PREC_IMAGE_FP16 16
 #define LITHO_PREC_DFT_MFMA 32
 #define LITHO_PREC_BANK_BF16X2 64
+#define LITHO_PREC_MASK_1BIT 128
 #define LITHO_PREC_MIXED (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16)
 #define LITHO_PREC_COMPACT                                                              \
     (LITHO_PREC_INTER_FP16 | LITHO_PREC_BANK_FP16 | LITHO_PREC_MASK_U8 | LITHO_PREC_DFT_MFMA | \
@@ -292,7 +295,7 @@ typedef struct {
     const litho_bank* bank;  /* kernel set of this clip size */
     int N;                   /* clip size in pixels */
     int count;               /* clips in the bucket (0 = skip) */
-    const void* d_masks;     /* count x N x N: f32, or uint8 levels with MASK_U8 */
+    const void* d_masks;     /* count x N x N: f32, uint8 levels (MASK_U8) or bits (MASK_1BIT) */
     void* d_images;          /* count x N x N: f32, or fp16 with IMAGE_FP16 */
 } LithoBucket;
 int litho_socs_image_buckets(litho_ctx* ctx, const LithoBucket* buckets, int nbuckets,

@@ -123,6 +123,36 @@ def test_buckets_matrix_core_form(gpu, orc, setup):
         assert e < 1e-3
 
 
+def test_buckets_one_bit_masks(gpu, orc, setup):
+    """The compact form on binary masks packed one bit per pixel equals the same
+    form on 8-bit levels to one fp16 step, and FP64 to 1e-3."""
+    ctx, buckets, host = setup
+    print("[case] mixed-size buckets, 1-bit masks", flush=True)
+    prec = (gpu.LITHO_PREC_COMPACT & ~gpu.LITHO_PREC_MASK_U8) | gpu.LITHO_PREC_MASK_1BIT
+    dev, bl1 = [], []
+    for (b, N, c, dm, di, dm8), (_, _, _, _, masks) in zip(buckets, host):
+        bits = np.packbits(masks.astype(bool), axis=-1, bitorder="little")
+        d1 = ctx.alloc(bits.nbytes)
+        ctx.h2d(d1, bits)
+        dev.append(d1)
+        bl1.append((b, N, c, d1, di))
+    ctx.socs_image_buckets(bl1, prec, True)
+    ctx.sync()
+    got = [ctx.d2h(np.zeros((c, N, N), np.float16), di).astype(np.float32)
+           for _, N, c, _, di in bl1]
+    bl8 = [(b, N, c, dm8, di) for b, N, c, dm, di, dm8 in buckets]
+    ctx.socs_image_buckets(bl8, gpu.LITHO_PREC_COMPACT, False)
+    ctx.sync()
+    for (N, Nx, sc, K, masks), (_, _, c, _, di), g in zip(host, bl8, got):
+        via_u8 = ctx.d2h(np.zeros((c, N, N), np.float16), di).astype(np.float32)
+        assert max_rel(g, via_u8) < 1e-3
+        e = max_rel(g[0], oracle_image(orc, N, Nx, sc, K, masks[0]))
+        print(f"[buckets 1-bit N={N}] max rel err vs FP64 {e:.2e}", flush=True)
+        assert e < 1e-3
+    for d in dev:
+        ctx.free(d)
+
+
 def test_bucket_argument_checks(gpu, setup):
     ctx, buckets, _ = setup
     b, N, c, dm, di, _ = buckets[0]
@@ -131,7 +161,7 @@ def test_bucket_argument_checks(gpu, setup):
     with pytest.raises(gpu.LithoError):
         ctx.socs_image_buckets([(b, N, -1, dm, di)], 0, False)
     with pytest.raises(gpu.LithoError):
-        ctx.socs_image_buckets([(b, N, c, dm, di)], 128, False)  # unknown precision flag
+        ctx.socs_image_buckets([(b, N, c, dm, di)], 256, False)  # unknown precision flag
     with pytest.raises(gpu.LithoError):  # two bank formats at once
         ctx.socs_image_buckets([(b, N, c, dm, di)],
                                gpu.LITHO_PREC_BANK_BF16 | gpu.LITHO_PREC_BANK_FP16, False)

@@ -325,7 +325,8 @@ def test_pipelined_batch_forms(gpu, orc, base_source, precision, graph):
                                     (512, 8 | 32 | 1 | 4), (256, 8 | 32), (1024, 8 | 4 | 16),
                                     (768, 8 | 16), (512, 8 | 1 | 4 | 16),
                                     (1024, 8 | 32 | 1 | 4 | 16), (512, 8 | 32 | 1 | 4 | 16),
-                                    (1024, 8 | 32 | 1 | 64 | 16)])
+                                    (1024, 8 | 32 | 1 | 64 | 16), (1024, 128 | 32 | 1 | 4 | 16),
+                                    (512, 128 | 32 | 1 | 4 | 16), (768, 128 | 32 | 16)])
 def test_matrix_core_forms(gpu, orc, base_source, N, prec):
     """u8 / f32 masks and f32 / fp16 images, with the band DFTs on the fp16
     matrix cores (flag 32) or as FP32 FFTs, stay within 1e-3 of FP64 on binary
@@ -338,10 +339,16 @@ def test_matrix_core_forms(gpu, orc, base_source, N, prec):
                                        K_n)
     B = 3
     u8 = bool(prec & gpu.LITHO_PREC_MASK_U8)
+    bits = bool(prec & gpu.LITHO_PREC_MASK_1BIT)
     out16 = bool(prec & gpu.LITHO_PREC_IMAGE_FP16)
     rng = np.random.default_rng(N + prec)
     masks = np.stack([gpu.manhattan_mask(N, 70 + i) for i in range(B)])
-    if u8:
+    if bits:
+        # binary masks, one bit per pixel (pixel 8j + i at bit i of byte j)
+        masks[2] = (rng.random((N, N)) < 0.5).astype(np.float32)  # a random pattern too
+        host_in = np.packbits(masks.astype(bool), axis=-1, bitorder="little")
+        assert host_in.shape == (B, N, N // 8)
+    elif u8:
         levels = (masks * 255).astype(np.uint8)
         # last clip: random grey levels 0..255 (mask = level / 255)
         levels[2] = rng.integers(0, 256, (N, N), dtype=np.uint8)
@@ -370,6 +377,19 @@ def test_matrix_core_forms(gpu, orc, base_source, N, prec):
             print(f"[matrix-core N={N} prec={prec} clip {i}] max rel err vs FP64 {e:.3e}",
                   flush=True)
             assert e < 1e-3
+        if bits:
+            # the same binary masks as u8 levels: the same images to FP32 round-off
+            lv = (masks * 255).astype(np.uint8)
+            du = ctx.alloc(lv.nbytes)
+            ctx.h2d(du, lv)
+            p8 = (prec & ~gpu.LITHO_PREC_MASK_1BIT) | gpu.LITHO_PREC_MASK_U8
+            ctx.socs_image_batch_pipelined(bank, du, B, N, di, p8, 1, False)
+            ctx.sync()
+            via_u8 = ctx.d2h(np.zeros((B, N, N), odt), di).astype(np.float32)
+            d = max_rel(eager.astype(np.float32), via_u8)
+            print(f"[matrix-core N={N}] bits vs u8 levels {d:.3e}", flush=True)
+            assert d < (1e-3 if out16 else 1e-5)
+            ctx.free(du)
         ctx.free(dm)
         ctx.free(di)
         bank.close()
@@ -383,7 +403,7 @@ def test_matrix_core_flag_rules(gpu, clip512):
     try:
         bank = ctx.bank(clip512["K"], clip512["sc"], clip512["Nx"], clip512["Nx"])
         d = ctx.alloc(4 * 512 * 512)
-        for bad in (128, 256, 2 | 4, 2 | 4 | 32, 4 | 64, 2 | 64):
+        for bad in (256, 512, 2 | 4, 2 | 4 | 32, 4 | 64, 2 | 64, 128, 128 | 8 | 32, 128 | 1):
             with pytest.raises(gpu.LithoError):
                 ctx.socs_image_batch_pipelined(bank, d, 1, 512, d, bad, 1, False)
         ctx.free(d)
