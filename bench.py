@@ -258,14 +258,17 @@ def surrogate_bench(ctx, src, sigma, train_s):
         return B * reps / (time.perf_counter() - t)
 
     inf = rate(lambda: model.forward(vm, 256, d_o), 256, 10)
+    inf32 = rate(lambda: model.forward_fp32(vm, 256, d_o), 256, 5)
     socs = rate(lambda: ctx.socs_image_batch(bank, vm, 256, N, d_o), 256, 20)
     srv = fno.SurrogateServer(ctx, model, bank, tol=1e-2, verify_every=16)
     served = rate(lambda: srv.serve(vm, nte, d_o), nte, 3)
     srv_all = fno.SurrogateServer(ctx, model, bank, tol=1e-2, verify_every=1)
     served_all = rate(lambda: srv_all.serve(vm, nte, d_o), nte, 3)
     out = {"form": "Fourier neural operator, width 32, 16x16 retained modes per ky sign, 4 layers, "
-                   "GELU, bf16 weights read with FP32 accumulate; trained by its own HIP "
-                   "forward/backward kernels (Adam, MSE) on SOCS images of seeded Manhattan masks",
+                   "GELU, bf16 weights; trained by its own HIP forward/backward kernels (Adam, MSE, "
+                   "FP32 arithmetic) on SOCS images of seeded Manhattan masks; inference on the "
+                   "matrix cores (one fused bf16-MFMA kernel per Fourier layer, activations and "
+                   "tables split into bf16 hi + lo, FP32 accumulate)",
            "N": N, "K_socs": 16, "train_pairs": ntr, "data_s": round(data_s, 2),
            "train_s": round(tr_s, 1), "steps": int(len(losses)), "batch": batch,
            "samples_per_s": round(len(losses) * batch / tr_s, 1),
@@ -274,6 +277,7 @@ def surrogate_bench(ctx, src, sigma, train_s):
            "max_rel_err": {"median": float(np.median(e)), "p90": float(np.percentile(e, 90)),
                            "max": float(e.max()), "frac_le_1e-2": float((e <= 1e-2).mean())},
            "inference_images_per_s": round(inf, 1),
+           "inference_fp32_vector_images_per_s": round(inf32, 1),
            "served_images_per_s": round(served, 1),
            "served_1_in_16": srv.stats(),
            "served_all_verified_images_per_s": round(served_all, 1),
@@ -282,7 +286,8 @@ def surrogate_bench(ctx, src, sigma, train_s):
            "inference_vs_socs": round(inf / socs, 5)}
     log(f"surrogate: {len(losses)} steps in {tr_s:.1f}s, held-out max rel err median "
         f"{np.median(e):.2e} p90 {np.percentile(e, 90):.2e} max {e.max():.2e}; inference "
-        f"{inf:.0f} images/s vs SOCS {socs:.0f} images/s at 256^2")
+        f"{inf:.0f} images/s (FP32 vector kernels {inf32:.0f}) vs SOCS {socs:.0f} images/s at "
+        f"256^2")
     for p_ in (tm, ti, vm, vi, d_o):
         ctx.free(p_)
     model.close()

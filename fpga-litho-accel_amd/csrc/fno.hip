@@ -129,7 +129,9 @@ __global__ __launch_bounds__(256) void dftx_kernel(const float* __restrict__ in,
     out[(r0 + r) * MX + kx] = make_float2(ar * s, ai * s);
 }
 
-// planes [P][N][MX] -> [P][MY][MX]: sum_y in[y][kx] e^{-2 pi i ky y / N}
+// planes [P][N][MX] -> [P][MY][MX]: sum_y in[y][kx] e^{-2 pi i ky y / N}.
+// Thread (kx, p) forms ky = p and ky = -p from one twiddle (the second is
+// its conjugate); p = 0 forms ky = 0 and ky = -16.
 __global__ __launch_bounds__(256) void dfty_fwd_kernel(const float2* __restrict__ in,
                                                        const float2* __restrict__ tw,
                                                        float2* __restrict__ out, int N) {
@@ -140,20 +142,21 @@ __global__ __launch_bounds__(256) void dfty_fwd_kernel(const float2* __restrict_
     for (int i = threadIdx.x; i < N; i += 256) ts[i] = tw[i];
     for (int i = threadIdx.x; i < N * MX; i += 256) xs[i] = in[pl * N * MX + i];
     __syncthreads();
-    for (int o = threadIdx.x; o < NM; o += 256) {
-        const int kyi = o / MX, kx = o - kyi * MX;
-        const int ky = ky_of(kyi, N);
-        float ar = 0.f, ai = 0.f;
-        int idx = 0;
-        for (int y = 0; y < N; y++) {
-            const float2 v = xs[y * MX + kx], w = ts[idx];
-            ar += v.x * w.x - v.y * w.y;
-            ai += v.x * w.y + v.y * w.x;
-            idx += ky;
-            if (idx >= N) idx -= N;
-        }
-        out[pl * NM + o] = make_float2(ar, ai);
+    const int kx = threadIdx.x & 15, p = threadIdx.x >> 4;
+    const int step = p ? p : MY / 2;
+    float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
+    int idx = 0;
+    for (int y = 0; y < N; y++) {
+        const float2 v = xs[y * MX + kx], w = ts[idx];  // w = e^{-i step y}
+        const float2 wa = p ? w : make_float2(1.f, 0.f);
+        ar += v.x * wa.x - v.y * wa.y;
+        ai += v.x * wa.y + v.y * wa.x;
+        br += v.x * w.x + v.y * w.y;  // v conj(w)
+        bi += v.y * w.x - v.x * w.y;
+        idx = (idx + step) & (N - 1);
     }
+    out[pl * NM + p * MX + kx] = make_float2(ar, ai);
+    out[pl * NM + (p ? MY - p : MY / 2) * MX + kx] = make_float2(br, bi);
 }
 
 // planes [P][MY][MX] -> [P][N][MX]: sum_kyi in[kyi][kx] e^{+2 pi i ky y / N}
@@ -539,6 +542,378 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
     h[e] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// ---------------------------------------------------------------------------
+// Inference on the matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulate).
+//
+// One kernel per Fourier layer works on whole image rows, one row y per wave,
+// with the activations pixel-major ([B][N][N][C]: the 32 channels of a pixel
+// contiguous).  For the 32 pixels x of one chunk it forms
+//   a[x][o] = sum_i x[x][i] W[o][i] + sum_j T[x][j] Z[y][o][j] + b[o]
+// as one product with K = 64: the pointwise term (K = the 32 input channels)
+// and the inverse row DFT of the spectral term (K = j: the 16 retained kx,
+// cosine and sine parts, Z carrying the c2r weights and 1/N^2).  GELU of a is
+// the next layer's input; it is stored, and the same registers feed a second
+// product, the next layer's truncated row DFT
+//   D[o][kx] = sum_x x'[x][o] e^{-2 pi i kx x / N},
+// accumulated over the row's chunks.  The MFMA output fragment (lane l holds
+// rows 4(l/16) + i, column l%16) becomes that product's B operand without a
+// transpose: lane group g holds pixels 4g + i of both 16-pixel tiles of the
+// chunk, so the K order of the second product is those 8 pixels, and its
+// cos / sin table is stored in the same order.  The last layer forms a
+// transposed (a[o][x]) so that a lane holds 8 channels of one pixel, and the
+// projection Q1 a (K = the channels in that order) follows on the matrix
+// cores, then GELU, the Q2 sum over the 4 lane groups and one store per pixel.
+//
+// Numerics.  The weights are bf16 (the copy every kernel reads), exact as
+// MFMA operands.  Activations, spectral coefficients and the cos / sin tables
+// are split into bf16 hi + lo parts (hi: the top 16 bits, lo: the rest
+// rounded down to bf16), and each product takes hi*hi + hi*lo + lo*hi (two
+// terms where one side is an exact weight), so a product is the FP32 one to
+// about 2^-16 relative.  GELU uses the erfc approximation of Abramowitz &
+// Stegun 7.1.26 (absolute error below 1.5e-7).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+constexpr int FR = 8;  // rows (waves) per block of the matrix-core layer kernels
+
+__device__ __forceinline__ f4v mfma_bf(u4v a, u4v b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a),
+                                                   __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+
+// top 16 bits of a (low half) and b (high half)
+__device__ __forceinline__ unsigned pack_hi(float a, float b) {
+    return __builtin_amdgcn_perm(__float_as_uint(b), __float_as_uint(a), 0x07060302u);
+}
+
+__device__ __forceinline__ float rest(float a) {
+    return a - __uint_as_float(__float_as_uint(a) & 0xffff0000u);
+}
+
+// 8 FP32 values -> bf16 hi and lo fragments (element s at bf16 position s)
+__device__ __forceinline__ void split8(const float (&v)[8], u4v& hi, u4v& lo) {
+    hi = (u4v){pack_hi(v[0], v[1]), pack_hi(v[2], v[3]), pack_hi(v[4], v[5]),
+               pack_hi(v[6], v[7])};
+    lo = (u4v){pack_hi(rest(v[0]), rest(v[1])), pack_hi(rest(v[2]), rest(v[3])),
+               pack_hi(rest(v[4]), rest(v[5])), pack_hi(rest(v[6]), rest(v[7]))};
+}
+
+// GELU(a) = a/2 (1 + erf(a / sqrt 2)) with 1 + erf(z) = erfc(-z): for z < 0
+// erfc(|z|) directly (no cancellation), for z >= 0 2 - erfc(z)
+__device__ __forceinline__ float gelu_fast(float a) {
+    const float z = fabsf(a) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(1.f + 0.3275911f * z);
+    const float p =
+        t * (0.254829592f +
+             t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float q = p * __expf(-z * z);  // erfc(z)
+    return 0.5f * a * (a >= 0.f ? 2.f - q : q);
+}
+
+// layer 0's row DFT straight from the mask: x0 = pw m + pb, so
+// D0[b][o][y][kx] = pw[o] M[y][kx] + pb[o] N [kx = 0], M the mask's row DFT
+__global__ __launch_bounds__(256) void lift_rows_kernel(const float* __restrict__ mask,
+                                                        const float2* __restrict__ tw,
+                                                        const unsigned short* __restrict__ wb,
+                                                        size_t opw, size_t opb,
+                                                        float2* __restrict__ D, int N) {
+    extern __shared__ float sm[];
+    float* rs = sm;                       // [16][N]
+    float2* ts = (float2*)(sm + 16 * N);  // [N]
+    const int b = blockIdx.y, y0 = blockIdx.x * 16;
+    for (int i = threadIdx.x; i < N; i += 256) ts[i] = tw[i];
+    for (int i = threadIdx.x; i < 16 * N; i += 256) rs[i] = mask[((long)b * N + y0) * N + i];
+    __syncthreads();
+    const int r = threadIdx.x >> 4, kx = threadIdx.x & 15;
+    float ar = 0.f, ai = 0.f;
+    int idx = 0;
+    for (int x = 0; x < N; x++) {
+        const float2 w = ts[idx];
+        ar += rs[r * N + x] * w.x;
+        ai += rs[r * N + x] * w.y;
+        idx = (idx + kx) & (N - 1);
+    }
+    const int y = y0 + r;
+    for (int o = 0; o < CW; o++) {
+        const float pw = bf(wb[opw + o]);
+        const float dc = kx == 0 ? bf(wb[opb + o]) * (float)N : 0.f;
+        D[(((long)b * CW + o) * N + y) * MX + kx] = make_float2(pw * ar + dc, pw * ai);
+    }
+}
+
+// the inverse column DFT of the mixed modes, in the layer kernels' layout:
+// Z[b][y][o][kx] = w_kx / N^2 Re S, Z[b][y][o][16 + kx] = -w_kx / N^2 Im S,
+// S = sum_kyi Y[b][o][kyi][kx] e^{+2 pi i ky y / N} (w_0 = 1, w_kx = 2)
+__global__ __launch_bounds__(256) void dfty_inv_z_kernel(const float2* __restrict__ Y,
+                                                         const float2* __restrict__ tw,
+                                                         float* __restrict__ Z, int N) {
+    __shared__ float2 ys[NM];
+    __shared__ float2 ts[256];
+    const long pl = blockIdx.x;
+    const long b = pl / CW;
+    const int o = (int)(pl - b * CW);
+    for (int i = threadIdx.x; i < N; i += 256) ts[i] = tw[i];
+    for (int i = threadIdx.x; i < NM; i += 256) ys[i] = Y[pl * NM + i];
+    __syncthreads();
+    const int kx = threadIdx.x & 15;
+    const float wsc = (kx ? 2.f : 1.f) / ((float)N * (float)N);
+    float2 yv[MY];  // this thread's column kx
+#pragma unroll
+    for (int k = 0; k < MY; k++) yv[k] = ys[k * MX + kx];
+    for (int y = threadIdx.x >> 4; y < N; y += 16) {
+        // ky = +p and -p share the twiddle w = e^{-i p y}: u conj(w) + v w
+        float ar = yv[0].x, ai = yv[0].y;
+#pragma unroll
+        for (int p = 1; p < MY / 2; p++) {
+            const float2 w = ts[(p * y) & (N - 1)];
+            const float2 u = yv[p], v = yv[MY - p];
+            ar += u.x * w.x + u.y * w.y + v.x * w.x - v.y * w.y;
+            ai += u.y * w.x - u.x * w.y + v.x * w.y + v.y * w.x;
+        }
+        const float2 w = ts[(MY / 2 * y) & (N - 1)];  // ky = -16
+        ar += yv[MY / 2].x * w.x - yv[MY / 2].y * w.y;
+        ai += yv[MY / 2].x * w.y + yv[MY / 2].y * w.x;
+        float* zp = Z + (((long)b * N + y) * CW + o) * 32;
+        zp[kx] = wsc * ar;
+        zp[16 + kx] = -wsc * ai;
+    }
+}
+
+// B-operand fragments of one row that stay in registers: the layer's weights
+// W (col o = 16 ot + r, K = input channels 8g..8g+7), exact in bf16, and the
+// row's spectral coefficients Z (col o, K = j = 8g..8g+7) split hi + lo
+__device__ __forceinline__ void row_operands(const unsigned short* __restrict__ wb, size_t ow,
+                                             const float* __restrict__ zrow, int g, int r,
+                                             u4v (&wf)[2], u4v (&zh)[2], u4v (&zl)[2]) {
+#pragma unroll
+    for (int ot = 0; ot < 2; ot++) {
+        const int o = 16 * ot + r;
+        wf[ot] = *(const u4v*)(wb + ow + o * CW + 8 * g);
+        const float4* zp = (const float4*)(zrow + o * 32 + 8 * g);
+        const float4 z0 = zp[0], z1 = zp[1];
+        const float v[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
+        split8(v, zh[ot], zl[ot]);
+    }
+}
+
+// Fourier layers 0..2 (GELU) on the matrix cores: x_in (or, for the first
+// layer, the lifted mask) and Z -> x_out and the next layer's row DFT D.
+// Grid (N / FR, B), FR waves per block, wave w on row y = FR blockIdx.x + w.
+template <int N, bool FIRST>
+__global__ __launch_bounds__(64 * FR)
+__attribute__((amdgpu_waves_per_eu(4))) void layer_mfma_kernel(
+    const float* __restrict__ xin, const float* __restrict__ mask, const float* __restrict__ Z,
+    const u4v* __restrict__ Tf, const u4v* __restrict__ Ef, const unsigned short* __restrict__ wb,
+    size_t ow, size_t ob, size_t opw, size_t opb, float* __restrict__ xout,
+    float2* __restrict__ D) {
+    constexpr int NT = N / 16, NS = N / 32;
+    __shared__ u4v Ts[NT * 128];  // [xt][hi, lo][lane]
+    __shared__ u4v Es[NS * 256];  // [S][jt][hi, lo][lane]
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    const int b = blockIdx.y, y = blockIdx.x * FR + (tid >> 6);
+    for (int i = tid; i < NT * 128; i += 64 * FR) Ts[i] = Tf[i];
+    for (int i = tid; i < NS * 256; i += 64 * FR) Es[i] = Ef[i];
+    u4v wf[2], zh[2], zl[2];
+    row_operands(wb, ow, Z + ((long)b * N + y) * CW * 32, g, r, wf, zh, zl);
+    const float bias[2] = {bf(wb[ob + r]), bf(wb[ob + 16 + r])};
+    __shared__ float lift[2 * CW];  // pw, pb of the first layer's input
+    if (FIRST && tid < 2 * CW) lift[tid] = bf(wb[(tid < CW ? opw : opb - CW) + tid]);
+    __syncthreads();
+    const long prow = ((long)b * N + y) * N;  // pixel index of (b, y, 0)
+    f4v acc3[2][2];
+#pragma unroll
+    for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+        for (int ot = 0; ot < 2; ot++) acc3[jt][ot] = (f4v){0.f, 0.f, 0.f, 0.f};
+    for (int S = 0; S < NS; S++) {
+        f4v acc[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int xt = 2 * S + h, x = 16 * xt + r;
+            float v[8];
+            if (FIRST) {
+                const float m = mask[prow + x];
+#pragma unroll
+                for (int s = 0; s < 8; s++) v[s] = lift[8 * g + s] * m + lift[CW + 8 * g + s];
+            } else {
+                const float4* p = (const float4*)(xin + (prow + x) * CW + 8 * g);
+                const float4 a0 = p[0], a1 = p[1];
+                v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w;
+                v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+            }
+            u4v xh, xl;
+            split8(v, xh, xl);
+            const u4v th = Ts[(2 * xt) * 64 + lane], tl = Ts[(2 * xt + 1) * 64 + lane];
+#pragma unroll
+            for (int ot = 0; ot < 2; ot++) {
+                f4v c = mfma_bf(xh, wf[ot], (f4v){0.f, 0.f, 0.f, 0.f});
+                c = mfma_bf(xl, wf[ot], c);
+                c = mfma_bf(th, zh[ot], c);
+                c = mfma_bf(th, zl[ot], c);
+                c = mfma_bf(tl, zh[ot], c);
+                acc[h][ot] = c;
+            }
+        }
+        // lane: pixel x = 32 S + 16 h + 4 g + i, channel o = 16 ot + r
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int ot = 0; ot < 2; ot++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float v = gelu_fast(acc[h][ot][i] + bias[ot]);
+                    acc[h][ot][i] = v;
+                    xout[(prow + 32 * S + 16 * h + 4 * g + i) * CW + 16 * ot + r] = v;
+                }
+        // next layer's row DFT over these 32 pixels
+#pragma unroll
+        for (int ot = 0; ot < 2; ot++) {
+            const float v[8] = {acc[0][ot][0], acc[0][ot][1], acc[0][ot][2], acc[0][ot][3],
+                                acc[1][ot][0], acc[1][ot][1], acc[1][ot][2], acc[1][ot][3]};
+            u4v bh, bl;
+            split8(v, bh, bl);
+#pragma unroll
+            for (int jt = 0; jt < 2; jt++) {
+                const u4v eh = Es[((2 * S + jt) * 2) * 64 + lane];
+                const u4v el = Es[((2 * S + jt) * 2 + 1) * 64 + lane];
+                f4v c = mfma_bf(eh, bh, acc3[jt][ot]);
+                c = mfma_bf(eh, bl, c);
+                acc3[jt][ot] = mfma_bf(el, bh, c);
+            }
+        }
+    }
+    // lane: kx = 4 g + i (jt 0: real part, 1: imaginary part), channel 16 ot + r
+#pragma unroll
+    for (int ot = 0; ot < 2; ot++) {
+        float4* d = (float4*)(D + (((long)b * CW + 16 * ot + r) * N + y) * MX + 4 * g);
+        d[0] = make_float4(acc3[0][ot][0], acc3[1][ot][0], acc3[0][ot][1], acc3[1][ot][1]);
+        d[1] = make_float4(acc3[0][ot][2], acc3[1][ot][2], acc3[0][ot][3], acc3[1][ot][3]);
+    }
+}
+
+// Fourier layer 3 (no GELU) and the projection on the matrix cores:
+// x_in and Z -> out[b][y][x] = q2b + sum_j Q2[j] GELU(Q1[j] . a[.][x] + q1b[j]).
+constexpr int LR = 4;  // rows (waves) per block of the last layer's kernel
+
+template <int N>
+__global__ __launch_bounds__(64 * LR) void last_mfma_kernel(
+    const float* __restrict__ xin, const float* __restrict__ Z, const u4v* __restrict__ Tf,
+    const unsigned short* __restrict__ wb, size_t ow, size_t ob, size_t oq1, size_t oq1b,
+    size_t oq2, size_t oq2b, float* __restrict__ out) {
+    constexpr int NT = N / 16;
+    __shared__ u4v Ts[NT * 128];
+    __shared__ float qb[PJ], q2[PJ];
+    __shared__ u4v Qs[PJ / 16 * 64];
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    const int b = blockIdx.y, y = blockIdx.x * LR + (tid >> 6);
+    for (int i = tid; i < NT * 128; i += 64 * LR) Ts[i] = Tf[i];
+    for (int i = tid; i < PJ; i += 64 * LR) {
+        qb[i] = bf(wb[oq1b + i]);
+        q2[i] = bf(wb[oq2 + i]);
+    }
+    u4v wf[2], zh[2], zl[2];
+    row_operands(wb, ow, Z + ((long)b * N + y) * CW * 32, g, r, wf, zh, zl);
+    // a[o][x] fragments: lane holds channels o = 16 ot + 4 g + i of pixel r
+    float bias[2][4];
+#pragma unroll
+    for (int ot = 0; ot < 2; ot++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) bias[ot][i] = bf(wb[ob + 16 * ot + 4 * g + i]);
+    // Q1 A fragments (row j = 16 jt + r) in the same channel order:
+    // 4g..4g+3, then 16+4g..16+4g+3; fragment jt of lane l at Qs[64 jt + l]
+    for (int i = tid; i < PJ / 16 * 64; i += 64 * LR) {
+        const int jt = i >> 6, l = i & 63;
+        const unsigned short* q = wb + oq1 + (16 * jt + (l & 15)) * CW + 4 * (l >> 4);
+        const uint2 lo = *(const uint2*)q, hi = *(const uint2*)(q + 16);
+        Qs[i] = (u4v){lo.x, lo.y, hi.x, hi.y};
+    }
+    const float q2b = bf(wb[oq2b]);
+    __syncthreads();
+    const long prow = ((long)b * N + y) * N;
+    for (int xt = 0; xt < NT; xt++) {
+        const int x = 16 * xt + r;
+        const float4* p = (const float4*)(xin + (prow + x) * CW + 8 * g);
+        const float4 a0 = p[0], a1 = p[1];
+        const float v[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+        u4v xh, xl;
+        split8(v, xh, xl);
+        const u4v th = Ts[(2 * xt) * 64 + lane], tl = Ts[(2 * xt + 1) * 64 + lane];
+        f4v a[2];
+#pragma unroll
+        for (int ot = 0; ot < 2; ot++) {
+            f4v c = mfma_bf(wf[ot], xh, (f4v){0.f, 0.f, 0.f, 0.f});
+            c = mfma_bf(wf[ot], xl, c);
+            c = mfma_bf(zh[ot], th, c);
+            c = mfma_bf(zl[ot], th, c);
+            a[ot] = mfma_bf(zh[ot], tl, c);
+        }
+        const float av[8] = {a[0][0] + bias[0][0], a[0][1] + bias[0][1], a[0][2] + bias[0][2],
+                             a[0][3] + bias[0][3], a[1][0] + bias[1][0], a[1][1] + bias[1][1],
+                             a[1][2] + bias[1][2], a[1][3] + bias[1][3]};
+        u4v ah, al;
+        split8(av, ah, al);
+        float part = 0.f;
+#pragma unroll
+        for (int jt = 0; jt < PJ / 16; jt++) {
+            const u4v qf = Qs[jt * 64 + lane];
+            f4v c = mfma_bf(qf, ah, (f4v){0.f, 0.f, 0.f, 0.f});
+            c = mfma_bf(qf, al, c);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int j = 16 * jt + 4 * g + i;
+                part += q2[j] * gelu_fast(c[i] + qb[j]);
+            }
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (g == 0) out[prow + x] = part + q2b;
+    }
+}
+
+// SOCS as judge of served clips: clip j of the check is img + j istride
+// against ref + j NN; err[j] = max |img - ref| / max |ref| (0 when both are
+// all zero, +inf when only ref is)
+__global__ __launch_bounds__(256) void judge_kernel(const float* __restrict__ img, long istride,
+                                                    const float* __restrict__ ref, long NN,
+                                                    float* __restrict__ err) {
+    __shared__ float rd[256], rr[256];
+    const long j = blockIdx.x;
+    const float4* a = (const float4*)(img + j * istride);
+    const float4* r = (const float4*)(ref + j * NN);
+    float md = 0.f, mr = 0.f;
+    for (long i = threadIdx.x; i < NN / 4; i += 256) {
+        const float4 u = a[i], v = r[i];
+        md = fmaxf(md, fmaxf(fmaxf(fabsf(u.x - v.x), fabsf(u.y - v.y)),
+                             fmaxf(fabsf(u.z - v.z), fabsf(u.w - v.w))));
+        mr = fmaxf(mr, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        // a NaN anywhere makes the clip fail the check
+        if (u.x != u.x || u.y != u.y || u.z != u.z || u.w != u.w) md = __builtin_inff();
+    }
+    rd[threadIdx.x] = md;
+    rr[threadIdx.x] = mr;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            rd[threadIdx.x] = fmaxf(rd[threadIdx.x], rd[threadIdx.x + k]);
+            rr[threadIdx.x] = fmaxf(rr[threadIdx.x], rr[threadIdx.x + k]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) err[j] = rd[0] == 0.f ? 0.f : rd[0] / rr[0];
+}
+
+// clips whose err is above tol (or NaN) take the SOCS image
+__global__ __launch_bounds__(256) void fallback_kernel(float* __restrict__ img, long istride,
+                                                       const float* __restrict__ ref, long NN,
+                                                       const float* __restrict__ err, float tol) {
+    const long j = blockIdx.y;
+    if (err[j] <= tol) return;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < NN; i += (long)gridDim.x * 256)
+        img[j * istride + i] = ref[j * NN + i];
+}
+
 inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -565,6 +940,18 @@ struct litho_fno {
     float *a1 = nullptr, *out = nullptr, *dout = nullptr, *dv = nullptr, *dx = nullptr;
     float* part = nullptr;     // per-chunk partial sums of the gradient reductions
     std::vector<void*> owned;
+    size_t nfixed = 0;  // blocks of owned allocated at creation
+    // matrix-core inference: cos / sin tables in fragment order, and its own
+    // activation buffers (pixel-major ping-pong, row DFT D, Z, modes)
+    u4v *Tf = nullptr, *Ef = nullptr;
+    int icap = 0;
+    float* xi[2] = {};
+    float2* Di = nullptr;
+    float* Zi = nullptr;
+    float2 *X2i = nullptr, *Y2i = nullptr;
+    std::vector<void*> iowned;
+    float* jerr = nullptr;  // per-clip errors of litho_fno_verify
+    int jcap = 0;
 };
 
 namespace {
@@ -579,9 +966,8 @@ int fno_alloc(litho_fno* f, void** p, size_t bytes) {
 }
 
 void fno_free_acts(litho_fno* f) {
-    // activations are the owned blocks after the first six (params, bf16,
-    // grads, two moments, twiddles + loss)
-    while (f->owned.size() > 7) {
+    // activations are the owned blocks after those of litho_fno_create
+    while (f->owned.size() > f->nfixed) {
         (void)hipFree(f->owned.back());
         f->owned.pop_back();
     }
@@ -644,6 +1030,127 @@ int forward(litho_fno* f, const float* d_mask, int B, bool train, hipStream_t st
     }
     hipLaunchKernelGGL(proj_fwd_kernel, dim3(nb(B * NN)), dim3(256), 0, st, f->x[NL], f->h, f->L.q1,
                        f->L.q1b, f->L.q2, f->L.q2b, f->out, train ? f->a1 : nullptr, B, NN);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+// bf16 hi (nearest) + lo (the rounded rest) of v
+void bf_split(double v, unsigned short& hi, unsigned short& lo) {
+    auto rne = [](float x) {
+        unsigned u;
+        std::memcpy(&u, &x, 4);
+        return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    };
+    hi = rne((float)v);
+    const unsigned uh = (unsigned)hi << 16;
+    float fh;
+    std::memcpy(&fh, &uh, 4);
+    lo = rne((float)(v - (double)fh));
+}
+
+// The cos / sin tables of the matrix-core layers, 8 bf16 per lane fragment.
+// T (inverse row DFT, row x = 16 xt + r, K = j = 8g + s): cos(2 pi j x / N)
+// for j < 16, sin(2 pi (j - 16) x / N) above; stored [xt][hi, lo][lane].
+// E (forward row DFT, row j = 16 jt + r, K = pixel x = 32 S + 4g + s for
+// s < 4, 32 S + 16 + 4g + s - 4 above): cos for j < 16, -sin above; stored
+// [S][jt][hi, lo][lane].  (lane = 16 g + r.)
+void mfma_tables(int N, std::vector<unsigned short>& T, std::vector<unsigned short>& E) {
+    const int NT = N / 16, NS = N / 32;
+    auto cs = [N](int k, int x, bool sine) {
+        const double t = 2.0 * M_PI * (double)(((long)k * x) % N) / N;
+        return sine ? std::sin(t) : std::cos(t);
+    };
+    T.assign((size_t)NT * 128 * 8, 0);
+    for (int xt = 0; xt < NT; xt++)
+        for (int lane = 0; lane < 64; lane++)
+            for (int s = 0; s < 8; s++) {
+                const int x = 16 * xt + (lane & 15), j = 8 * (lane >> 4) + s;
+                unsigned short hi, lo;
+                bf_split(j < 16 ? cs(j, x, false) : cs(j - 16, x, true), hi, lo);
+                T[((size_t)(2 * xt) * 64 + lane) * 8 + s] = hi;
+                T[((size_t)(2 * xt + 1) * 64 + lane) * 8 + s] = lo;
+            }
+    E.assign((size_t)NS * 256 * 8, 0);
+    for (int S = 0; S < NS; S++)
+        for (int jt = 0; jt < 2; jt++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int s = 0; s < 8; s++) {
+                    const int g = lane >> 4, j = 16 * jt + (lane & 15);
+                    const int x = 32 * S + (s < 4 ? 4 * g + s : 16 + 4 * g + s - 4);
+                    unsigned short hi, lo;
+                    bf_split(j < 16 ? cs(j, x, false) : -cs(j - 16, x, true), hi, lo);
+                    E[((size_t)((2 * S + jt) * 2) * 64 + lane) * 8 + s] = hi;
+                    E[((size_t)((2 * S + jt) * 2 + 1) * 64 + lane) * 8 + s] = lo;
+                }
+}
+
+void fno_free_infer(litho_fno* f) {
+    for (void* p : f->iowned) (void)hipFree(p);
+    f->iowned.clear();
+    f->icap = 0;
+}
+
+int infer_alloc(litho_fno* f, void** p, size_t bytes) {
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+        *p = nullptr;
+        return LITHO_ENOMEM;
+    }
+    f->iowned.push_back(*p);
+    return LITHO_OK;
+}
+
+int fno_reserve_infer(litho_fno* f, int B) {
+    if (B <= f->icap) return LITHO_OK;
+    LITHO_HIP(hipStreamSynchronize(ctx_stream(f->ctx)));
+    fno_free_infer(f);
+    const size_t N = f->N, plane = (size_t)B * N * N * CW * sizeof(float);
+    for (int k = 0; k < 2; k++) LITHO_TRY(infer_alloc(f, (void**)&f->xi[k], plane));
+    LITHO_TRY(infer_alloc(f, (void**)&f->Di, (size_t)B * CW * N * MX * sizeof(float2)));
+    LITHO_TRY(infer_alloc(f, (void**)&f->Zi, (size_t)B * N * CW * 32 * sizeof(float)));
+    LITHO_TRY(infer_alloc(f, (void**)&f->X2i, (size_t)B * CW * NM * sizeof(float2)));
+    LITHO_TRY(infer_alloc(f, (void**)&f->Y2i, (size_t)B * CW * NM * sizeof(float2)));
+    f->icap = B;
+    return LITHO_OK;
+}
+
+template <int N>
+void launch_layers_mfma(litho_fno* f, const float* d_mask, int B, float* d_out, hipStream_t st) {
+    const Layout& L = f->L;
+    const dim3 grid(N / FR, B), blk(64 * FR);
+    hipLaunchKernelGGL(lift_rows_kernel, dim3(N / 16, B), dim3(256),
+                       (size_t)16 * N * sizeof(float) + (size_t)N * sizeof(float2), st, d_mask,
+                       f->tw, f->h, L.pw, L.pb, f->Di, N);
+    for (int l = 0; l < NL; l++) {
+        hipLaunchKernelGGL(dfty_fwd_kernel, dim3((unsigned)(B * CW)), dim3(256),
+                           (size_t)(N * MX + N) * sizeof(float2), st, f->Di, f->tw, f->X2i, N);
+        hipLaunchKernelGGL(mix_fwd_kernel, dim3(nb((long)B * CW * NM)), dim3(256), 0, st, f->X2i,
+                           f->h, L.rre[l], L.rim[l], f->Y2i, B);
+        hipLaunchKernelGGL(dfty_inv_z_kernel, dim3((unsigned)(B * CW)), dim3(256), 0, st, f->Y2i,
+                           f->tw, f->Zi, N);
+        if (l == 0)
+            hipLaunchKernelGGL((layer_mfma_kernel<N, true>), grid, blk, 0, st, nullptr, d_mask,
+                               f->Zi, f->Tf, f->Ef, f->h, L.w[l], L.b[l], L.pw, L.pb, f->xi[0],
+                               f->Di);
+        else if (l < NL - 1)
+            hipLaunchKernelGGL((layer_mfma_kernel<N, false>), grid, blk, 0, st,
+                               f->xi[(l - 1) & 1], nullptr, f->Zi, f->Tf, f->Ef, f->h, L.w[l],
+                               L.b[l], L.pw, L.pb, f->xi[l & 1], f->Di);
+        else
+            hipLaunchKernelGGL((last_mfma_kernel<N>), dim3(N / LR, B), dim3(64 * LR), 0, st,
+                               f->xi[(l - 1) & 1],
+                               f->Zi, f->Tf, f->h, L.w[l], L.b[l], L.q1, L.q1b, L.q2, L.q2b,
+                               d_out);
+    }
+}
+
+// inference on the matrix cores, straight into d_out
+int forward_mfma(litho_fno* f, const float* d_mask, int B, float* d_out, hipStream_t st) {
+    switch (f->N) {
+        case 32: launch_layers_mfma<32>(f, d_mask, B, d_out, st); break;
+        case 64: launch_layers_mfma<64>(f, d_mask, B, d_out, st); break;
+        case 128: launch_layers_mfma<128>(f, d_mask, B, d_out, st); break;
+        case 256: launch_layers_mfma<256>(f, d_mask, B, d_out, st); break;
+        default: return LITHO_EARG;
+    }
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
@@ -715,6 +1222,11 @@ int litho_fno_create(litho_ctx* ctx, int N, unsigned long long seed, litho_fno**
     LITHO_TRY(fno_alloc(f.get(), (void**)&f->m2, P * sizeof(float)));
     LITHO_TRY(fno_alloc(f.get(), (void**)&f->tw, N * sizeof(float2)));
     LITHO_TRY(fno_alloc(f.get(), (void**)&f->loss, sizeof(float)));
+    std::vector<unsigned short> Th, Eh;
+    mfma_tables(N, Th, Eh);
+    LITHO_TRY(fno_alloc(f.get(), (void**)&f->Tf, Th.size() * sizeof(unsigned short)));
+    LITHO_TRY(fno_alloc(f.get(), (void**)&f->Ef, Eh.size() * sizeof(unsigned short)));
+    f->nfixed = f->owned.size();
     // initialisation: spectral weights U[0,1)/(C C) in both parts, pointwise
     // and projection weights U(+-1/sqrt(fan_in)), lift U(+-1), biases 0
     std::vector<float> h(P, 0.f);
@@ -737,6 +1249,10 @@ int litho_fno_create(litho_ctx* ctx, int N, unsigned long long seed, litho_fno**
     hipStream_t st = ctx_stream(ctx);
     LITHO_HIP(hipMemcpyAsync(f->p, h.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
     LITHO_HIP(hipMemcpyAsync(f->tw, tw.data(), N * sizeof(float2), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(f->Tf, Th.data(), Th.size() * sizeof(unsigned short),
+                             hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(f->Ef, Eh.data(), Eh.size() * sizeof(unsigned short),
+                             hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(fill_kernel, dim3(nb(P)), dim3(256), 0, st, f->m1, (long)P, 0.f);
     hipLaunchKernelGGL(fill_kernel, dim3(nb(P)), dim3(256), 0, st, f->m2, (long)P, 0.f);
     hipLaunchKernelGGL(to_bf16_kernel, dim3(nb(P)), dim3(256), 0, st, f->p, f->h, (long)P);
@@ -750,6 +1266,8 @@ int litho_fno_destroy(litho_fno* f) {
     if (!f) return LITHO_EARG;
     (void)hipStreamSynchronize(ctx_stream(f->ctx));
     for (void* p : f->owned) (void)hipFree(p);
+    fno_free_infer(f);
+    if (f->jerr) (void)hipFree(f->jerr);
     delete f;
     return LITHO_OK;
 }
@@ -776,6 +1294,39 @@ int litho_fno_set_params(litho_fno* f, const float* host, long long n) {
 }
 
 int litho_fno_forward(litho_fno* f, const float* d_mask, int B, float* d_out) {
+    if (!f || !d_mask || !d_out || B <= 0 || B > 65535) return LITHO_EARG;
+    LITHO_TRY(fno_reserve_infer(f, B));
+    return forward_mfma(f, d_mask, B, d_out, ctx_stream(f->ctx));
+}
+
+int litho_fno_verify(litho_fno* f, float* d_images, int clip_stride, const float* d_socs, int n,
+                     float tol, float* errs) {
+    if (!f || !d_images || !d_socs || !errs || n <= 0 || n > 65535 || clip_stride < 1 ||
+        !(tol >= 0.f))
+        return LITHO_EARG;
+    hipStream_t st = ctx_stream(f->ctx);
+    if (n > f->jcap) {
+        LITHO_HIP(hipStreamSynchronize(st));
+        if (f->jerr) LITHO_HIP(hipFree(f->jerr));
+        f->jerr = nullptr;
+        f->jcap = 0;
+        if (hipMalloc(&f->jerr, (size_t)n * sizeof(float)) != hipSuccess) return LITHO_ENOMEM;
+        f->jcap = n;
+    }
+    const long NN = (long)f->N * f->N, istride = (long)clip_stride * NN;
+    hipLaunchKernelGGL(judge_kernel, dim3((unsigned)n), dim3(256), 0, st, d_images, istride,
+                       d_socs, NN, f->jerr);
+    hipLaunchKernelGGL(fallback_kernel, dim3((unsigned)std::min<long>(nb(NN), 64), (unsigned)n),
+                       dim3(256), 0, st, d_images, istride, d_socs, NN, f->jerr, tol);
+    LITHO_HIP(hipGetLastError());
+    LITHO_HIP(hipMemcpyAsync(errs, f->jerr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    int fell = 0;
+    for (int j = 0; j < n; j++) fell += !(errs[j] <= tol);
+    return fell;
+}
+
+int litho_fno_forward_fp32(litho_fno* f, const float* d_mask, int B, float* d_out) {
     if (!f || !d_mask || !d_out || B <= 0) return LITHO_EARG;
     LITHO_TRY(fno_reserve(f, B));
     hipStream_t st = ctx_stream(f->ctx);

@@ -41,7 +41,8 @@ EXPORTS = [
     "litho_socs_image_buckets", "litho_bucket_times", "litho_autotune", "litho_plan_get",
     "litho_plan_set",
     "litho_fno_create", "litho_fno_destroy", "litho_fno_param_count", "litho_fno_get_params",
-    "litho_fno_set_params", "litho_fno_forward", "litho_fno_loss_grad", "litho_fno_adam",
+    "litho_fno_set_params", "litho_fno_forward", "litho_fno_forward_fp32", "litho_fno_loss_grad",
+    "litho_fno_adam", "litho_fno_verify",
 ]
 LITHO_TCC_FP64 = 0
 LITHO_TCC_SPLIT_BF16 = 1
@@ -127,6 +128,8 @@ def lib():
         L.litho_fno_get_params.argtypes = [vp, _f32p, C.c_longlong, i]
         L.litho_fno_set_params.argtypes = [vp, _f32p, C.c_longlong]
         L.litho_fno_forward.argtypes = [vp, vp, i, vp]
+        L.litho_fno_forward_fp32.argtypes = [vp, vp, i, vp]
+        L.litho_fno_verify.argtypes = [vp, vp, i, vp, i, f, _f32p]
         L.litho_fno_loss_grad.argtypes = [vp, vp, vp, i, C.POINTER(f)]
         L.litho_fno_adam.argtypes = [vp, f, f, f, f]
         bp = C.POINTER(LithoBucket)

@@ -325,8 +325,8 @@ int litho_plan_get(litho_ctx* ctx, int N, int Nx, int nk, int precision, int* G,
  * A Fourier neural operator mask -> aerial image (width 32, 16 x 16 retained
  * modes per sign of ky, GELU, 4 Fourier layers, 128-channel projection),
  * trained on the GPU by its own forward / backward kernels.  Weights are
- * FP32 masters read through a bf16 copy; arithmetic FP32.  N: power of two,
- * 32..256.  Masks and images are device f32 [B][N][N] on the context stream. */
+ * FP32 masters read through a bf16 copy.  N: power of two, 32..256.  Masks
+ * and images are device f32 [B][N][N] on the context stream. */
 typedef struct litho_fno litho_fno;
 int litho_fno_create(litho_ctx* ctx, int N, unsigned long long seed, litho_fno** out);
 int litho_fno_destroy(litho_fno* f);
@@ -334,7 +334,19 @@ long long litho_fno_param_count(const litho_fno* f);
 /* which = 0: FP32 weights, 1: gradients of the last litho_fno_loss_grad */
 int litho_fno_get_params(litho_fno* f, float* host, long long n, int which);
 int litho_fno_set_params(litho_fno* f, const float* host, long long n);
+/* Inference on the matrix cores: bf16 weights (exact) times activations and
+ * cos / sin tables split into bf16 hi + lo parts, FP32 accumulate, one fused
+ * kernel per Fourier layer (B <= 65535). */
 int litho_fno_forward(litho_fno* f, const float* d_masks, int B, float* d_images);
+/* The same map on the vector ALUs in FP32 arithmetic: the training forward. */
+int litho_fno_forward_fp32(litho_fno* f, const float* d_masks, int B, float* d_images);
+/* SOCS as judge and fallback of served clips, on the device: clip j of the
+ * check is d_images + j * clip_stride * N^2 against d_socs + j * N^2 (n
+ * clips).  errs[j] (host) = max |image - socs| / max |socs| (0 when both are
+ * all zero); every clip with errs[j] > tol, or NaN, is overwritten with its
+ * SOCS image.  Returns the number of clips replaced (>= 0) or a status. */
+int litho_fno_verify(litho_fno* f, float* d_images, int clip_stride, const float* d_socs, int n,
+                     float tol, float* errs);
 /* MSE loss over the batch and the gradients of every weight (synchronous
  * when loss is non-NULL) */
 int litho_fno_loss_grad(litho_fno* f, const float* d_masks, const float* d_targets, int B,

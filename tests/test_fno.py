@@ -72,7 +72,7 @@ def test_gpu_fno_matches_oracle(gpu, N):
         dm, dt, do = ctx.alloc(m.nbytes), ctx.alloc(t.nbytes), ctx.alloc(t.nbytes)
         ctx.h2d(dm, m)
         ctx.h2d(dt, t)
-        model.forward(dm, B, do)
+        model.forward_fp32(dm, B, do)
         out = ctx.d2h(np.zeros_like(t), do)
         ref_out = F.forward(th, m.astype(np.float64))
         assert np.abs(out - ref_out).max() <= 1e-5 * np.abs(ref_out).max()
@@ -93,6 +93,100 @@ def test_gpu_fno_matches_oracle(gpu, N):
         assert np.array_equal(model.params(grads=True).astype(np.float64), g)
         for p in (dm, dt, do):
             ctx.free(p)
+        model.close()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", [32, 64, 128, 256])
+def test_gpu_fno_matrix_core_inference(gpu, N):
+    """litho_fno_forward (fused matrix-core layer kernels, bf16 hi + lo
+    operands, FP32 accumulate) against the FP64 oracle and the FP32
+    vector-ALU forward at the same bf16-rounded weights, within 1e-4 of the
+    output's range; binary, grey-level and empty masks; a second call with a
+    smaller batch reuses the buffers of the first."""
+    from litho_amd.fno import FNO
+    print(f"[case] fno matrix-core inference N={N}", flush=True)
+    rng = np.random.default_rng(100 + N)
+    B = 3
+    th = F.round_bf16(F.init_params(N + 7))
+    m = np.stack([(rng.random((N, N)) < 0.4), rng.random((N, N)), np.zeros((N, N))])
+    m = m.astype(np.float32)
+    ctx = gpu.Context()
+    try:
+        model = FNO(ctx, N, seed=2)
+        model.set_params(th.astype(np.float32))
+        dm, do = ctx.alloc(m.nbytes), ctx.alloc(m.nbytes)
+        ctx.h2d(dm, m)
+        model.forward(dm, B, do)
+        out = ctx.d2h(np.zeros_like(m), do)
+        model.forward_fp32(dm, B, do)
+        f32 = ctx.d2h(np.zeros_like(m), do)
+        ref = F.forward(th, m.astype(np.float64))
+        tol = 1e-4 * np.abs(ref).max()
+        e_ref, e_f32 = np.abs(out - ref).max(), np.abs(out - f32).max()
+        print(f"[fno mfma N={N}] max abs err vs oracle {e_ref:.3e}, vs FP32 kernels {e_f32:.3e}, "
+              f"range {np.abs(ref).max():.3e}", flush=True)
+        assert e_ref <= tol and e_f32 <= tol
+        model.forward(dm, 1, do)
+        one = ctx.d2h(np.zeros((1, N, N), np.float32), do)
+        assert np.array_equal(one[0], out[0])
+        ctx.free(dm)
+        ctx.free(do)
+        model.close()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fno_verify_judge(gpu):
+    """litho_fno_verify: per-clip max relative error against SOCS images on the
+    device (numpy's on the host), the clips above tol (or NaN) overwritten
+    with their SOCS images, every other clip untouched; a clip stride of 2
+    checks clips 0, 2, 4 of the served batch."""
+    import ctypes as C
+    from litho_amd.fno import FNO
+    print("[case] fno verify judge", flush=True)
+    N, n = 32, 6
+    rng = np.random.default_rng(7)
+    ref = rng.random((n, N, N)).astype(np.float32) + 0.5
+    img = ref.copy()
+    scale = [0.0, 5e-3, 2e-2, 9.9e-3, 0.5, 0.0]
+    for j, e in enumerate(scale):
+        img[j, 3, 5] += e * ref[j].max()
+    img[5, 7, 7] = np.nan
+    ref[0] = 0.0
+    img[0] = 0.0  # both all zero: error 0
+    ctx = gpu.Context()
+    try:
+        model = FNO(ctx, N, seed=1)
+        di, dr = ctx.alloc(img.nbytes), ctx.alloc(ref.nbytes)
+        ctx.h2d(di, img)
+        ctx.h2d(dr, ref)
+        err = np.zeros(n, np.float32)
+        fell = gpu.lib().litho_fno_verify(model.h, di, 1, dr, n, 1e-2, err)
+        want = np.abs(img - ref).max(axis=(1, 2)) / np.where(ref.max(axis=(1, 2)) > 0,
+                                                             np.abs(ref).max(axis=(1, 2)), 1)
+        want[5] = np.inf
+        assert fell == 3
+        assert np.allclose(err[:5], want[:5], rtol=1e-6, atol=0) and np.isinf(err[5])
+        out = ctx.d2h(np.zeros_like(img), di)
+        for j in range(n):
+            expect = ref[j] if j in (2, 4, 5) else img[j]
+            assert np.array_equal(out[j], expect, equal_nan=True), j
+        # stride 2: clips 0, 2, 4 of the batch against three SOCS images
+        ctx.h2d(di, img)
+        ref3 = np.ascontiguousarray(ref[[0, 2, 4]])
+        ctx.h2d(dr, ref3)
+        err3 = np.zeros(3, np.float32)
+        assert gpu.lib().litho_fno_verify(model.h, di, 2, dr, 3, 1e-2, err3) == 2
+        out = ctx.d2h(np.zeros_like(img), di)
+        assert np.array_equal(out[2], ref[2]) and np.array_equal(out[4], ref[4])
+        assert np.array_equal(out[1], img[1]) and np.array_equal(out[3], img[3])
+        assert gpu.lib().litho_fno_verify(model.h, di, 0, dr, 3, 1e-2, err3) == -1
+        ctx.free(di)
+        ctx.free(dr)
         model.close()
     finally:
         ctx.close()
