@@ -15,6 +15,7 @@ The CPU baseline is the project's FP64 C restatement of the reference CPU path
 error check of the GPU images against FP64.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -155,10 +156,42 @@ def mixed_size_bench(ctx, src, sigma, steps):
             ctx.socs_image_buckets(b, prec, True)
         ctx.sync()
         rate = 512 * steps / (time.perf_counter() - t0)
+        # each bucket alone through the same entry (its own graph): the sum of
+        # these is the time of the set run one size after another, each
+        # filling the chip by itself
+        def bucket_ms(bk):
+            for _ in range(3):
+                ctx.socs_image_buckets([bk], prec, True)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.socs_image_buckets([bk], prec, True)
+            ctx.sync()
+            return (time.perf_counter() - t0) / steps * 1e3
+
+        alone, marginal = [], []
+        for bk in b:
+            alone.append(bucket_ms(bk))
+            # the same clips twice in one bucket: t(2c) - t(c) is the bucket's
+            # cost without its fixed launch and fill overheads
+            bank, N, cnt, dm, di = bk
+            mb = cnt * N * N * 4 if prec == L.LITHO_PREC_FP32 else cnt * N * N // 8
+            dm2, di2 = ctx.alloc(2 * mb), ctx.alloc(2 * cnt * N * N * 4)
+            ctx.d2d(dm2, dm, mb)
+            ctx.d2d(ctypes.c_void_p(dm2.value + mb), dm, mb)
+            marginal.append(bucket_ms((bank, N, 2 * cnt, dm2, di2)) - alone[-1])
+            ctx.free(dm2)
+            ctx.free(di2)
         out[name] = {"value": round(rate, 1), "unit": "images/s",
                      "p50_latency_ms": round(float(np.percentile(lat, 50)), 4),
                      "p99_latency_ms": round(float(np.percentile(lat, 99)), 4),
-                     "autotune_s": round(tune_s, 3)}
+                     "autotune_s": round(tune_s, 3),
+                     "set_ms": round(512 / rate * 1e3, 4),
+                     "bucket_alone_ms": {str(bk[1]): round(t, 4) for bk, t in zip(b, alone)},
+                     "buckets_in_sequence_ms": round(sum(alone), 4),
+                     "bucket_marginal_ms": {str(bk[1]): round(t, 4) for bk, t in zip(b, marginal)},
+                     "work_conserving_ms": round(sum(marginal), 4),
+                     "packing_efficiency": round(sum(marginal) / (512 / rate * 1e3), 3)}
         out["plans"][name] = {str(N): list(ctx.plan(N, L.compute_nx(NA, PIX_NM * N, sigma, LAMBDA),
                                                      16, prec)[:3]) for N in (256, 512, 768, 1024)}
         log(f"mixed sizes ({name}): {rate:.1f} images/s, per-clip latency p50 "

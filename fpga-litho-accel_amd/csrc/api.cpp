@@ -1503,10 +1503,11 @@ static int mfma_tables(litho_ctx* c, int N, int Nx, const void** tm, const void*
     return dft_table(c, 1, N, std::get<2>(ki), std::get<3>(ki), ti);
 }
 
-// the five launches of one same-size batch on stream st
+// the five launches of one same-size batch on stream st (bit s of stages:
+// launch s = mask rows, mask columns, SOCS, FI columns, image rows)
 static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask, int B, int N,
                         void* d_image, int prec, const BandPlan& pl, const BandWs& w,
-                        hipStream_t st) {
+                        hipStream_t st, int stages = 31) {
     const bool mixed = (prec & LITHO_PREC_INTER_FP16) != 0;
     const bool mfma = (prec & LITHO_PREC_DFT_MFMA) != 0;
     const bool u8 = (prec & LITHO_PREC_MASK_U8) != 0;
@@ -1530,30 +1531,37 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
         tm = a->second.p;
         ti = b->second.p;
     }
-    if (mfma)
-        LITHO_TRY(launch_mask_rows_mfma(N, d_mask, mfmt, tm, w.R, B, Nx + 1, mixed, st));
-    else
-        LITHO_TRY(launch_rows_dft_t(N, d_mask, w.R, (long)B * N, 0, Nx + 1, mixed, pl.lean_mem, u8,
-                                    st));
-    LITHO_TRY(
-        launch_mask_cols_t(N, w.R, (float2*)w.M, B, Nx + 1, -Nx, W, mixed, pl.lean_mem, st, W));
-    LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->band.d_scales, bank->band.n, Nx, B,
-                               (float2*)w.C, pl.socs_shape, st));
+    if (stages & 1) {
+        if (mfma)
+            LITHO_TRY(launch_mask_rows_mfma(N, d_mask, mfmt, tm, w.R, B, Nx + 1, mixed, st));
+        else
+            LITHO_TRY(launch_rows_dft_t(N, d_mask, w.R, (long)B * N, 0, Nx + 1, mixed,
+                                        pl.lean_mem, u8, st));
+    }
+    if (stages & 2)
+        LITHO_TRY(launch_mask_cols_t(N, w.R, (float2*)w.M, B, Nx + 1, -Nx, W, mixed, pl.lean_mem,
+                                     st, W));
+    if (stages & 4)
+        LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->band.d_scales,
+                                   bank->band.n, Nx, B, (float2*)w.C, pl.socs_shape, st));
     if (mfma) {
         // U for the image-side product: at 1024 points the column pass writes
         // it already split into fp16 hi + lo parts (its own form, fewer bytes);
         // elsewhere FP32, split by the product on the fly
         const bool usplit = N == 1024 && 2 * band + 1 <= 160;
-        if (usplit)
+        if ((stages & 8) && usplit)
             LITHO_TRY(launch_fi_cols_split(N, (float2*)w.C, w.U, B, band,
                                            image_rows_split_cols(band + 1), st));
-        else
+        else if (stages & 8)
             LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st));
-        LITHO_TRY(launch_image_rows_mfma(N, w.U, band + 1, ti, d_image, B, out16, usplit, st));
+        if (stages & 16)
+            LITHO_TRY(launch_image_rows_mfma(N, w.U, band + 1, ti, d_image, B, out16, usplit, st));
     } else {
-        LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st));
-        LITHO_TRY(launch_image_rows_t(N, w.U, d_image, (long)B * N, band, mixed, pl.lean_mem,
-                                      out16, st));
+        if (stages & 8)
+            LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st));
+        if (stages & 16)
+            LITHO_TRY(launch_image_rows_t(N, w.U, d_image, (long)B * N, band, mixed, pl.lean_mem,
+                                          out16, st));
     }
     return LITHO_OK;
 }
@@ -1665,19 +1673,23 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         hipStream_t s0 = c->stream;
         LITHO_HIP(hipEventRecord(c->evStart, s0));
         for (int i = 1; i < ns; i++) LITHO_HIP(hipStreamWaitEvent(c->xs[i - 1], c->evStart, 0));
-        for (int k = 0; k < chunks; k++) {
-            const int b0 = k * cb;
-            const int nb = std::min(cb, B - b0);
-            if (nb <= 0) break;
-            hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
-            BandWs wk = w[k % ns];
-            wk.M = (float2*)w[0].M + (size_t)b0 * W * W;
-            wk.C = (float2*)w[0].C + (size_t)b0 * (2 * band + 1) * (band + 1);
-            BandPlan pl = tuned;
-            if (chunks > 1) pl.lean_mem = true;
-            LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN / 8 * mb8, nb, N,
-                                   (char*)d_image + (size_t)b0 * NN * ib, precision, pl, wk, st));
-        }
+        // stage-major over the sub-batches (launch s of each before launch
+        // s + 1 of any): the graph issues its nodes in this order
+        for (int sg = 0; sg < 5; sg++)
+            for (int k = 0; k < chunks; k++) {
+                const int b0 = k * cb;
+                const int nb = std::min(cb, B - b0);
+                if (nb <= 0) break;
+                hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
+                BandWs wk = w[k % ns];
+                wk.M = (float2*)w[0].M + (size_t)b0 * W * W;
+                wk.C = (float2*)w[0].C + (size_t)b0 * (2 * band + 1) * (band + 1);
+                BandPlan pl = tuned;
+                if (chunks > 1) pl.lean_mem = true;
+                LITHO_TRY(enqueue_band(c, bank, (const char*)d_mask + (size_t)b0 * NN / 8 * mb8, nb,
+                                       N, (char*)d_image + (size_t)b0 * NN * ib, precision, pl, wk,
+                                       st, 1 << sg));
+            }
         for (int i = 1; i < ns; i++) {
             LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
             LITHO_HIP(hipStreamWaitEvent(s0, c->xe[i - 1], 0));
@@ -1747,20 +1759,32 @@ int litho_socs_image_buckets(litho_ctx* c, const LithoBucket* bk, int nb, int pr
     std::vector<BandPlan> plans(nb);
     for (int i = 0; i < nb; i++)
         if (bk[i].count) plans[i] = plan_of(c, bk[i].N, bk[i].bank->Nx, bk[i].bank->nk, precision);
+    // heaviest bucket first (pixels x clips): its chain is the longest, so it
+    // starts at once and the lighter chains fill the CUs it leaves idle
+    std::vector<int> order;
+    for (int i = 0; i < nb; i++)
+        if (bk[i].count) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (long)bk[a].count * bk[a].N * bk[a].N > (long)bk[b].count * bk[b].N * bk[b].N;
+    });
     auto enqueue = [&](bool timing) -> int {
         hipStream_t s0 = c->stream;
         if (timing) LITHO_HIP(hipEventRecord(c->bev0, s0));
         LITHO_HIP(hipEventRecord(c->evStart, s0));
         for (int i = 1; i < ns; i++) LITHO_HIP(hipStreamWaitEvent(c->xs[i - 1], c->evStart, 0));
-        int k = 0;
-        for (int i = 0; i < nb; i++) {
-            if (!bk[i].count) continue;
-            hipStream_t st = (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1];
-            k++;
-            LITHO_TRY(enqueue_band(c, bk[i].bank, bk[i].d_masks, bk[i].count, bk[i].N,
-                                   bk[i].d_images, precision, plans[i], w[i], st));
-            if (timing) LITHO_HIP(hipEventRecord(c->bev[i], st));
-        }
+        // stage-major: launch s of every bucket before launch s + 1 of any.
+        // The graph keeps the buckets' chains independent either way, but it
+        // issues its nodes in this order, so the chains advance together
+        // (0.316 -> 0.282 ms for 512 mixed clips in the compact form).
+        auto stream_of = [&](int k) { return (k % ns) == 0 ? s0 : c->xs[(k % ns) - 1]; };
+        for (int sg = 0; sg < 5; sg++)
+            for (int k = 0; k < (int)order.size(); k++) {
+                const int i = order[k];
+                LITHO_TRY(enqueue_band(c, bk[i].bank, bk[i].d_masks, bk[i].count, bk[i].N,
+                                       bk[i].d_images, precision, plans[i], w[i], stream_of(k),
+                                       1 << sg));
+                if (timing && sg == 4) LITHO_HIP(hipEventRecord(c->bev[i], stream_of(k)));
+            }
         for (int i = 1; i < ns; i++) {
             LITHO_HIP(hipEventRecord(c->xe[i - 1], c->xs[i - 1]));
             LITHO_HIP(hipStreamWaitEvent(s0, c->xe[i - 1], 0));
