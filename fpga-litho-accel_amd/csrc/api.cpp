@@ -2065,11 +2065,54 @@ int litho_socs_tmpimg_dev(litho_ctx* c, const litho_bank* bank, const float* d_m
                           d_output, st);
 }
 
+int litho_image_tcc_rect(const float* mask, int Lx, int Ly, const double* tcc_re,
+                         const double* tcc_im, int Nx, int Ny, float* image) {
+    if (!mask || !tcc_re || !tcc_im || !image || Nx < 0 || Ny < 0 || !is_pow2(Lx) ||
+        !is_pow2(Ly) || Lx < 32 || Ly < 32 || Lx > 4096 || Ly > 4096 || 4 * Nx + 1 > Lx ||
+        4 * Ny + 1 > Ly)
+        return LITHO_EARG;
+    const int W = 2 * Nx + 1, H = 2 * Ny + 1, n = W * H;
+    for (size_t i = 0; i < (size_t)n * n; i++)
+        if (!std::isfinite(tcc_re[i]) || !std::isfinite(tcc_im[i])) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    const size_t LL = (size_t)Lx * Ly;
+    std::vector<double2> ht((size_t)n * n);
+    for (size_t i = 0; i < ht.size(); i++) ht[i] = make_double2(tcc_re[i], tcc_im[i]);
+    Dev dm, dR, dM, dT, dC, dV, dI;
+    LITHO_TRY(dm.alloc(LL * sizeof(float)));
+    LITHO_TRY(dR.alloc((size_t)Ly * W * sizeof(float2)));
+    LITHO_TRY(dM.alloc((size_t)n * sizeof(float2)));
+    LITHO_TRY(dT.alloc(ht.size() * sizeof(double2)));
+    LITHO_TRY(dC.alloc((size_t)W * (4 * Ny + 1) * sizeof(float2)));
+    LITHO_TRY(dV.alloc((size_t)Ly * W * sizeof(float2)));
+    LITHO_TRY(dI.alloc(LL * sizeof(float)));
+    LITHO_HIP(hipMemcpyAsync(dm.p, mask, LL * sizeof(float), hipMemcpyHostToDevice, st));
+    LITHO_HIP(hipMemcpyAsync(dT.p, ht.data(), ht.size() * sizeof(double2), hipMemcpyHostToDevice,
+                             st));
+    // mask window M [H][W] = centred spectrum / (Lx Ly) (FT_r2c_padded): the
+    // column pass divides by Ly^2, so the band spectrum (quadratic in M) is
+    // rescaled by (Ly / Lx)^2 in the column pass of the image
+    LITHO_TRY(launch_rows_dft(Lx, dm.as<float>(), dR.as<float2>(), Ly, -Nx, W, true, c->tw, st));
+    LITHO_TRY(launch_mask_cols(Ly, dR.as<float2>(), dM.as<float2>(), 1, W, -Ny, H, c->tw, st));
+    LITHO_TRY(launch_tcc_image_spectrum(dM.as<float2>(), dT.as<double2>(), Nx, Ny,
+                                        dC.as<float2>(), st));
+    const float s = (float)(((double)Ly / Lx) * ((double)Ly / Lx));
+    LITHO_TRY(launch_tcc_image_cols(dC.as<float2>(), Nx, Ny, Ly, s, dV.as<float2>(), st));
+    LITHO_TRY(launch_image_rows(Lx, dV.as<float2>(), dI.as<float>(), Ly, 2 * Nx, c->tw, st));
+    LITHO_HIP(hipMemcpyAsync(image, dI.p, LL * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    return LITHO_OK;
+}
+
 int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double* tcc_im, int Nx,
                     int Ny, float* image) {
-    if (!mask || !tcc_re || !tcc_im || !image || Nx < 0 || Ny != Nx || !is_pow2(N) || N < 256 ||
-        N > 1024 || 4 * Nx + 1 > N)
-        return LITHO_EARG;
+    if (!mask || !tcc_re || !tcc_im || !image || Nx < 0 || Ny < 0) return LITHO_EARG;
+    // the band-grid kernels for the square sizes they cover, the general
+    // power-of-two path otherwise
+    if (Ny != Nx || !is_pow2(N) || N < 256 || N > 1024 || 4 * Nx + 1 > N)
+        return litho_image_tcc_rect(mask, N, N, tcc_re, tcc_im, Nx, Ny, image);
     litho_ctx* c;
     LITHO_TRY(default_ctx(&c));
     hipStream_t st = c->stream;

@@ -303,4 +303,39 @@ int launch_tcc_image_spectrum(const float2* M, const double2* T, int Nx, int Ny,
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
+// The column pass of the TCC-direct image on any Ly: from the band spectrum
+// C [2Nx+1][4Ny+1] (kx2 = 0..2Nx, ky2 = -2Ny..2Ny, as the spectrum kernel
+// writes it) to V [Ly][2Nx+1] = s (-1)^(kx2 + ky2) sum_ky2 C e^{+2 pi i ky2 y / Ly}.
+// The (-1) factors put the phase origin at the clip centre (the fftshift of
+// FT_c2r_padded, litho_thread.cpp L371-425); the row pass is a plain c2r.
+__global__ __launch_bounds__(256) void tcc_image_cols_kernel(const float2* __restrict__ C, int Nx,
+                                                             int Ny, int Ly, float s,
+                                                             float2* __restrict__ V) {
+    const int nb = 2 * Nx + 1, nk = 4 * Ny + 1;
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)Ly * nb) return;
+    const int y = (int)(e / nb), kx2 = (int)(e - (long)y * nb);
+    double ar = 0.0, ai = 0.0;
+    for (int j = 0; j < nk; j++) {
+        const int ky2 = j - 2 * Ny;
+        const long ph = (((long)ky2 * y) % Ly + Ly) % Ly;
+        double sn, cs;
+        sincospi(2.0 * (double)ph / Ly, &sn, &cs);
+        const float2 c = C[(long)kx2 * nk + j];
+        const double sg = (ky2 & 1) ? -1.0 : 1.0;
+        ar += sg * (c.x * cs - c.y * sn);
+        ai += sg * (c.x * sn + c.y * cs);
+    }
+    const double f = ((kx2 & 1) ? -1.0 : 1.0) * s;
+    V[e] = make_float2((float)(f * ar), (float)(f * ai));
+}
+
+int launch_tcc_image_cols(const float2* C, int Nx, int Ny, int Ly, float s, float2* V,
+                          hipStream_t st) {
+    const long n = (long)Ly * (2 * Nx + 1);
+    hipLaunchKernelGGL(tcc_image_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       C, Nx, Ny, Ly, s, V);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
 }  // namespace litho

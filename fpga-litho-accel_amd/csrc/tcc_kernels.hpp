@@ -36,5 +36,9 @@ int launch_herm_gemm_bf16s(const double2* X, const double* sx, const double2* Y,
 // [(2Ny+1)][(2Nx+1)] and the n x n TCC (calcImage)
 int launch_tcc_image_spectrum(const float2* M, const double2* T, int Nx, int Ny, float2* C,
                               hipStream_t st);
+// band spectrum C (as launch_tcc_image_spectrum writes it) -> V [Ly][2Nx+1]:
+// the inverse column DFT with the centring signs, times s
+int launch_tcc_image_cols(const float2* C, int Nx, int Ny, int Ly, float s, float2* V,
+                          hipStream_t st);
 
 }  // namespace litho

@@ -32,7 +32,7 @@ EXPORTS = [
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_setup_timing", "litho_tmpimg_grid", "litho_socs_tmpimg",
     "litho_socs_tmpimg_on_grid", "litho_socs_tmpimg_dev", "litho_socs_image", "litho_fi",
-    "litho_image_tcc",
+    "litho_image_tcc", "litho_image_tcc_rect",
     "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
     "litho_socs_image_batch_noalloc",
     "litho_ctx_set_path", "litho_ctx_set_timing",
@@ -151,6 +151,7 @@ def lib():
         L.litho_socs_image.argtypes = [_f32p, i, _f32p, _f32p, _f32p, i, i, i, _f32p]
         L.litho_fi.argtypes = [_f32p, i, i, i, _f32p]
         L.litho_image_tcc.argtypes = [_f32p, i, _f64p, _f64p, i, i, _f32p]
+        L.litho_image_tcc_rect.argtypes = [_f32p, i, i, _f64p, _f64p, i, i, _f32p]
         L.litho_bank_create.argtypes = [vp, _f32p, _f32p, _f32p, i, i, i, C.POINTER(vp)]
         L.litho_bank_destroy.argtypes = [vp]
         L.litho_bank_info.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
@@ -350,15 +351,19 @@ def calcSOCS(mskf, krns, scales, Nx, Ny):
 
 
 def calcImage(mask, tcc, Nx, Ny):
-    """TCC-direct image (calcImage + FT_c2r_padded, L539-583 / L371-425) on the GPU."""
+    """TCC-direct image (calcImage + FT_c2r_padded, L539-583 / L371-425) on the GPU;
+    mask [Ly][Lx] with power-of-two sides."""
     mask = np.ascontiguousarray(mask, np.float32)
-    N = mask.shape[0]
+    Ly, Lx = mask.shape
     tcc = np.asarray(tcc, np.complex128)
-    out = np.zeros(N * N, np.float32)
-    _check(lib().litho_image_tcc(mask.ravel(), N, np.ascontiguousarray(tcc.real).ravel(),
-                                 np.ascontiguousarray(tcc.imag).ravel(), Nx, Ny, out),
-           "litho_image_tcc")
-    return out.reshape(N, N)
+    tr, ti = np.ascontiguousarray(tcc.real).ravel(), np.ascontiguousarray(tcc.imag).ravel()
+    out = np.zeros(Lx * Ly, np.float32)
+    if Lx == Ly:
+        _check(lib().litho_image_tcc(mask.ravel(), Lx, tr, ti, Nx, Ny, out), "litho_image_tcc")
+    else:
+        _check(lib().litho_image_tcc_rect(mask.ravel(), Lx, Ly, tr, ti, Nx, Ny, out),
+               "litho_image_tcc_rect")
+    return out.reshape(Ly, Lx)
 
 
 def socs_image(mask, krns, scales, Nx, Ny):

@@ -182,10 +182,15 @@ int litho_fi(const float* tmpImgp, int G, int N, int band, float* image);
 
 /* ---- Module 5 (verification): calcImage + FT_c2r_padded ------------------- */
 /* TCC-direct aerial image (litho_thread.cpp L539-583, L371-425): the full-rank
- * reference image SOCS converges to as nk -> n.  Square N (256, 512, 1024),
- * Nx == Ny, tcc_* n x n (both triangles read). */
+ * reference image SOCS converges to as nk -> n.  N a power of two, 32..4096,
+ * 4 Nx + 1 <= N and 4 Ny + 1 <= N, tcc_* n x n with n = (2Nx+1)(2Ny+1) (both
+ * triangles read; non-finite entries are rejected). */
 int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double* tcc_im, int Nx,
                     int Ny, float* image);
+/* The same on a rectangular Ly x Lx clip (calcImage's fftLx x fftLy): mask and
+ * image [Ly][Lx], Lx and Ly powers of two in 32..4096. */
+int litho_image_tcc_rect(const float* mask, int Lx, int Ly, const double* tcc_re,
+                         const double* tcc_im, int Nx, int Ny, float* image);
 
 /* ---- device-resident batched path (the bench path) ------------------------- */
 /* Upload one kernel set (f32 planes as in kernels/krn_k_{r,i}.bin, each

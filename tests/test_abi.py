@@ -48,6 +48,16 @@ def test_arguments_rejected_before_device_work(gpu):
     tn[10] = np.nan  # a non-finite TCC is refused before the eigensolver
     assert L.litho_kernels_topk(tn, np.zeros(81), 9, 1, 1, 2, sc, k, k, None) == -1
     assert L.litho_socs_image_batch(None, None, None, 1, 512, None) == -1
+    # TCC-direct image: sides must be powers of two in 32..4096 holding the band
+    img = np.zeros(64 * 96, np.float32)
+    te = np.eye(45).ravel()
+    z = np.zeros(45 * 45)
+    assert L.litho_image_tcc_rect(img, 96, 64, te, z, 4, 2, img) == -1  # 96 not a power of two
+    assert L.litho_image_tcc_rect(img, 64, 16, te, z, 4, 2, img) == -1  # below 32
+    assert L.litho_image_tcc_rect(img, 64, 32, te, z, 16, 2, img) == -1  # 4Nx+1 > Lx
+    te[3] = np.inf
+    assert L.litho_image_tcc_rect(img, 64, 32, te, z, 4, 2, img) == -1  # non-finite TCC
+    assert L.litho_image_tcc(img, 64, te, z, 4, 2, img) == -1
 
 
 def test_tmpimg_grid_and_arguments(gpu):

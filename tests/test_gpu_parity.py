@@ -482,6 +482,29 @@ def test_tcc_direct_image_c1_and_full_rank_socs(gpu, orc, clip512):
     assert max_rel(socs, img) < 1e-4
 
 
+@pytest.mark.parametrize("shape,nxy,period", [((128, 128), (4, 4), (512, 512)),
+                                              ((64, 128), (4, 2), (512, 256)),
+                                              ((256, 64), (3, 8), (256, 1024)),
+                                              ((2048, 2048), (4, 4), (448, 448))])
+def test_tcc_direct_image_general_shapes(gpu, orc, shape, nxy, period):
+    """calcImage on the general power-of-two path: square sizes outside the band
+    grids (128^2, 2048^2), rectangular Ly x Lx clips and Nx != Ny, against the
+    oracle's calcImage + FT_c2r_padded on the same spectrum.  The TCC depends on
+    the period (nm) and Nx, Ny only; the 2048^2 case takes a 448 nm period so
+    that Nx = 4 covers the pupil (n = 81)."""
+    Ly, Lx = shape
+    Nx, Ny = nxy
+    case(f"tcc-direct image {Ly}x{Lx}, Nx={Nx}, Ny={Ny}")
+    src = orc.source_annular(41, 0.6, 0.9)
+    s = orc.outer_sigma(src)
+    T = orc.calc_tcc(src, s, BASE_LAMBDA, BASE_NA, 0.0, period[0], period[1], Nx, Ny)
+    m = gpu.manhattan_mask(max(Lx, Ly), 91)[:Ly, :Lx]
+    img = gpu.calcImage(m, T, Nx, Ny)
+    ref = orc.calc_image_tcc(orc.mask_spectrum(m), T, Nx, Ny)
+    assert img.shape == (Ly, Lx)
+    assert max_rel(img, ref) < IMG_TOL
+
+
 def test_large_clip_implicit_tcc(gpu, orc, base_source):
     """4096^2 clip at 4 nm (period 16384 nm): Nx = 137, a 75625-dim TCC that is
     never formed; the source-space Gram route gives K = 64 kernels (checked
