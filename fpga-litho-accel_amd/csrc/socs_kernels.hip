@@ -1528,6 +1528,158 @@ __global__ __launch_bounds__(256) void image_rows_kernel(const float2* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// 4096-point row transforms of a row pair (two real rows as one complex
+// sequence), three radix-16 stages in registers, one pair per 256-thread block:
+//   n = t + 256 n3 (t = n1 + 16 n2), k = a + 16 c + 256 d
+//   A: thread t, DFT16 over n3 -> a, times W_4096^(t a)
+//   B: thread (a, n1), DFT16 over n2 -> c, times W_256^(n1 c)
+//   C: thread (a, c), DFT16 over n1 -> d: X[a + 16 c + 256 d]
+// with two transposes through LDS.  The inter-stage twiddles come from two
+// 64-entry tables, W_4096^m = W_64^(m >> 6) W_4096^(m & 63).  The inverse is
+// the same algebra with conjugate roots (input index k, output index n).
+// ---------------------------------------------------------------------------
+constexpr int F4K_P1 = 257;  // pitch of the [a][t] transpose (t < 256)
+constexpr int F4K_P2 = 17;   // pitch of the [a c][n1] transpose
+constexpr int F4K_BUF = 256 * F4K_P2 > 16 * F4K_P1 ? 256 * F4K_P2 : 16 * F4K_P1;
+
+__device__ __forceinline__ void f4k_tables(float2* __restrict__ lo, float2* __restrict__ hi,
+                                           int tid) {
+    if (tid < 128) {
+        const int j = tid & 63;
+        float sn, cs;
+        // lo: W_4096^j; hi: W_64^j = W_4096^(64 j)
+        sincospif(2.0f * (float)(tid < 64 ? j : 64 * j) / 4096.0f, &sn, &cs);
+        (tid < 64 ? lo : hi)[j] = make_float2(cs, -sn);
+    }
+}
+
+template <bool INV>
+__device__ __forceinline__ float2 f4k_tw(const float2* __restrict__ lo,
+                                         const float2* __restrict__ hi, int m, float2 v) {
+    const float2 w = cmul(hi[(m >> 6) & 63], lo[m & 63]);
+    return cmul_w<false>(v, INV ? make_float2(w.x, -w.y) : w);
+}
+
+// v (thread t: inputs t + 256 n3, n3 = 0..15) -> v (thread (a, c) = (tid >> 4,
+// tid & 15): outputs a + 16 c + 256 d, d = 0..15).  All 256 threads call it.
+template <bool INV>
+__device__ __forceinline__ void fft4096_pair(int tid, float2 (&v)[16], float2* __restrict__ buf,
+                                             const float2* __restrict__ lo,
+                                             const float2* __restrict__ hi) {
+    dft_reg<16, INV>(v);
+#pragma unroll
+    for (int a = 1; a < 16; a++) v[a] = f4k_tw<INV>(lo, hi, tid * a, v[a]);
+#pragma unroll
+    for (int a = 0; a < 16; a++) buf[a * F4K_P1 + tid] = v[a];
+    __syncthreads();
+    const int a = tid >> 4, r = tid & 15;  // stage B: (a, n1 = r); stage C: (a, c = r)
+#pragma unroll
+    for (int n2 = 0; n2 < 16; n2++) v[n2] = buf[a * F4K_P1 + r + 16 * n2];
+    dft_reg<16, INV>(v);
+#pragma unroll
+    for (int c = 1; c < 16; c++) v[c] = f4k_tw<INV>(lo, hi, 16 * r * c, v[c]);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 16; c++) buf[(16 * a + c) * F4K_P2 + r] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) v[n1] = buf[(16 * a + r) * F4K_P2 + n1];
+    dft_reg<16, INV>(v);
+}
+
+// Mask rows of 4096 pixels -> window R [rows][count] (kx = kmin + j, |kx| < 256),
+// as rows_dft_kernel: two real rows per transform, (-1)^kx when ALT.
+template <bool ALT>
+__global__ __launch_bounds__(256) void rows4096_kernel(const float* __restrict__ in,
+                                                       float2* __restrict__ out, int nrows,
+                                                       int kmin, int count) {
+    constexpr int L = 4096;
+    __shared__ float2 buf[F4K_BUF];
+    __shared__ float2 lo[64], hi[64];
+    const int tid = threadIdx.x;
+    const long p = blockIdx.x;  // row pair
+    const float* r0 = in + 2 * p * (long)L + tid;
+    float2 v[16];
+#pragma unroll
+    for (int n3 = 0; n3 < 16; n3++)
+        v[n3] = make_float2(__builtin_nontemporal_load(r0 + 256 * n3),
+                            __builtin_nontemporal_load(r0 + L + 256 * n3));
+    f4k_tables(lo, hi, tid);
+    __syncthreads();
+    fft4096_pair<false>(tid, v, buf, lo, hi);
+    // X[k] for |k| < 256: d = 0 (k = a + 16 c) and d = 15 (k - 4096)
+    __syncthreads();
+    buf[tid] = v[0];
+    buf[256 + tid] = v[15];
+    __syncthreads();
+    auto at = [&](int k) {  // X[k mod 4096], |k| < 256
+        const int m = k >= 0 ? k : k + 256;  // -k' -> 4096 - k' = 3840 + (256 - k')
+        return k >= 0 ? buf[(m & 15) * 16 + (m >> 4)] : buf[256 + (m & 15) * 16 + (m >> 4)];
+    };
+    for (int j = tid; j < count; j += 256) {
+        const int k = kmin + j;
+        const float2 z1 = at(k), z2 = cconj(at(-k));
+        float2 x1 = cscale(cadd(z1, z2), 0.5f);
+        const float2 dd = csub(z1, z2);
+        float2 x2 = make_float2(0.5f * dd.y, -0.5f * dd.x);
+        if (ALT && (k & 1)) {
+            x1 = cscale(x1, -1.f);
+            x2 = cscale(x2, -1.f);
+        }
+        out[(2 * p) * (long)count + j] = x1;
+        out[(2 * p + 1) * (long)count + j] = x2;
+    }
+}
+
+// U [rows][band+1] (kx = 0..band) -> image rows of 4096 pixels, as
+// image_rows_kernel: the c2r of two rows as one complex inverse transform.
+__global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict__ U,
+                                                        float* __restrict__ img, int nrows,
+                                                        int band) {
+    constexpr int L = 4096;
+    __shared__ float2 buf[F4K_BUF];
+    __shared__ float2 lo[64], hi[64];
+    const int tid = threadIdx.x;
+    const long p = blockIdx.x;
+    const int nb = band + 1;
+    const float2* u1 = U + (2 * p) * (long)nb;
+    const float2* u2 = u1 + nb;
+    // spectrum S[k], k = tid + 256 d: u1 + i u2 at k <= band, conj(u1) + i conj(u2) at
+    // L - k (k = 1..band), zero elsewhere
+    float2 v[16];
+#pragma unroll
+    for (int d = 0; d < 16; d++) {
+        const int k = tid + 256 * d;
+        float2 sv = make_float2(0.f, 0.f);
+        if (k <= band) {
+            const float2 a1 = u1[k], a2 = u2[k];
+            sv = k == 0 ? make_float2(a1.x, a2.x) : make_float2(a1.x - a2.y, a1.y + a2.x);
+        } else if (L - k <= band) {
+            const float2 a1 = u1[L - k], a2 = u2[L - k];
+            sv = make_float2(a1.x + a2.y, a2.x - a1.y);
+        }
+        v[d] = sv;
+    }
+    f4k_tables(lo, hi, tid);
+    __syncthreads();
+    fft4096_pair<true>(tid, v, buf, lo, hi);
+    // thread (a, c) holds pixels a + 16 c + 256 d: back through LDS in pixel
+    // order, so each store instruction writes 1 KB of a row
+    const int n0 = (tid >> 4) + 16 * (tid & 15);
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 16; d++) buf[n0 + 256 * d] = v[d];
+    __syncthreads();
+    float* o1 = img + (2 * p) * (long)L + tid;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const float2 z = buf[tid + 256 * j];
+        __builtin_nontemporal_store(z.x, o1 + 256 * j);
+        __builtin_nontemporal_store(z.y, o1 + L + 256 * j);
+    }
+}
+
 // ===========================================================================
 // host-side launchers with size dispatch
 // ===========================================================================
@@ -1549,6 +1701,17 @@ static int launch_rows_dft_L(const float* in, float2* out, long nrows, int kmin,
 int launch_rows_dft(int L, const float* in, float2* out, long nrows, int kmin, int count,
                     bool alt, const float2* tw, hipStream_t st) {
     if (nrows % 2 || count <= 0 || nrows > 0x7fffffffL) return LITHO_EARG;
+    if (L == 4096 && kmin > -256 && kmin + count - 1 < 256 && nrows / 2 <= 0x7fffffffL) {
+        // the register transform: a window of |kx| < 256 (every Nx of a 4096 clip)
+        const dim3 grid((unsigned)(nrows / 2));
+        if (alt)
+            hipLaunchKernelGGL((rows4096_kernel<true>), grid, dim3(256), 0, st, in, out,
+                               (int)nrows, kmin, count);
+        else
+            hipLaunchKernelGGL((rows4096_kernel<false>), grid, dim3(256), 0, st, in, out,
+                               (int)nrows, kmin, count);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    }
 #define LITHO_RF2(p, q, nth)                                                                  \
     if (L == p * q) {                                                                         \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                          \
@@ -1867,6 +2030,11 @@ static int launch_image_rows_N(const float2* U, float* img, long nrows, int band
 int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, const float2* tw,
                       hipStream_t st) {
     if (nrows % 2 || 2 * band + 1 > N || nrows > 0x7fffffffL) return LITHO_EARG;
+    if (N == 4096) {
+        hipLaunchKernelGGL(image4096_kernel, dim3((unsigned)(nrows / 2)), dim3(256), 0, st, U, img,
+                           (int)nrows, band);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    }
 #define LITHO_IR2(p, q, nz, nth)                                                          \
     if (N == p * q && 2 * band + 1 <= nz * q) {                                           \
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                      \
