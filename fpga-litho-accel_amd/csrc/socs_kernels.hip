@@ -315,6 +315,14 @@ __global__ __launch_bounds__(512) void socs_accum_kernel(const float2* __restric
 
 
 // ---------------------------------------------------------------------------
+// LDS written by some lanes of a wavefront and read by others of the same
+// wavefront: a wavefront-scope fence orders it (no block barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // socs_accum2: same contract as socs_accum, with the conv-grid FFTs done as
 // two-stage register FFTs (G = P*Q, T = min(P,Q) threads per transform, one
 // LDS transpose).  The window is placed at indices [0, W) on both axes (a
@@ -342,6 +350,7 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     constexpr int RB = GR / SLOTS;
     constexpr int PER = Geo::PER;
     static_assert(GR % SLOTS == 0 && RB >= 1, "rows per block must fill the slots");
+    static_assert(64 % T == 0, "a transform's lanes within one wavefront");
     extern __shared__ float2 lds[];
     float2* twG = lds;
     float2* xbuf = lds + G;
@@ -363,15 +372,9 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     for (int k = 0; k < nk; k++) {
         const long kbase = (long)k * W * W;
         const float lam = scales[k];
-        if constexpr (PRE) {
-            // rows g0 .. g0 + GR - 1 of entry k: GR consecutive values per column
-            const float2* yk = Yt + ((b * nk + k) * (long)W) * G + g0;
-            for (int i = tid; i < GR * W; i += NTH) {
-                const int col = i / GR, r = i - col * GR;
-                Y[r * W + col] = yk[(long)col * G + r];
-            }
-            __syncthreads();
-        }
+        // PRE: each transform reads its row of entry k straight from the
+        // column pass's output (the block's GR rows share each 128-byte line)
+        const float2* yk = PRE ? Yt + ((b * nk + k) * (long)W) * G + g0 : nullptr;
         for (int c0 = 0; c0 < (PRE ? 0 : W); c0 += SLOTS) {
             const int col = c0 + slot;
             const bool act = col < W;
@@ -390,17 +393,23 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
             });
             __syncthreads();
         }
+        // each transform's T lanes lie in one wavefront, as does its transpose
+        // buffer xb: with the rows read in place (PRE), wave-level syncs order it
 #pragma unroll
         for (int rb = 0; rb < RB; rb++) {
-            const float2* yr = Y + (rb * SLOTS + slot) * W;
+            const int r = rb * SLOTS + slot;
+            const float2* yr = Y + r * W;
             fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
-                return n < W ? yr[n] : make_float2(0.f, 0.f);
+                if (n >= W) return make_float2(0.f, 0.f);
+                return PRE ? yk[(long)n * G + r] : yr[n];
             });
-            __syncthreads();
+            if (PRE) wave_sync();
+            else __syncthreads();
             fft2_stage2<Geo, true>(t, xb, [&](int sidx, int, float2 val) {
                 acc[rb * PER + sidx] += lam * (val.x * val.x + val.y * val.y);
             });
-            __syncthreads();
+            if (PRE) wave_sync();
+            else __syncthreads();
         }
     }
     float* ob = out + b * (long)G * G;
@@ -495,14 +504,6 @@ __global__ __launch_bounds__(NTH) void rows_fwd2_kernel(const void* __restrict__
             out[(2 * p + 1) * (long)count + j] = S::st(x2);
         }
     }
-}
-
-// LDS written by some lanes of a wavefront and read by others of the same
-// wavefront: a wavefront-scope fence orders it (no block barrier).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Stage transpose of a 32 x 32 two-stage FFT through a float plane buffer xb
@@ -1680,6 +1681,80 @@ __global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict
     }
 }
 
+// Column transforms of 4096 points on the same register core, one column per
+// block.  mask_cols4096: R [B][4096][cx] -> M [B][cy][cx] as mask_cols_kernel
+// (ky = kymin + j, |ky| < 256, times (-1)^ky / N^2).
+__global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __restrict__ R,
+                                                            float2* __restrict__ M, int cx,
+                                                            int kymin, int cy) {
+    constexpr int N = 4096;
+    __shared__ float2 buf[F4K_BUF];
+    __shared__ float2 lo[64], hi[64];
+    const int tid = threadIdx.x;
+    const int col = blockIdx.x;
+    const long b = blockIdx.y;
+    const float2* Rc = R + b * (long)N * cx + col;
+    float2 v[16];
+#pragma unroll
+    for (int n3 = 0; n3 < 16; n3++) v[n3] = Rc[(long)(tid + 256 * n3) * cx];
+    f4k_tables(lo, hi, tid);
+    __syncthreads();
+    fft4096_pair<false>(tid, v, buf, lo, hi);
+    __syncthreads();
+    buf[tid] = v[0];         // X[a + 16 c]
+    buf[256 + tid] = v[15];  // X[a + 16 c + 3840]
+    __syncthreads();
+    const float inv = 1.0f / ((float)N * (float)N);
+    float2* Mb = M + b * (long)cy * cx;
+    for (int j = tid; j < cy; j += 256) {
+        const int ky = kymin + j;
+        const int m = ky >= 0 ? ky : ky + 256;
+        const float2 x = buf[(ky >= 0 ? 0 : 256) + (m & 15) * 16 + (m >> 4)];
+        Mb[(long)j * cx + col] = cscale(x, (ky & 1) ? -inv : inv);
+    }
+}
+
+// fi_cols4096: P [B][G][nb] -> U [B][4096][nb] as fi_cols_kernel: the G-point
+// forward transform of a column in LDS, the band |ky| <= band embedded in a
+// 4096-point spectrum with the centring signs, the inverse on the register core.
+template <int G>
+__global__ __launch_bounds__(256) void fi_cols4096_kernel(const float2* __restrict__ P,
+                                                          float2* __restrict__ U, int band,
+                                                          int yalt,
+                                                          const float2* __restrict__ tw) {
+    constexpr int N = 4096;
+    __shared__ float2 sa[G], sb[G];
+    __shared__ float2 buf[F4K_BUF];
+    __shared__ float2 lo[64], hi[64];
+    const int tid = threadIdx.x;
+    const int nb = band + 1;
+    const int col = blockIdx.x;
+    const long b = blockIdx.y;
+    const float2* Pb = P + b * (long)G * nb + col;
+    for (int g = tid; g < G; g += 256) sa[g] = Pb[(long)g * nb];
+    f4k_tables(lo, hi, tid);
+    __syncthreads();
+    const float2* r1 = fft_lds<G, false>(sa, sb, 1, tw, tid, 256);
+    const float inv = 1.0f / ((float)G * (float)G);
+    float2 v[16];
+#pragma unroll
+    for (int d = 0; d < 16; d++) {
+        const int y = tid + 256 * d;
+        const int ky = y < N / 2 ? y : y - N;
+        float2 x = make_float2(0.f, 0.f);
+        if (ky >= -band && ky <= band)
+            x = cscale(r1[((ky % G) + G) % G], (!yalt && (ky & 1)) ? -inv : inv);
+        v[d] = x;
+    }
+    fft4096_pair<true>(tid, v, buf, lo, hi);
+    // thread (a, c): rows y = a + 16 c + 256 d
+    const int y0 = (tid >> 4) + 16 * (tid & 15);
+    float2* Uc = U + b * (long)N * nb + col;
+    const float sg = (col & 1) ? -1.f : 1.f;
+#pragma unroll
+    for (int d = 0; d < 16; d++) Uc[(long)(y0 + 256 * d) * nb] = cscale(v[d], sg);
+}
+
 // ===========================================================================
 // host-side launchers with size dispatch
 // ===========================================================================
@@ -1753,6 +1828,11 @@ static int launch_mask_cols_N(const float2* R, float2* M, int B, int cx, int kym
 int launch_mask_cols(int N, const float2* R, float2* M, int B, int cx, int kymin, int cy,
                      const float2* tw, hipStream_t st) {
     if (B <= 0 || B > 65535 || cx <= 0 || cy <= 0) return LITHO_EARG;
+    if (N == 4096 && kymin > -256 && kymin + cy - 1 < 256) {
+        hipLaunchKernelGGL(mask_cols4096_kernel, dim3(cx, B), dim3(256), 0, st, R, M, cx, kymin,
+                           cy);
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    }
     switch (N) {
         case 32: return launch_mask_cols_N<32>(R, M, B, cx, kymin, cy, tw, st);
         case 64: return launch_mask_cols_N<64>(R, M, B, cx, kymin, cy, tw, st);
@@ -1851,7 +1931,7 @@ static int launch_socs2_pre(const float2* M, const void* bank, const float2* ksc
     constexpr int SLOTS = NTH / Geo::T;
     const int W = 2 * Nx + 1;
     const size_t lds_c = (size_t)(G + SLOTS * Geo::XBUF) * sizeof(float2);
-    const size_t lds_r = (size_t)(G + SLOTS * Geo::XBUF + GR * W) * sizeof(float2);
+    const size_t lds_r = (size_t)(G + SLOTS * Geo::XBUF) * sizeof(float2);  // rows read in place
     if (lds_c > 160 * 1024 || lds_r > 160 * 1024) return LITHO_EARG;
     auto kc = socs_cols_kernel<P, Q, NZ, NTH, BK>;
     auto kr = socs_accum2_kernel<P, Q, GR, NZ, NTH, BK, true>;
@@ -1997,7 +2077,11 @@ static int launch_fi_cols_G(int N, const float2* P, float2* U, int B, int band, 
         case 512: if constexpr (G <= 512) return launch_fi_cols_GN<G, 512>(P, U, B, band, yalt, tw, st); break;
         case 1024: if constexpr (G <= 1024) return launch_fi_cols_GN<G, 1024>(P, U, B, band, yalt, tw, st); break;
         case 2048: return launch_fi_cols_GN<G, 2048>(P, U, B, band, yalt, tw, st);
-        case 4096: return launch_fi_cols_GN<G, 4096>(P, U, B, band, yalt, tw, st);
+        case 4096:
+            // the 4096-point inverse on the register core, one column per block
+            hipLaunchKernelGGL((fi_cols4096_kernel<G>), dim3(band + 1, B), dim3(256), 0, st, P, U,
+                               band, yalt, tw);
+            return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
         default: break;
     }
     return LITHO_EARG;
