@@ -1543,6 +1543,8 @@ __global__ __launch_bounds__(256) void image_rows_kernel(const float2* __restric
 constexpr int F4K_P1 = 257;  // pitch of the [a][t] transpose (t < 256)
 constexpr int F4K_P2 = 17;   // pitch of the [a c][n1] transpose
 constexpr int F4K_BUF = 256 * F4K_P2 > 16 * F4K_P1 ? 256 * F4K_P2 : 16 * F4K_P1;
+static_assert(F4K_BUF >= 4096 + 4096 / 32, "room for the padded pixel-order copy");
+constexpr int F4K_NEG = 256 + 256 / 32;  // the d = 15 band after the d = 0 band (padded)
 
 __device__ __forceinline__ void f4k_tables(float2* __restrict__ lo, float2* __restrict__ hi,
                                            int tid) {
@@ -1609,14 +1611,16 @@ __global__ __launch_bounds__(256) void rows4096_kernel(const float* __restrict__
     f4k_tables(lo, hi, tid);
     __syncthreads();
     fft4096_pair<false>(tid, v, buf, lo, hi);
-    // X[k] for |k| < 256: d = 0 (k = a + 16 c) and d = 15 (k - 4096)
+    // X[k] for |k| < 256 in natural order: d = 0 (k = a + 16 c) and d = 15
+    // (k - 4096); entry m at m + m / 32 (lanes 16 apart on different banks)
     __syncthreads();
-    buf[tid] = v[0];
-    buf[256 + tid] = v[15];
+    const int m0 = (tid >> 4) + 16 * (tid & 15);
+    buf[m0 + (m0 >> 5)] = v[0];
+    buf[F4K_NEG + m0 + (m0 >> 5)] = v[15];
     __syncthreads();
     auto at = [&](int k) {  // X[k mod 4096], |k| < 256
         const int m = k >= 0 ? k : k + 256;  // -k' -> 4096 - k' = 3840 + (256 - k')
-        return k >= 0 ? buf[(m & 15) * 16 + (m >> 4)] : buf[256 + (m & 15) * 16 + (m >> 4)];
+        return buf[(k >= 0 ? 0 : F4K_NEG) + m + (m >> 5)];
     };
     for (int j = tid; j < count; j += 256) {
         const int k = kmin + j;
@@ -1666,16 +1670,21 @@ __global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict
     __syncthreads();
     fft4096_pair<true>(tid, v, buf, lo, hi);
     // thread (a, c) holds pixels a + 16 c + 256 d: back through LDS in pixel
-    // order, so each store instruction writes 1 KB of a row
+    // order, so each store instruction writes 1 KB of a row (pixel n at
+    // n + n / 32: lanes 16 pixels apart fall on different banks)
     const int n0 = (tid >> 4) + 16 * (tid & 15);
     __syncthreads();
 #pragma unroll
-    for (int d = 0; d < 16; d++) buf[n0 + 256 * d] = v[d];
+    for (int d = 0; d < 16; d++) {
+        const int n = n0 + 256 * d;
+        buf[n + (n >> 5)] = v[d];
+    }
     __syncthreads();
     float* o1 = img + (2 * p) * (long)L + tid;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-        const float2 z = buf[tid + 256 * j];
+        const int n = tid + 256 * j;
+        const float2 z = buf[n + (n >> 5)];
         __builtin_nontemporal_store(z.x, o1 + 256 * j);
         __builtin_nontemporal_store(z.y, o1 + L + 256 * j);
     }
@@ -1701,15 +1710,16 @@ __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __rest
     __syncthreads();
     fft4096_pair<false>(tid, v, buf, lo, hi);
     __syncthreads();
-    buf[tid] = v[0];         // X[a + 16 c]
-    buf[256 + tid] = v[15];  // X[a + 16 c + 3840]
+    const int m0 = (tid >> 4) + 16 * (tid & 15);
+    buf[m0 + (m0 >> 5)] = v[0];            // X[m0]
+    buf[F4K_NEG + m0 + (m0 >> 5)] = v[15];  // X[m0 + 3840]
     __syncthreads();
     const float inv = 1.0f / ((float)N * (float)N);
     float2* Mb = M + b * (long)cy * cx;
     for (int j = tid; j < cy; j += 256) {
         const int ky = kymin + j;
         const int m = ky >= 0 ? ky : ky + 256;
-        const float2 x = buf[(ky >= 0 ? 0 : 256) + (m & 15) * 16 + (m >> 4)];
+        const float2 x = buf[(ky >= 0 ? 0 : F4K_NEG) + m + (m >> 5)];
         Mb[(long)j * cx + col] = cscale(x, (ky & 1) ? -inv : inv);
     }
 }
