@@ -485,13 +485,16 @@ def test_tcc_direct_image_c1_and_full_rank_socs(gpu, orc, clip512):
 @pytest.mark.parametrize("shape,nxy,period", [((128, 128), (4, 4), (512, 512)),
                                               ((64, 128), (4, 2), (512, 256)),
                                               ((256, 64), (3, 8), (256, 1024)),
-                                              ((2048, 2048), (4, 4), (448, 448))])
+                                              ((2048, 2048), (4, 4), (448, 448)),
+                                              ((256, 4096), (8, 4), (896, 448)),
+                                              ((4096, 256), (4, 8), (448, 896))])
 def test_tcc_direct_image_general_shapes(gpu, orc, shape, nxy, period):
     """calcImage on the general power-of-two path: square sizes outside the band
     grids (128^2, 2048^2), rectangular Ly x Lx clips and Nx != Ny, against the
     oracle's calcImage + FT_c2r_padded on the same spectrum.  The TCC depends on
     the period (nm) and Nx, Ny only; the 2048^2 case takes a 448 nm period so
-    that Nx = 4 covers the pupil (n = 81)."""
+    that Nx = 4 covers the pupil (n = 81).  The 4096-point sides run the
+    register-core row, column and image-row transforms."""
     Ly, Lx = shape
     Nx, Ny = nxy
     case(f"tcc-direct image {Ly}x{Lx}, Nx={Nx}, Ny={Ny}")
