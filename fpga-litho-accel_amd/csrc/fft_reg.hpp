@@ -164,8 +164,52 @@ __device__ __forceinline__ void dft_reg3(float2* v) {
     }
 }
 
+// Split of N into coprime A = 3^a and B = 2^b (both > 1), or A = 1.
+constexpr int pfa_three_part(int n) {
+    int a = 1;
+    while (n % 3 == 0) {
+        n /= 3;
+        a *= 3;
+    }
+    return n > 1 && a > 1 ? a : 1;
+}
+
+constexpr int inverse_mod(int x, int m) {
+    for (int y = 1; y < m; y++)
+        if ((x * y) % m == 1) return y;
+    return 1;
+}
+
+// N = A B with gcd(A, B) = 1 by the prime-factor (Good-Thomas) mapping:
+// input n = (B n1 + A n2) mod N, output k = (B (B^-1 mod A) k1 + A (A^-1 mod B) k2)
+// mod N, so W_N^{nk} = W_A^{n1 k1} W_B^{n2 k2}: A B-point DFTs, then B
+// A-point DFTs, and no twiddle products between them.  The index maps are
+// compile-time after unrolling.
+template <int N, int A, bool INV>
+__device__ __forceinline__ void dft_reg_pfa(float2* v) {
+    constexpr int B = N / A;
+    constexpr int CA = B * inverse_mod(B % A, A), CB = A * inverse_mod(A % B, B);
+    float2 y[A][B];
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) y[n1][n2] = v[(B * n1 + A * n2) % N];
+        dft_reg<B, INV>(y[n1]);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) {
+        float2 z[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) z[n1] = y[n1][k2];
+        dft_reg<A, INV>(z);
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) v[(CA * k1 + CB * k2) % N] = z[k1];
+    }
+}
+
 // N = R M by decimation in time (R = 4 when 4 | N and N > 4, else 2, else 3):
 // R interleaved M-point DFTs, twiddles W_N^{r k}, then R-point DFTs across.
+// Sizes with both factors 2 and 3 take the prime-factor split instead.
 template <int N, bool INV>
 __device__ __forceinline__ void dft_reg(float2* v) {
     if constexpr (N == 1) {
@@ -175,6 +219,8 @@ __device__ __forceinline__ void dft_reg(float2* v) {
         dft_reg3<INV>(v);
     } else if constexpr (N == 4) {
         dft_reg4<INV>(v);
+    } else if constexpr (pfa_three_part(N) > 1) {
+        dft_reg_pfa<N, pfa_three_part(N), INV>(v);
     } else {
         constexpr int R = (N % 4 == 0) ? 4 : (N % 2 == 0 ? 2 : 3);
         constexpr int M = N / R;
