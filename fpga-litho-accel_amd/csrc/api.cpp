@@ -1265,7 +1265,20 @@ namespace litho {
 // its row-DFT window P, U, and on grids of 512 points and up the SOCS column
 // pass Yt.  Every kernel overwrites its workspace in full before the next one
 // reads it, so none has to start at zero.
-int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[6], int* band_path) {
+// Conv grid of the pipeline that is not the band one: the reference's
+// nextPow2(4 Nx + 1) (litho_thread.cpp calcSOCS, L833), except 4096-pixel
+// clips on the automatic path, which take 576 = 24^2 when the intensity band
+// fits (4 Nx + 1 <= 576): the image does not depend on the grid once it holds
+// the band (exact band-limited interpolation), and 576 points do 3.4x fewer
+// row transforms of about half the length.
+static int image_grid(int path, int N, int Nx) {
+    const int G = next_pow2(4 * Nx + 1);
+    if (path == LITHO_PATH_AUTO && N == 4096 && 4 * Nx + 1 <= 576 && G > 576) return 576;
+    return G;
+}
+
+int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[6], int* band_path,
+                      int* grid) {
     if (!bank || B <= 0 || B > 65535) return LITHO_EARG;
     const int Nx = bank->Nx;
     const size_t W = 2 * Nx + 1, band = 2 * Nx;
@@ -1273,6 +1286,7 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
     const int Gb = band_grid(Nx);
     if (path == LITHO_PATH_AUTO && Gb && Gb <= N && N >= 256 && N <= 1024) {
         *band_path = 1;
+        *grid = Gb;
         sz[0] = (size_t)B * (Nx + 1) * N * sizeof(float2);
         sz[1] = (size_t)B * W * W * sizeof(float2);
         sz[2] = (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2);
@@ -1281,9 +1295,10 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
         sz[5] = 0;
         return LITHO_OK;
     }
-    const int G = next_pow2(4 * Nx + 1);
+    const int G = image_grid(path, N, Nx);
     if (!is_pow2(N) || G > N || G > 1024) return LITHO_EARG;
     *band_path = 0;
+    *grid = G;
     sz[0] = (size_t)B * N * W * sizeof(float2);
     sz[1] = (size_t)B * W * W * sizeof(float2);
     sz[2] = (size_t)B * G * G * sizeof(float);
@@ -1296,7 +1311,7 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
 // The launches of one batch on the context stream, into the workspaces ws
 // (sized by image_batch_sizes); per-stage events when timing.
 int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B, int N,
-                        float* d_image, int band_path, void* const ws[6], bool timing,
+                        float* d_image, int band_path, int grid, void* const ws[6], bool timing,
                         int bank_fmt) {
     const int Nx = bank->Nx;
     const KernelSet& ks = bank->band;
@@ -1333,7 +1348,7 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
         c->timed = timing;
         return LITHO_OK;
     }
-    const int G = next_pow2(4 * Nx + 1);
+    const int G = grid;
     void *R = ws[0], *M = ws[1], *T = ws[2], *P = ws[3], *U = ws[4];
     static const char* names[] = {"mask_rows", "mask_cols", "socs_accum", "fi_rows", "fi_cols",
                                   "image_rows"};
@@ -1369,12 +1384,12 @@ int litho_socs_image_batch(litho_ctx* c, const litho_bank* bank, const float* d_
                            int N, float* d_image) {
     if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
     size_t sz[6];
-    int band_path = 0;
-    LITHO_TRY(image_batch_sizes(c->path, bank, B, N, sz, &band_path));
+    int band_path = 0, grid = 0;
+    LITHO_TRY(image_batch_sizes(c->path, bank, B, N, sz, &band_path, &grid));
     void* ws[6] = {};
     for (int i = 0; i < 6; i++)
         if (sz[i]) LITHO_TRY(ctx_ws(c, batch_slot(i), sz[i], &ws[i]));
-    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, c->timing, 0);
+    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, grid, ws, c->timing, 0);
 }
 
 // fp16 DFT table of the matrix-core kernels: kind 0 mask rows (N, kmin, count),
@@ -1613,8 +1628,9 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
                          : (precision & LITHO_PREC_BANK_BF16X2) ? 3
                                                                 : 0;
         size_t sz[6];
-        int band_path = 0;
-        LITHO_TRY(image_batch_sizes(LITHO_PATH_POW2, bank, B, N, sz, &band_path));
+        int band_path = 0, grid = 0;
+        LITHO_TRY(image_batch_sizes(c->path, bank, B, N, sz, &band_path, &grid));
+        if (band_path) return LITHO_EARG;
         void* ws[6] = {};
         for (int i = 0; i < 6; i++)
             if (sz[i]) LITHO_TRY(ctx_ws(c, batch_slot(i), sz[i], &ws[i]));
@@ -1622,10 +1638,11 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         c->timed = false;
         auto enqueue = [&]() -> int {
             return image_batch_enqueue(c, bank, (const float*)d_mask, B, N, (float*)d_image,
-                                       band_path, ws, false, bfmt);
+                                       band_path, grid, ws, false, bfmt);
         };
         if (!use_graph) return enqueue();
-        std::vector<long> key = {3, bank->serial, (long)d_mask, (long)d_image, B, N, precision};
+        std::vector<long> key = {3, bank->serial, (long)d_mask, (long)d_image, B, N, precision,
+                                 grid};
         for (int i = 0; i < 6; i++) key.push_back((long)ws[i]);
         return run_graph(c, key, enqueue);
     }

@@ -6,14 +6,15 @@
 
 namespace litho {
 
-// workspace bytes of litho_socs_image_batch (six slots, unused ones 0) and
-// whether the band pipeline runs; LITHO_EARG for a shape it does not take
+// workspace bytes of litho_socs_image_batch (six slots, unused ones 0),
+// whether the band pipeline runs and, if not, the conv grid G of the other
+// pipeline; LITHO_EARG for a shape it does not take
 int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[6],
-                      int* band_path);
+                      int* band_path, int* grid);
 // the batch's launches on the context stream into the given workspaces; the
 // kernel bank read as bank_fmt (0 float2, 1 bf16, 2 fp16, 3 bf16 hi + lo)
 int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mask, int B, int N,
-                        float* d_image, int band_path, void* const ws[6], bool timing,
+                        float* d_image, int band_path, int grid, void* const ws[6], bool timing,
                         int bank_fmt);
 int ctx_path(litho_ctx* c);
 int ctx_device(litho_ctx* c);

@@ -55,8 +55,8 @@ extern "C" int litho_socs_image_batch_noalloc(litho_ctx* c, const litho_bank* ba
     using namespace litho;
     if (!c || !bank || !d_mask || !d_image || B <= 0 || B > 65535) return LITHO_EARG;
     size_t sz[6];
-    int band_path = 0;
-    LITHO_TRY(image_batch_sizes(ctx_path(c), bank, B, N, sz, &band_path));
+    int band_path = 0, grid = 0;
+    LITHO_TRY(image_batch_sizes(ctx_path(c), bank, B, N, sz, &band_path, &grid));
     size_t off[6], total = 0;
     for (int i = 0; i < 6; i++) {
         off[i] = total;
@@ -74,5 +74,5 @@ extern "C" int litho_socs_image_batch_noalloc(litho_ctx* c, const litho_bank* ba
     void* ws[6] = {};
     for (int i = 0; i < 6; i++)
         if (sz[i]) ws[i] = base + off[i];
-    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, ws, false, 0);
+    return image_batch_enqueue(c, bank, d_mask, B, N, d_image, band_path, grid, ws, false, 0);
 }

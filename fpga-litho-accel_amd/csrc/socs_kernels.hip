@@ -350,7 +350,9 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     constexpr int RB = GR / SLOTS;
     constexpr int PER = Geo::PER;
     static_assert(GR % SLOTS == 0 && RB >= 1, "rows per block must fill the slots");
-    static_assert(64 % T == 0, "a transform's lanes within one wavefront");
+    // a transform's lanes within one wavefront: its transpose needs only
+    // wavefront-level syncs (else block barriers, T = 24 at the 576 grid)
+    constexpr bool WSYNC = 64 % T == 0;
     extern __shared__ float2 lds[];
     float2* twG = lds;
     float2* xbuf = lds + G;
@@ -403,12 +405,12 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
                 if (n >= W) return make_float2(0.f, 0.f);
                 return PRE ? yk[(long)n * G + r] : yr[n];
             });
-            if (PRE) wave_sync();
+            if (PRE && WSYNC) wave_sync();
             else __syncthreads();
             fft2_stage2<Geo, true>(t, xb, [&](int sidx, int, float2 val) {
                 acc[rb * PER + sidx] += lam * (val.x * val.x + val.y * val.y);
             });
-            if (PRE) wave_sync();
+            if (PRE && WSYNC) wave_sync();
             else __syncthreads();
         }
     }
@@ -1743,8 +1745,25 @@ __global__ __launch_bounds__(256) void fi_cols4096_kernel(const float2* __restri
     const float2* Pb = P + b * (long)G * nb + col;
     for (int g = tid; g < G; g += 256) sa[g] = Pb[(long)g * nb];
     f4k_tables(lo, hi, tid);
-    __syncthreads();
-    const float2* r1 = fft_lds<G, false>(sa, sb, 1, tw, tid, 256);
+    const float2* r1;
+    if constexpr ((G & (G - 1)) == 0) {
+        __syncthreads();
+        r1 = fft_lds<G, false>(sa, sb, 1, tw, tid, 256);
+    } else {
+        // G = Pg^2 (576 = 24^2): the register two-stage FFT by Pg lanes,
+        // twiddles W_G^m in sb, the transpose in buf (free until the inverse)
+        constexpr int Pg = G == 576 ? 24 : 0;
+        static_assert(Pg * Pg == G, "square non-power-of-two grids: 576");
+        using Geo = Fft2Geom<Pg, Pg>;
+        fill_twiddles<G>(sb, tid, 256);
+        __syncthreads();
+        if (tid < Pg)
+            fft2_stage1<Geo, Pg, false>(tid, buf, sb, [&](int n) { return sa[n]; });
+        __syncthreads();
+        if (tid < Pg) fft2_stage2<Geo, false>(tid, buf, [&](int, int g, float2 v) { sa[g] = v; });
+        __syncthreads();
+        r1 = sa;
+    }
     const float inv = 1.0f / ((float)G * (float)G);
     float2 v[16];
 #pragma unroll
@@ -1812,6 +1831,7 @@ int launch_rows_dft(int L, const float* in, float2* out, long nrows, int kmin, i
     LITHO_RF2(16, 16, 256)
     LITHO_RF2(16, 32, 256)
     LITHO_RF2(32, 32, 256)
+    LITHO_RF2(24, 24, 192)
 #undef LITHO_RF2
     switch (L) {
         case 32: return launch_rows_dft_L<32>(in, out, nrows, kmin, count, alt, tw, st);
@@ -2045,6 +2065,7 @@ int launch_socs_accum(int G, const float2* M, const void* bank, const float2* ks
     LITHO_S2(16, 32, 32, 16, 256)
     LITHO_S2(32, 32, 16, 9, 256)
     LITHO_S2(32, 32, 16, 32, 256)
+    LITHO_S2(24, 24, 16, 12, 192)
 #undef LITHO_S2
     int GR = socs_rows_per_block(G, Nx);
     if (!GR || bank_fmt) return LITHO_EARG;
@@ -2107,6 +2128,11 @@ int launch_fi_cols(int G, int N, const float2* P, float2* U, int B, int band, in
         case 256: return launch_fi_cols_G<256>(N, P, U, B, band, yalt, tw, st);
         case 512: return launch_fi_cols_G<512>(N, P, U, B, band, yalt, tw, st);
         case 1024: return launch_fi_cols_G<1024>(N, P, U, B, band, yalt, tw, st);
+        case 576:
+            if (N != 4096) return LITHO_EARG;
+            hipLaunchKernelGGL((fi_cols4096_kernel<576>), dim3(band + 1, B), dim3(256), 0, st, P,
+                               U, band, yalt, tw);
+            return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
         default: return LITHO_EARG;
     }
 }

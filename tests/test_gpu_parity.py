@@ -240,6 +240,44 @@ def test_band_and_pow2_grids_agree(gpu, orc, base_source, N):
     assert max_rel(outs[0], outs[1]) < IMG_TOL
 
 
+def test_large_clip_576_and_pow2_grids_agree(gpu, base_source):
+    """4096^2 clips take the 576 = 24^2 conv grid on the automatic path and the
+    reference's nextPow2 grid (1024) on LITHO_PATH_POW2: the same images to
+    FP32 round-off (the intensity band 4Nx+1 = 549 fits both), through the
+    batch entry point and the pipelined one with a 16-bit bank."""
+    case("576 vs 1024 conv grid at 4096^2, K=16")
+    N, nk = 4096, 16
+    src, s = base_source
+    Nx = gpu.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    assert 4 * Nx + 1 == 549
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx,
+                                       nk)
+    masks = np.stack([gpu.manhattan_mask(N, 60 + i) for i in range(2)])
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm = ctx.alloc(masks.nbytes)
+        di = ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        outs = {}
+        for path in (gpu.LITHO_PATH_AUTO, gpu.LITHO_PATH_POW2):
+            ctx.set_path(path)
+            ctx.socs_image_batch(bank, dm, 2, N, di)
+            outs[("batch", path)] = ctx.d2h(np.zeros_like(masks), di)
+            ctx.socs_image_batch_pipelined(bank, dm, 2, N, di, gpu.LITHO_PREC_BANK_FP16, 1, True)
+            outs[("fp16", path)] = ctx.d2h(np.zeros_like(masks), di)
+        ctx.free(dm)
+        ctx.free(di)
+        bank.close()
+    finally:
+        ctx.close()
+    a, p = gpu.LITHO_PATH_AUTO, gpu.LITHO_PATH_POW2
+    print("576 vs 1024 grid: fp32", max_rel(outs[("batch", a)], outs[("batch", p)]),
+          "fp16 bank", max_rel(outs[("fp16", a)], outs[("fp16", p)]))
+    assert max_rel(outs[("batch", a)], outs[("batch", p)]) < IMG_TOL
+    assert max_rel(outs[("fp16", a)], outs[("fp16", p)]) < IMG_TOL
+
+
 @pytest.mark.parametrize("N", [256, 2048])
 def test_other_clip_sizes(gpu, orc, base_source, N):
     case(f"socs_image N={N}")
