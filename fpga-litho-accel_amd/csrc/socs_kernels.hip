@@ -133,6 +133,22 @@ struct Bank<3> {
     }
 };
 
+// Linear (column, image) index of this block in an order that keeps
+// neighbouring columns on one XCD: the dispatcher hands linear block ids to
+// the 8 XCDs in turn (block L runs on XCD L mod 8), so XCD x is given the x-th
+// contiguous run of the column-major sequence, and blocks that share an L2
+// read (or write) the same 128-byte lines of a [rows][columns] array instead
+// of each XCD fetching every line.  A bijection of [0, gridDim.x gridDim.y).
+__device__ __forceinline__ int xcd_block_order() {
+    const int T = gridDim.x * gridDim.y;
+    const int L = blockIdx.x + gridDim.x * blockIdx.y;
+    const int q = T >> 3, r = T & 7, x = L & 7, i = L >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Column transforms of 4096 points on the same register core, one column per
+// block.  mask_cols4096: R [B][4096][cx] -> M [B][cy][cx] as mask_cols_kernel
+// (ky = kymin + j, |ky| < 256, times (-1)^ky / N^2).
 // ---------------------------------------------------------------------------
 // rows_dft: in  [nrows][L] f32 (nrows even), out [nrows][count] float2
 // out[r][j] = sgn(k) * X_r[k mod L], k = kmin + j, X_r = DFT_L(row r) forward,
@@ -195,8 +211,9 @@ __global__ __launch_bounds__(256) void mask_cols_kernel(const float2* __restrict
     __shared__ float2 sa[CG * N];
     __shared__ float2 sb[CG * N];
     const int tid = threadIdx.x;
-    const int c0 = blockIdx.x * CG;
-    const long b = blockIdx.y;
+    const int lin = xcd_block_order();
+    const int c0 = (lin % gridDim.x) * CG;
+    const long b = lin / gridDim.x;
     const float2* Rb = R + b * (long)N * cx;
     for (int i = tid; i < CG * N; i += 256) {
         int y = i / CG, c = i - y * CG;
@@ -1448,8 +1465,9 @@ __global__ __launch_bounds__(256) void fi_cols_kernel(const float2* __restrict__
     __shared__ float2 sb[CG * N];
     const int tid = threadIdx.x;
     const int nb = band + 1;
-    const int c0 = blockIdx.x * CG;
-    const long b = blockIdx.y;
+    const int lin = xcd_block_order();
+    const int c0 = (lin % gridDim.x) * CG;
+    const long b = lin / gridDim.x;
     const float2* Pb = P + b * (long)G * nb;
     for (int i = tid; i < CG * G; i += 256) {
         int g = i / CG, c = i - g * CG;
@@ -1692,9 +1710,6 @@ __global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict
     }
 }
 
-// Column transforms of 4096 points on the same register core, one column per
-// block.  mask_cols4096: R [B][4096][cx] -> M [B][cy][cx] as mask_cols_kernel
-// (ky = kymin + j, |ky| < 256, times (-1)^ky / N^2).
 __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __restrict__ R,
                                                             float2* __restrict__ M, int cx,
                                                             int kymin, int cy) {
@@ -1702,8 +1717,9 @@ __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __rest
     __shared__ float2 buf[F4K_BUF];
     __shared__ float2 lo[64], hi[64];
     const int tid = threadIdx.x;
-    const int col = blockIdx.x;
-    const long b = blockIdx.y;
+    const int lin = xcd_block_order();
+    const int col = lin % cx;
+    const long b = lin / cx;
     const float2* Rc = R + b * (long)N * cx + col;
     float2 v[16];
 #pragma unroll
@@ -1740,8 +1756,9 @@ __global__ __launch_bounds__(256) void fi_cols4096_kernel(const float2* __restri
     __shared__ float2 lo[64], hi[64];
     const int tid = threadIdx.x;
     const int nb = band + 1;
-    const int col = blockIdx.x;
-    const long b = blockIdx.y;
+    const int lin = xcd_block_order();
+    const int col = lin % nb;
+    const long b = lin / nb;
     const float2* Pb = P + b * (long)G * nb + col;
     for (int g = tid; g < G; g += 256) sa[g] = Pb[(long)g * nb];
     f4k_tables(lo, hi, tid);
