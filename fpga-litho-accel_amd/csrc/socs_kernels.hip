@@ -378,8 +378,11 @@ __global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restri
     const int tid = threadIdx.x;
     const int slot = tid / T, t = tid - slot * T;
     const int nblk = G / GR;
-    const long b = blockIdx.x / nblk;
-    const int g0 = (blockIdx.x - (int)b * nblk) * GR;
+    // the row blocks of an image in turn on one XCD: with the rows read in
+    // place (PRE), neighbouring row blocks share the column pass's lines
+    const int lin = PRE ? xcd_block_order() : (int)blockIdx.x;
+    const long b = lin / nblk;
+    const int g0 = (lin - (int)b * nblk) * GR;
     const float2* Mb = M + b * (long)W * W;
     float2* xb = xbuf + slot * Geo::XBUF;
     fill_twiddles<G>(twG, tid, NTH);
@@ -2082,7 +2085,7 @@ int launch_socs_accum(int G, const float2* M, const void* bank, const float2* ks
     LITHO_S2(16, 32, 32, 16, 256)
     LITHO_S2(32, 32, 16, 9, 256)
     LITHO_S2(32, 32, 16, 32, 256)
-    LITHO_S2(24, 24, 16, 12, 192)
+    LITHO_S2(24, 24, 8, 12, 192)
 #undef LITHO_S2
     int GR = socs_rows_per_block(G, Nx);
     if (!GR || bank_fmt) return LITHO_EARG;
