@@ -206,12 +206,14 @@ def mixed_size_bench(ctx, src, sigma, steps):
 
 def large_clip_bench(ctx, src, sigma, steps, with_oracle):
     """Config 5: 4096^2 clips at 4 nm (period 16384 nm), Nx = 137, K = 64 from
-    the implicit TCC (source-space Gram route); the SOCS loop in FP32 on the
-    reference's nextPow2 grid (1024 points), kernels stored as given (f32) and
-    as bf16 with per-entry scales (config 5's storage), fp16, bf16 hi + lo.
-    Images/s over graph-replayed batches of 8 clips resident in HBM; error of
-    one clip against the FP64 oracle fed the same kernels."""
-    N, B, nk = 4096, 8, 64
+    the implicit TCC (source-space Gram route); the SOCS loop in FP32 on a
+    576 = 24^2 conv grid (the intensity band 4 Nx + 1 = 549 fits; the
+    reference's nextPow2 grid is 1024 and gives the same image), kernels stored
+    as given (f32) and as bf16 with per-entry scales (config 5's storage),
+    fp16, bf16 hi + lo.  Images/s over graph-replayed batches of 32 clips
+    resident in HBM; error of one clip against the FP64 oracle fed the same
+    kernels."""
+    N, B, nk = 4096, 32, 64
     Nx = L.compute_nx(NA, PIX_NM * N, sigma, LAMBDA)
     t0 = time.perf_counter()
     sc, K, flags = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, PIX_NM * N, PIX_NM * N,
@@ -223,7 +225,7 @@ def large_clip_bench(ctx, src, sigma, steps, with_oracle):
     ctx.h2d(dm, masks)
     forms = {"fp32": L.LITHO_PREC_FP32, "bf16_bank": L.LITHO_PREC_BANK_BF16,
              "fp16_bank": L.LITHO_PREC_BANK_FP16, "bf16x2_bank": L.LITHO_PREC_BANK_BF16X2}
-    out = {"N": N, "Nx": Nx, "K": nk, "batch": B, "conv_grid": 1024,
+    out = {"N": N, "Nx": Nx, "K": nk, "batch": B, "conv_grid": 576 if 4 * Nx + 1 <= 576 else 1024,
            "entries": ctx.bank_info(bank)[0], "setup": setup, "forms": {}}
     imgs = {}
     for name, prec in forms.items():

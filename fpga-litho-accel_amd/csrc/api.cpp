@@ -1033,8 +1033,18 @@ static int kernels_from_source_impl(const LithoOptics* op, const float* src, int
     plan_clusters(wdesc, nk, &mtop, &split, clusters, rtol);
     LITHO_TRY(Vt.alloc((size_t)mtop * n * sizeof(double2)));
     LITHO_TRY(wd.alloc(mtop * sizeof(double)));
-    LITHO_TRY(launch_lift(P.A.as<double2>(), P.S.as<double>(), U.as<double2>(), w.as<double>(),
-                          order.as<int>(), n, m, mtop, Vt.as<double2>(), wd.as<double>(), st));
+    {
+        // v = A diag(sqrt S) u / sqrt(lambda) for the top mtop vectors: one
+        // FP64 matrix-core product (mtop x m) x (m x n)
+        Dev Xs;
+        LITHO_TRY(Xs.alloc((size_t)mtop * m * sizeof(double2)));
+        LITHO_TRY(launch_lift_gather(P.S.as<double>(), U.as<double2>(), w.as<double>(),
+                                     order.as<int>(), m, mtop, Xs.as<double2>(), wd.as<double>(),
+                                     st));
+        LITHO_TRY(launch_zgemm_nt(Xs.as<double2>(), m, 0, P.A.as<double2>(), m, 0, mtop, n, m,
+                                  Vt.as<double2>(), n, 1, nullptr, st));
+        LITHO_HIP(hipStreamSynchronize(st));
+    }
     LITHO_TRY(finish_kernels(c, Vt.as<double2>(), n, mtop, nk, op->Nx, op->Ny, clusters, krn_r,
                              krn_i, nullptr));
     for (int k = 0; k < nk; k++) scales[k] = wdesc[k];

@@ -197,33 +197,24 @@ __global__ __launch_bounds__(256) void gather_top_kernel(const double2* __restri
     if (i == 0) wd[r] = w[c];
 }
 
-// Gram route lift: Vt[r][i] = (1/sqrt(lam_r)) sum_s A[i][s] sqrt(S_s) U[s][ord[r]]
-// U is the Gram eigenvector matrix (row-major [m][m], column = vector).
-__global__ __launch_bounds__(256) void lift_kernel(const double2* __restrict__ A,
-                                                   const double* __restrict__ S,
-                                                   const double2* __restrict__ U,
-                                                   const double* __restrict__ w,
-                                                   const int* __restrict__ order, int n, int m,
-                                                   int mtop, double2* __restrict__ Vt,
-                                                   double* __restrict__ wd) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int r = blockIdx.y;
-    if (i >= n || r >= mtop) return;
+// Gram route lift, first half: Xs[r][s] = sqrt(S_s) U[s][ord[r]] / sqrt(lam_r)
+// (U row-major [m][m], column = vector), wd[r] = lam_r; the lift itself,
+// Vt[r][i] = sum_s A[i][s] Xs[r][s], is one FP64 matrix-core product.
+__global__ __launch_bounds__(256) void lift_gather_kernel(const double* __restrict__ S,
+                                                          const double2* __restrict__ U,
+                                                          const double* __restrict__ w,
+                                                          const int* __restrict__ order, int m,
+                                                          int mtop, double2* __restrict__ Xs,
+                                                          double* __restrict__ wd) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)mtop * m) return;
+    const int r = (int)(e / m), sidx = (int)(e % m);
     const int c = order[r];
     const double lam = w[c];
-    double sr = 0.0, si = 0.0;
-    const double2* Ai = A + (long)i * m;
-    for (int s = 0; s < m; s++) {
-        double2 a = Ai[s];
-        double f = sqrt(S[s]);
-        double2 u = U[(long)s * m + c];
-        double br = a.x * f, bi = a.y * f;
-        sr += br * u.x - bi * u.y;
-        si += br * u.y + bi * u.x;
-    }
-    double inv = lam > 0.0 ? 1.0 / sqrt(lam) : 0.0;
-    Vt[(long)r * n + i] = make_double2(sr * inv, si * inv);
-    if (i == 0) wd[r] = lam;
+    const double f = lam > 0.0 ? sqrt(S[sidx]) / sqrt(lam) : 0.0;
+    const double2 u = U[(long)sidx * m + c];
+    Xs[e] = make_double2(u.x * f, u.y * f);
+    if (sidx == 0) wd[r] = lam;
 }
 
 __device__ __forceinline__ int mirror_index(int id, int Nx, int Ny, int which) {
@@ -526,11 +517,10 @@ int launch_gather_top(const double2* V, const double* w, const int* order, int n
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
-int launch_lift(const double2* A, const double* S, const double2* U, const double* w,
-                const int* order, int n, int m, int mtop, double2* Vt, double* wd,
-                hipStream_t st) {
-    hipLaunchKernelGGL(lift_kernel, dim3(cdiv(n, 256), mtop), dim3(256), 0, st, A, S, U, w, order,
-                       n, m, mtop, Vt, wd);
+int launch_lift_gather(const double* S, const double2* U, const double* w, const int* order,
+                       int m, int mtop, double2* Xs, double* wd, hipStream_t st) {
+    hipLaunchKernelGGL(lift_gather_kernel, dim3(cdiv((long)mtop * m, 256)), dim3(256), 0, st, S, U,
+                       w, order, m, mtop, Xs, wd);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 

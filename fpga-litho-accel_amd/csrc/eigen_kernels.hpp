@@ -26,10 +26,10 @@ int launch_diag_rank(const double2* H, int n, double* w, int* order, hipStream_t
 // Vt[r][:] = eigenvector order[r] (vector-major), wd[r] = w[order[r]]
 int launch_gather_top(const double2* V, const double* w, const int* order, int n, int m,
                       double2* Vt, double* wd, hipStream_t st);
-// Gram route: Vt[r][:] = A diag(sqrt S) u_{order[r]} / sqrt(lambda)
-int launch_lift(const double2* A, const double* S, const double2* U, const double* w,
-                const int* order, int n, int m, int mtop, double2* Vt, double* wd,
-                hipStream_t st);
+// Gram route, the lift's right factor: Xs[r][s] = sqrt(S_s) u_{order[r]}[s] /
+// sqrt(lambda), wd[r] = lambda; Vt = Xs A^T follows as a matrix-core product
+int launch_lift_gather(const double* S, const double2* U, const double* w, const int* order,
+                       int m, int mtop, double2* Xs, double* wd, hipStream_t st);
 // canonical mirror basis inside each cluster {start, size}
 int launch_canon(double2* Vt, int n, int Nx, int Ny, const int2* clusters, int ncl,
                  hipStream_t st);
