@@ -149,6 +149,20 @@ __device__ __forceinline__ int xcd_block_order() {
 // Column transforms of 4096 points on the same register core, one column per
 // block.  mask_cols4096: R [B][4096][cx] -> M [B][cy][cx] as mask_cols_kernel
 // (ky = kymin + j, |ky| < 256, times (-1)^ky / N^2).
+// Linear block index with each pair of neighbouring blocks on one XCD and
+// otherwise the dispatch order: in every run of 16 linear ids, XCD x (ids x
+// and x + 8) is given blocks 2x and 2x + 1, which run at about the same time
+// and share the 128-byte lines they each read half of.  A partial last run
+// keeps the identity.
+__device__ __forceinline__ int xcd_pair_order() {
+    const int T = gridDim.x * gridDim.y * gridDim.z;
+    const int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int tile = L >> 4;
+    if ((tile + 1) * 16 > T) return L;
+    const int r = L & 15;
+    return (tile << 4) + 2 * (r & 7) + (r >> 3);
+}
+
 // ---------------------------------------------------------------------------
 // rows_dft: in  [nrows][L] f32 (nrows even), out [nrows][count] float2
 // out[r][j] = sgn(k) * X_r[k mod L], k = kmin + j, X_r = DFT_L(row r) forward,
@@ -1948,9 +1962,12 @@ __global__ __launch_bounds__(NTH) void socs_cols_kernel(const float2* __restrict
     const int W = 2 * Nx + 1;
     const int tid = threadIdx.x;
     const int slot = tid / T, t = tid - slot * T;
-    const int col = blockIdx.x * SLOTS + slot;
-    const int k = blockIdx.y;
-    const long b = blockIdx.z;
+    // neighbouring column groups (each half of a bank line) on one XCD
+    const int lin = xcd_pair_order();
+    const int cgrp = lin % gridDim.x, kb = lin / gridDim.x;
+    const int col = cgrp * SLOTS + slot;
+    const int k = kb % gridDim.y;
+    const long b = kb / gridDim.y;
     const bool act = col < W;
     const float2* Mb = M + b * (long)W * W;
     const long kbase = (long)k * W * W;
