@@ -356,6 +356,8 @@ def main():
                     help="skip the 4096^2 (config 5) measurement")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-size (batching form) measurement")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the 4096^2 (config 5) measurement and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -371,6 +373,13 @@ def main():
         print(f"bench.py measures one GPU: the library has no multi-GPU path "
               f"(--gpus {args.gpus} requested). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.large_only:
+        src = L.createSource(41, "Annular", 0.6, 0.9)
+        ctx = L.Context()
+        out = large_clip_bench(ctx, src, L.getOuterSigma(src), 4, not args.no_cpu)
+        print(json.dumps(out), flush=True)
+        ctx.close()
+        return
     B = args.batch
     global N, NK
     if args.setup_only and args.setup_n != N:
