@@ -54,6 +54,7 @@ enum { NSTAGE = 6 };
 
 struct litho_ctx {
     int device = 0;
+    int ncu = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     float2* tw = nullptr;
     DevBuf ws[8];
@@ -119,6 +120,15 @@ struct litho_bank {
 // accelerator-shaped entry)
 static int batch_slot(int i) { return i < 5 ? i : 7; }
 
+// Blocks per image of the wave-synced SOCS at the 144 grid (one 130 KB block
+// per CU): a launch of at most half as many images as the device has CUs
+// splits each image's kernel entries over two blocks (1024^2, K = 24: one
+// clip 0.109 -> 0.075 ms, 128 clips 0.313 -> 0.285 ms; 256 clips are 2 %
+// faster unsplit)
+static int socs_ksplit(const litho_ctx* c, int G, int B) {
+    return G == 144 && 2 * B <= c->ncu ? 2 : 1;
+}
+
 static int ctx_ws(litho_ctx* c, int slot, size_t bytes, void** out) {
     DevBuf& b = c->ws[slot];
     if (b.bytes < bytes) {
@@ -167,6 +177,7 @@ int litho_ctx_create(litho_ctx** out, int device) {
     std::unique_ptr<litho_ctx> c(new litho_ctx());
     c->device = device;
     LITHO_HIP(hipSetDevice(device));
+    LITHO_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     LITHO_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     std::vector<float2> tw(TW_MAX);
     for (int j = 0; j < TW_MAX; j++) {
@@ -1299,7 +1310,7 @@ int image_batch_sizes(int path, const litho_bank* bank, int B, int N, size_t sz[
         *grid = Gb;
         sz[0] = (size_t)B * (Nx + 1) * N * sizeof(float2);
         sz[1] = (size_t)B * W * W * sizeof(float2);
-        sz[2] = (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2);
+        sz[2] = 2 * (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2);  // two planes
         sz[3] = 0;
         sz[4] = (size_t)B * (band + 1) * N * sizeof(float2);
         sz[5] = 0;
@@ -1346,10 +1357,12 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
         if (timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
         LITHO_TRY(launch_mask_cols_t(N, R, (float2*)M, B, Nx + 1, -Nx, W, false, false, st, W));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
+        const int ksplit = socs_ksplit(c, Gb, B);
+        const long cplane = (long)B * (2 * band + 1) * (band + 1);
         LITHO_TRY(launch_socs_band(Gb, (float2*)M, kb, ks.d_kscale, bank_fmt, ks.d_scales, ks.n,
-                                   Nx, B, (float2*)C, 2, st));
+                                   Nx, B, (float2*)C, 2, st, ksplit, cplane));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
-        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st));
+        LITHO_TRY(launch_fi_cols_t(N, (float2*)C, U, B, band, false, false, st, ksplit, cplane));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
         LITHO_TRY(launch_image_rows_t(N, U, d_image, (long)B * N, band, false, false, false, st));
         if (timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
@@ -1444,8 +1457,10 @@ static int ensure_pipeline(litho_ctx* c, int nstreams) {
 // shape of the FFT row / column kernels (lean: 128 threads).
 // ---------------------------------------------------------------------------
 struct BandPlan {
+    bool tuned = false;  // set by litho_autotune or litho_plan_set
     int G = 0;
-    int socs_shape = 2;  // 0: 48-slot blocks, 1: 24-slot blocks, 2: wave-synced transforms
+    int socs_shape = 2;  // 0: 48-slot blocks, 1: 24-slot blocks, 2: wave-synced transforms,
+                         // 3: as 2 with each image's entries over two blocks
     bool lean_mem = false;
 };
 
@@ -1453,6 +1468,7 @@ static BandPlan plan_of(litho_ctx* c, int N, int Nx, int nk, int prec) {
     BandPlan p;
     auto it = c->plans.find(std::make_tuple(N, Nx, nk, prec));
     if (it != c->plans.end()) {
+        p.tuned = true;
         p.G = it->second[0];
         p.socs_shape = it->second[1];
         p.lean_mem = it->second[2] != 0;
@@ -1498,6 +1514,7 @@ static int grow(DevBuf& b, size_t bytes, void** out) {
 
 struct BandWs {
     void *R = nullptr, *U = nullptr, *M = nullptr, *C = nullptr;
+    long cplane = 0;  // float2 between the two planes of C (the entry-split SOCS)
 };
 
 // four workspaces (row window R: kx = 0..Nx of each row, mask window M,
@@ -1509,7 +1526,11 @@ static int band_ws(DevBuf* slots, int B, int N, int Nx, int prec, BandWs* w) {
     const size_t es = mixed ? 4 : sizeof(float2), ues = (mixed && !mfma) ? 4 : sizeof(float2);
     LITHO_TRY(grow(slots[0], (size_t)B * (Nx + 1) * N * es, &w->R));
     LITHO_TRY(grow(slots[1], (size_t)B * W * W * sizeof(float2), &w->M));
-    LITHO_TRY(grow(slots[2], (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &w->C));
+    // C in two planes: the SOCS of plan shape 3 splits each image's entries
+    // over two blocks, one plane each
+    const size_t cper = (size_t)B * (2 * band + 1) * (band + 1);
+    LITHO_TRY(grow(slots[2], 2 * cper * sizeof(float2), &w->C));
+    w->cplane = (long)cper;
     LITHO_TRY(grow(slots[3], (size_t)B * (band + 1) * N * ues, &w->U));
     return LITHO_OK;
 }
@@ -1566,9 +1587,16 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
     if (stages & 2)
         LITHO_TRY(launch_mask_cols_t(N, w.R, (float2*)w.M, B, Nx + 1, -Nx, W, mixed, pl.lean_mem,
                                      st, W));
+    // plan shape 3: the wave-synced SOCS with each image's entries over two
+    // blocks, their band spectra added by the FI column pass
+    const int ksplit = pl.socs_shape == 3                 ? 2
+                       : pl.socs_shape == 2 && !pl.tuned ? socs_ksplit(c, pl.G, B)
+                                                          : 1;
+    if (ksplit > 1 && w.cplane <= 0) return LITHO_EARG;
     if (stages & 4)
         LITHO_TRY(launch_socs_band(pl.G, (float2*)w.M, kb, ks, bfmt, bank->band.d_scales,
-                                   bank->band.n, Nx, B, (float2*)w.C, pl.socs_shape, st));
+                                   bank->band.n, Nx, B, (float2*)w.C,
+                                   ksplit > 1 ? 2 : pl.socs_shape, st, ksplit, w.cplane));
     if (mfma) {
         // U for the image-side product: at 1024 points the column pass writes
         // it already split into fp16 hi + lo parts (its own form, fewer bytes);
@@ -1576,14 +1604,17 @@ static int enqueue_band(litho_ctx* c, const litho_bank* bank, const void* d_mask
         const bool usplit = N == 1024 && 2 * band + 1 <= 160;
         if ((stages & 8) && usplit)
             LITHO_TRY(launch_fi_cols_split(N, (float2*)w.C, w.U, B, band,
-                                           image_rows_split_cols(band + 1), st));
+                                           image_rows_split_cols(band + 1), st, ksplit,
+                                           w.cplane));
         else if (stages & 8)
-            LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st));
+            LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, false, pl.lean_mem, st,
+                                       ksplit, w.cplane));
         if (stages & 16)
             LITHO_TRY(launch_image_rows_mfma(N, w.U, band + 1, ti, d_image, B, out16, usplit, st));
     } else {
         if (stages & 8)
-            LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st));
+            LITHO_TRY(launch_fi_cols_t(N, (float2*)w.C, w.U, B, band, mixed, pl.lean_mem, st,
+                                       ksplit, w.cplane));
         if (stages & 16)
             LITHO_TRY(launch_image_rows_t(N, w.U, d_image, (long)B * N, band, mixed, pl.lean_mem,
                                           out16, st));
@@ -1676,13 +1707,15 @@ int litho_socs_image_batch_pipelined(litho_ctx* c, const litho_bank* bank, const
         void *R, *M, *C, *U;
         LITHO_TRY(ctx_ws(c, 0, ns * rstr, &R));
         LITHO_TRY(ctx_ws(c, 1, (size_t)B * W * W * sizeof(float2), &M));
-        LITHO_TRY(ctx_ws(c, 2, (size_t)B * (2 * band + 1) * (band + 1) * sizeof(float2), &C));
+        const size_t cper = (size_t)B * (2 * band + 1) * (band + 1);
+        LITHO_TRY(ctx_ws(c, 2, 2 * cper * sizeof(float2), &C));  // two planes (shape 3)
         LITHO_TRY(ctx_ws(c, 4, ns * ustr, &U));
         for (int k = 0; k < ns; k++) {
             w[k].R = (char*)R + k * rstr;
             w[k].U = (char*)U + k * ustr;
             w[k].M = M;
             w[k].C = C;
+            w[k].cplane = (long)cper;
         }
     }
     LITHO_TRY(ensure_pipeline(c, ns));
@@ -1868,11 +1901,15 @@ int litho_autotune(litho_ctx* c, const LithoBucket* bk, int nb, int precision, i
         std::array<int, 3> pick = {band_grid(Nx), 0, 0};
         for (int G : {36, 81, 144}) {
             if (G < 4 * Nx + 1 || G > b.N) continue;
+            // shape 3 (entries over two blocks) is not a candidate: it wins on
+            // the 32-clip sample but loses in full mixed-size sets, whose
+            // buckets already fill the chip together
             for (int ls = 0; ls < 3; ls++) {
                 if (ls == 1 && G != 144) continue;
                 for (int lm = 0; lm < 2; lm++) {
                     if (lm && b.N != 1024) continue;
                     BandPlan pl;
+                    pl.tuned = true;  // each candidate as given (no automatic split)
                     pl.G = G;
                     pl.socs_shape = ls;
                     pl.lean_mem = lm;
@@ -1904,7 +1941,7 @@ int litho_autotune(litho_ctx* c, const LithoBucket* bk, int nb, int precision, i
 int litho_plan_set(litho_ctx* c, int N, int Nx, int nk, int precision, int G, int socs_shape,
                    int lean_mem) {
     if (!c || !band_shape_ok(N, Nx) || nk <= 0 || !prec_ok(precision) || G < 4 * Nx + 1 ||
-        G > N || (G != 36 && G != 81 && G != 144) || socs_shape < 0 || socs_shape > 2 ||
+        G > N || (G != 36 && G != 81 && G != 144) || socs_shape < 0 || socs_shape > 3 ||
         (socs_shape == 1 && G != 144) || lean_mem < 0 || lean_mem > 1 || (lean_mem && N != 1024))
         return LITHO_EARG;
     c->plans[std::make_tuple(N, Nx, nk, precision)] = {G, socs_shape, lean_mem};

@@ -1082,7 +1082,8 @@ __global__ __launch_bounds__(NTH) void mask_cols16_kernel(const void* __restrict
 
 template <int Q, int NZ, bool H16, int NTH>
 __global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict__ C,
-                                                        void* __restrict__ U_, int band) {
+                                                        void* __restrict__ U_, int band, int nsum,
+                                                        long cplane) {
     constexpr int N = 16 * Q, S1 = Q / 16, FPB = NTH / 16, LDQ = Q + 1;
     __shared__ float2 twG[N];
     __shared__ float xplane[FPB * 16 * LDQ];
@@ -1103,6 +1104,7 @@ __global__ __launch_bounds__(NTH) void fi_cols16_kernel(const float2* __restrict
             float2 z = make_float2(0.f, 0.f);
             if (n1 < NZ && n <= 2 * band) {
                 z = Cb[n];
+                if (nsum > 1) z = cadd(z, Cb[n + cplane]);
                 if ((n - band) & 1) z = cscale(z, -1.f);
             }
             v[s][n1] = z;
@@ -1298,7 +1300,8 @@ __global__ __launch_bounds__(NTH) void mask_cols2_kernel(const void* __restrict_
 // band placed at indices ky + band and the shift undone on the output.
 template <int P, int Q, int NZ, int NTH, bool H16 = false>
 __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict__ C,
-                                                       void* __restrict__ U_, int band) {
+                                                       void* __restrict__ U_, int band, int nsum,
+                                                       long cplane) {
     using Geo = Fft2Geom<P, Q>;
     constexpr int N = Geo::G;
     constexpr int T = Geo::T;
@@ -1320,6 +1323,7 @@ __global__ __launch_bounds__(NTH) void fi_cols2_kernel(const float2* __restrict_
         float2 v = make_float2(0.f, 0.f);
         if (n <= 2 * band) {
             v = Cb[n];
+            if (nsum > 1) v = cadd(v, Cb[n + cplane]);
             if ((n - band) & 1) v = cscale(v, -1.f);
         }
         return v;
@@ -1398,7 +1402,7 @@ __global__ __launch_bounds__(NTH) void mask_cols_band1024_kernel(const void* __r
 template <bool H16, int NTH = 256, bool SPLIT = false>
 __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __restrict__ C,
                                                                void* __restrict__ U_, int band,
-                                                               int ns) {
+                                                               int ns, int nsum, long cplane) {
     using Geo = Fft2Geom<32, 32>;
     constexpr int N = 1024, T = 32, FPB = NTH / T, NZ = 5;
     __shared__ float2 twG[N];
@@ -1419,6 +1423,7 @@ __global__ __launch_bounds__(NTH) void fi_cols_band1024_kernel(const float2* __r
         float2 z = make_float2(0.f, 0.f);
         if (n1 < NZ && n <= 2 * band) {
             z = Cb[n];
+            if (nsum > 1) z = cadd(z, Cb[n + cplane]);  // the second half of the entries
             if ((n - band) & 1) z = cscale(z, -1.f);
         }
         v[n1] = z;
@@ -2261,13 +2266,19 @@ __device__ __forceinline__ void stage1_regs(int t, float2 (&v)[P], const float2 
     }
 }
 
+// ksplit > 1: the bank entries of an image are split over ksplit blocks
+// (block b ksplit + part takes entries [part nk / ksplit, (part + 1) nk /
+// ksplit)), each writing the band spectrum of its share to plane part of Cout
+// (planes cplane float2 apart); the FI column pass adds the planes in order.
+// For launches of fewer images than CUs: twice the blocks, each half the work.
 template <int P, int NZ, int NW, int BK = 0>
 __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __restrict__ M,
                                                             const void* __restrict__ bank,
                                                             const float2* __restrict__ kscale,
                                                             const float* __restrict__ scales,
                                                             int nk, int Nx,
-                                                            float2* __restrict__ Cout) {
+                                                            float2* __restrict__ Cout, int ksplit,
+                                                            long cplane) {
     using Geo = Fft2Geom<P, P>;
     static_assert(Geo::LDQ == P + 1 && Geo::S1 == 1 && Geo::S2 == 1, "square two-stage FFT");
     constexpr int G = Geo::G;
@@ -2288,7 +2299,9 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
     const int j = lane / P, t = lane - j * P;
     const bool on = j < TPW;
     const int slot = wv * TPW + (on ? j : 0);
-    const long b = blockIdx.x;
+    const long b = blockIdx.x / ksplit;
+    const int part = (int)(blockIdx.x - b * ksplit);
+    const int k0 = part * nk / ksplit, k1 = (part + 1) * nk / ksplit;
     const float2* Mb = M + b * (long)W * W;
     float2* xb = xbuf + slot * Geo::XBUF;
     fill_twiddles<G>(twG, tid, 64 * NW);
@@ -2318,13 +2331,13 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
                 if (on && col < W && n < W) braw[cr][n1] = Bank<BK>::raw(bank, kbase + n * W + col);
             }
     };
-    if (nk > 0) load_bank(0);
+    if (k1 > k0) load_bank(k0);
     __syncthreads();
     float2 twr[P];
 #pragma unroll
     for (int k1 = 0; k1 < P; k1++) twr[k1] = twG[t * k1];  // t, k1 < P: index < G
 
-    for (int k = 0; k < nk; k++) {
+    for (int k = k0; k < k1; k++) {
         const float lam = scales[k];
         const float2 ksc = BK ? kscale[k] : make_float2(1.f, 1.f);
         float2 xin[CR][NZ];
@@ -2333,7 +2346,7 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
 #pragma unroll
             for (int n1 = 0; n1 < NZ; n1++)
                 xin[cr][n1] = cmul(Bank<BK>::dec(braw[cr][n1], ksc), mreg[cr][n1]);
-        if (k + 1 < nk) load_bank(k + 1);
+        if (k + 1 < k1) load_bank(k + 1);
 #pragma unroll
         for (int cr = 0; cr < CR; cr++) {
             if (cr * SL >= W) break;
@@ -2395,7 +2408,7 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
     __syncthreads();
     // columns kx = 0..band: forward FFT over the rows, keep |ky| <= band
     const float inv = 1.0f / ((float)G * (float)G);
-    float2* Cb = Cout + b * (long)(2 * band + 1) * W;
+    float2* Cb = Cout + part * cplane + b * (long)(2 * band + 1) * W;
     for (int c0 = 0; c0 < W; c0 += SL) {
         const int col = c0 + slot;
         const bool act = on && col < W;
@@ -2418,7 +2431,7 @@ __global__ __launch_bounds__(64 * NW) void socs_wave_kernel(const float2* __rest
 
 template <int P, int NZ, int NW, int BK>
 static int launch_wave(const float2* M, const void* bank, const float2* kscale, const float* scales,
-                       int nk, int Nx, int B, float2* C, hipStream_t st) {
+                       int nk, int Nx, int B, float2* C, hipStream_t st, int ksplit, long cplane) {
     using Geo = Fft2Geom<P, P>;
     constexpr int SL = (64 / P) * NW;
     const int W = 2 * Nx + 1;
@@ -2426,7 +2439,8 @@ static int launch_wave(const float2* M, const void* bank, const float2* kscale, 
     if (lds > 160 * 1024) return LITHO_EARG;
     auto kern = socs_wave_kernel<P, NZ, NW, BK>;
     LITHO_TRY(allow_full_lds((const void*)kern));
-    hipLaunchKernelGGL(kern, dim3(B), dim3(64 * NW), lds, st, M, bank, kscale, scales, nk, Nx, C);
+    hipLaunchKernelGGL(kern, dim3((unsigned)((long)B * ksplit)), dim3(64 * NW), lds, st, M, bank,
+                       kscale, scales, nk, Nx, C, ksplit, cplane);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
@@ -2445,10 +2459,11 @@ static int launch_band(const float2* M, const void* bank, const float2* kscale, 
 
 int launch_socs_band(int G, const float2* M, const void* bank, const float2* kscale, int bank_fmt,
                      const float* scales, int nk, int Nx, int B, float2* C, int shape,
-                     hipStream_t st) {
+                     hipStream_t st, int ksplit, long cplane) {
     const int W = 2 * Nx + 1;
     if (B <= 0 || nk < 0 || 4 * Nx + 1 > G || bank_fmt < 0 || bank_fmt > 3 || shape < 0 ||
-        shape > 2)
+        shape > 2 || ksplit < 1 || ksplit > 2 || (ksplit > 1 && (shape != 2 || cplane <= 0)) ||
+        (long)B * ksplit > 0x7fffffffL)
         return LITHO_EARG;
 #define LITHO_BAND(p, nz, slots)                                                                  \
     if (G == p * p && W <= nz * p) {                                                              \
@@ -2462,12 +2477,16 @@ int launch_socs_band(int G, const float2* M, const void* bank, const float2* ksc
 #define LITHO_WAVE(p, nz, nw)                                                                     \
     if (G == p * p && W <= nz * p) {                                                              \
         if (bank_fmt == 1)                                                                        \
-            return launch_wave<p, nz, nw, 1>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
+            return launch_wave<p, nz, nw, 1>(M, bank, kscale, scales, nk, Nx, B, C, st, ksplit,   \
+                                             cplane);                                             \
         if (bank_fmt == 2)                                                                        \
-            return launch_wave<p, nz, nw, 2>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
+            return launch_wave<p, nz, nw, 2>(M, bank, kscale, scales, nk, Nx, B, C, st, ksplit,   \
+                                             cplane);                                             \
         if (bank_fmt == 3)                                                                        \
-            return launch_wave<p, nz, nw, 3>(M, bank, kscale, scales, nk, Nx, B, C, st);          \
-        return launch_wave<p, nz, nw, 0>(M, bank, kscale, scales, nk, Nx, B, C, st);             \
+            return launch_wave<p, nz, nw, 3>(M, bank, kscale, scales, nk, Nx, B, C, st, ksplit,   \
+                                             cplane);                                             \
+        return launch_wave<p, nz, nw, 0>(M, bank, kscale, scales, nk, Nx, B, C, st, ksplit,      \
+                                         cplane);                                                 \
     }
     if (shape == 2) {
         LITHO_WAVE(6, 6, 4)
@@ -2634,27 +2653,28 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
 }
 
 int launch_fi_cols_split(int N, const float2* C, void* U, int B, int band, int ns,
-                         hipStream_t st) {
-    if (B <= 0 || B > 65535 || N != 1024 || 2 * band + 1 > 160 || ns < 1 || ns > band + 1)
+                         hipStream_t st, int nsum, long cplane) {
+    if (B <= 0 || B > 65535 || N != 1024 || 2 * band + 1 > 160 || ns < 1 || ns > band + 1 ||
+        nsum < 1 || nsum > 2)
         return LITHO_EARG;
     const dim3 grid(cdiv(band + 1, 8), B);
     hipLaunchKernelGGL((fi_cols_band1024_kernel<false, 256, true>), grid, dim3(256), 0, st, C, U,
-                       band, ns);
+                       band, ns, nsum, cplane);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
-                     hipStream_t st) {
-    if (B <= 0 || B > 65535 || 2 * band + 1 > N) return LITHO_EARG;
+                     hipStream_t st, int nsum, long cplane) {
+    if (B <= 0 || B > 65535 || 2 * band + 1 > N || nsum < 1 || nsum > 2) return LITHO_EARG;
 #define LITHO_FC16(q)                                                                          \
     if (N == 16 * q && 2 * band + 1 <= 3 * q) {                                                \
         const dim3 grid(cdiv(band + 1, 8), B);                                                 \
         if (h16)                                                                               \
             hipLaunchKernelGGL((fi_cols16_kernel<q, 3, true, 128>), grid, dim3(128), 0, st, C, \
-                               U, band);                                                       \
+                               U, band, nsum, cplane);                                         \
         else                                                                                   \
             hipLaunchKernelGGL((fi_cols16_kernel<q, 3, false, 128>), grid, dim3(128), 0, st, C, \
-                               U, band);                                                       \
+                               U, band, nsum, cplane);                                         \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                        \
     }
     LITHO_FC16(16)  // every plan: one kernel, the same bits
@@ -2664,9 +2684,11 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
     if (N == 1024 && 2 * band + 1 <= 160) {  // every plan: one kernel, the same bits
         const dim3 grid(cdiv(band + 1, 8), B);
         if (h16)
-            hipLaunchKernelGGL((fi_cols_band1024_kernel<true>), grid, dim3(256), 0, st, C, U, band, 0);
+            hipLaunchKernelGGL((fi_cols_band1024_kernel<true>), grid, dim3(256), 0, st, C, U, band, 0,
+                               nsum, cplane);
         else
-            hipLaunchKernelGGL((fi_cols_band1024_kernel<false>), grid, dim3(256), 0, st, C, U, band, 0);
+            hipLaunchKernelGGL((fi_cols_band1024_kernel<false>), grid, dim3(256), 0, st, C, U, band,
+                               0, nsum, cplane);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
 #define LITHO_FC(p, q, nz, nth)                                                                 \
@@ -2674,10 +2696,12 @@ int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16,
         constexpr int FPB = nth / Fft2Geom<p, q>::T;                                            \
         if (h16)                                                                                \
             hipLaunchKernelGGL((fi_cols2_kernel<p, q, nz, nth, true>),                          \
-                               dim3(cdiv(band + 1, FPB), B), dim3(nth), 0, st, C, U, band);     \
+                               dim3(cdiv(band + 1, FPB), B), dim3(nth), 0, st, C, U, band, nsum, \
+                               cplane);                                                         \
         else                                                                                    \
             hipLaunchKernelGGL((fi_cols2_kernel<p, q, nz, nth, false>),                         \
-                               dim3(cdiv(band + 1, FPB), B), dim3(nth), 0, st, C, U, band);     \
+                               dim3(cdiv(band + 1, FPB), B), dim3(nth), 0, st, C, U, band, nsum, \
+                               cplane);                                                         \
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;                         \
     }
     LITHO_FC(16, 16, 3, 256)

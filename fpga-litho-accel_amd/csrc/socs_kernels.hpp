@@ -51,18 +51,20 @@ int launch_mask_cols_t(int N, const void* R, float2* M, int B, int cx, int kymin
 // bank_fmt 0: float2; 1: bf16 (re, im) pairs; 2: fp16 pairs; kscale[k] scales the real (x)
 // and imaginary (y) parts of entry k.
 // shape 0: 48-slot blocks, 1: 24-slot (lean) blocks at G = 144, 2: wave-synced
-// transforms (socs_wave_kernel)
+// transforms (socs_wave_kernel); ksplit = 2 (shape 2 only): each image's
+// entries over two blocks, the second half's band spectrum in a second plane
+// of C, cplane float2 after the first
 int launch_socs_band(int G, const float2* M, const void* bank, const float2* kscale, int bank_fmt,
                      const float* scales, int nk, int Nx, int B, float2* C, int shape,
-                     hipStream_t st);
-// C -> U [B][2Nx+1][N]
+                     hipStream_t st, int ksplit = 1, long cplane = 0);
+// C -> U [B][2Nx+1][N]; nsum = 2 adds the plane cplane float2 after C
 int launch_fi_cols_t(int N, const float2* C, void* U, int B, int band, bool h16, bool lean,
-                     hipStream_t st);
+                     hipStream_t st, int nsum = 1, long cplane = 0);
 // C -> U in the compact form of launch_image_rows_mfma (usplit): FP32 for
 // the ns split columns, fp16 above; N = 1024 with 2 band + 1 <= 160 only
 // (else LITHO_EARG)
 int launch_fi_cols_split(int N, const float2* C, void* U, int B, int band, int ns,
-                         hipStream_t st);
+                         hipStream_t st, int nsum = 1, long cplane = 0);
 // U [B][band+1][N] -> image [B*N][N]
 // img: f32, or fp16 when out16
 int launch_image_rows_t(int N, const void* U, void* img, long nrows, int band, bool h16,

@@ -510,7 +510,7 @@ class Context:
                "litho_plan_set")
 
     def plan(self, N, Nx, nk, precision=0):
-        """(conv grid, SOCS block shape 0/1/2, lean FFT block, tuned?) for a shape."""
+        """(conv grid, SOCS block shape 0/1/2/3, lean FFT block, tuned?) for a shape."""
         g, ls, lm = C.c_int(), C.c_int(), C.c_int()
         t = lib().litho_plan_get(self.h, N, Nx, nk, precision, C.byref(g), C.byref(ls),
                                  C.byref(lm))

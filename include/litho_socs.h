@@ -318,7 +318,10 @@ int litho_bucket_times(litho_ctx* ctx, float* ms, int nbuckets);
  * shapes tuned. */
 int litho_autotune(litho_ctx* ctx, const LithoBucket* buckets, int nbuckets, int precision,
                    int reps);
-/* Pin the plan of a shape without tuning (same candidate set; LITHO_EARG for
+/* Pin the plan of a shape without tuning (the candidate set, plus SOCS
+ * block shape 3: shape 2 with each clip's kernel entries split over two
+ * blocks and their band spectra added by the FI column pass, which halves the
+ * SOCS latency of batches much smaller than the GPU's 256 CUs; LITHO_EARG for
  * a combination that is not built). */
 int litho_plan_set(litho_ctx* ctx, int N, int Nx, int nk, int precision, int G, int socs_shape,
                    int lean_mem);

@@ -89,14 +89,16 @@ def test_autotuned_plans_keep_parity(gpu, orc, setup):
 
 
 def test_every_plan_matches_oracle(gpu, orc, setup):
-    """Each conv grid and SOCS block shape (incl. the wave-synced transforms)
-    gives the FP64 image to 1e-5 on every clip size."""
+    """Each conv grid and SOCS block shape (incl. the wave-synced transforms,
+    and shape 3: each clip's kernel entries over two blocks, the two band
+    spectra added by the FI column pass) gives the FP64 image to 1e-5 on every
+    clip size."""
     ctx, buckets, host = setup
     print("[case] every band plan", flush=True)
     for (b, N, c, dm, di, _), (_, Nx, sc, K, masks) in zip(buckets, host):
         ref = oracle_image(orc, N, Nx, sc, K, masks[0])
         for G in (36, 81, 144):
-            for shape in (0, 1, 2):
+            for shape in (0, 1, 2, 3):
                 for lean in (0, 1):
                     try:
                         ctx.set_plan(N, Nx, 16, 0, G, shape, lean)
