@@ -218,8 +218,8 @@ int litho_socs_image_batch(litho_ctx* ctx, const litho_bank* bank, const float* 
  * device array of LITHO_NOALLOC_SCRATCH_BYTES (64 MiB) that the library holds
  * once per GPU until the process ends: same launches, same images bit for
  * bit, nothing allocated or freed.  LITHO_ESCRATCH when the batch's
- * workspaces do not fit (about 0.92 MiB per clip at 1024^2, Nx = 34, so
- * B <= 69; 60 KiB per clip at 256^2, Nx = 8).  All calls on one GPU share
+ * workspaces do not fit (about 0.99 MiB per clip at 1024^2, Nx = 34, so
+ * B <= 64; 63 KiB per clip at 256^2, Nx = 8).  All calls on one GPU share
  * the array: calls that overlap in time on one GPU (two contexts or streams)
  * race and must be serialised by the caller.  Records no per-stage timing. */
 #define LITHO_NOALLOC_SCRATCH_BYTES (64u << 20)
