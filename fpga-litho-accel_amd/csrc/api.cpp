@@ -834,7 +834,7 @@ static int contract_herm(litho_ctx* c, const double2* X, const double* sx, const
                          int m, int imag, int prec, double2* C) {
     hipStream_t st = c->stream;
     if (prec == LITHO_TCC_FP64) {
-        const int ns = herm_gemm_slices(n, m);
+        const int ns = herm_gemm_slices(n, m, imag);
         Dev part;
         if (ns > 1) LITHO_TRY(part.alloc((size_t)ns * n * n * sizeof(double2)));
         LITHO_TRY(launch_herm_gemm(X, sx, Y, n, m, imag, C, part.as<double2>(), ns, st));

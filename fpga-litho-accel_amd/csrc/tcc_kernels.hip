@@ -191,6 +191,117 @@ __global__ __launch_bounds__(256) void herm_gemm_mfma_kernel(const double2* __re
     }
 }
 
+// ---------------------------------------------------------------------------
+// Real-pupil contraction (zero defocus: the pupil matrix is 0/1, so X and Y
+// are real and C = (X diag(sx)) Y^T is real symmetric): the real parts only,
+// 64 x 64 block tiles of four 32 x 32 wave tiles (2 x 2 MFMA f64 16x16x4
+// each, A and B fragments reused twice), K staged 16 at a time through two
+// LDS buffers with the next stage's global loads in flight during the MFMAs,
+// one barrier per stage.  Against the 32 x 32 complex tiles above: a quarter
+// of the L2 reads per output and four MFMAs per fragment load pair.
+// ---------------------------------------------------------------------------
+constexpr int TR_T = 64;  // block tile
+constexpr int TR_K = 16;  // K stage
+
+__global__ __launch_bounds__(256) void herm_gemm_real_kernel(const double2* __restrict__ X,
+                                                             const double* __restrict__ sx,
+                                                             const double2* __restrict__ Y, int n,
+                                                             int m, int kslice,
+                                                             double2* __restrict__ C,
+                                                             double2* __restrict__ P) {
+    const int nt = (n + TR_T - 1) / TR_T;
+    long bid = blockIdx.x;
+    int ti = 0;
+    {
+        long rowlen = nt;
+        while (bid >= rowlen) {
+            bid -= rowlen;
+            ti++;
+            rowlen--;
+        }
+    }
+    const int tj = ti + (int)bid;
+    __shared__ double Xs[2][TR_T][TR_K + 1];
+    __shared__ double Ys[2][TR_T][TR_K + 1];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int i0 = ti * TR_T, j0 = tj * TR_T;
+    const int sbeg = blockIdx.y * kslice, send = min(m, sbeg + kslice);
+    // staging: element e = tid + 256 q (q < 4) of a stage is row e / 16, term
+    // e % 16: sixteen lanes read 256 contiguous bytes of one row
+    const int lc = tid & 15, lr0 = tid >> 4;
+    double xr[4], yr[4];
+    auto gload = [&](int s0) {
+        const int sidx = s0 + lc;
+        const bool ok = sidx < send;
+        const int sc = ok ? sidx : sbeg;  // clamped: every load in flight at once
+        const double f = sx ? sx[sc] : 1.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = lr0 + 16 * q;
+            const bool xin = i0 + r < n, yin = j0 + r < n;
+            const double xv = X[(long)(xin ? i0 + r : 0) * m + sc].x * f;
+            const double yv = Y[(long)(yin ? j0 + r : 0) * m + sc].x;
+            xr[q] = ok && xin ? xv : 0.0;
+            yr[q] = ok && yin ? yv : 0.0;
+        }
+    };
+    auto sstore = [&](int bf) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            Xs[bf][lr0 + 16 * q][lc] = xr[q];
+            Ys[bf][lr0 + 16 * q][lc] = yr[q];
+        }
+    };
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    if (sbeg < send) {
+        gload(sbeg);
+        sstore(0);
+    }
+    __syncthreads();
+    int bf = 0;
+    for (int s0 = sbeg; s0 < send; s0 += TR_K) {
+        const bool more = s0 + TR_K < send;
+        if (more) gload(s0 + TR_K);
+#pragma unroll
+        for (int kk = 0; kk < TR_K; kk += 4) {
+            const int k = kk + (lane >> 4), r = lane & 15;
+            const double a0 = Xs[bf][wi * 32 + r][k], a1 = Xs[bf][wi * 32 + 16 + r][k];
+            const double b0 = Ys[bf][wj * 32 + r][k], b1 = Ys[bf][wj * 32 + 16 + r][k];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (more) sstore(bf ^ 1);
+        __syncthreads();
+        bf ^= 1;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + wi * 32 + 16 * a + (lane >> 4) + 4 * v;
+                const int j = j0 + wj * 32 + 16 * b + (lane & 15);
+                if (i >= n || j >= n) continue;
+                if (ti == tj && j < i) continue;  // diagonal tile: upper part only
+                const double re = acc[a][b][v];
+                if (P) {
+                    P[(long)blockIdx.y * n * n + (long)i * n + j] = make_double2(re, 0.0);
+                    continue;
+                }
+                C[(long)i * n + j] = make_double2(re, 0.0);
+                if (i != j) C[(long)j * n + i] = make_double2(re, 0.0);
+            }
+}
+
 int launch_pupil(const int2* srcs, int nsrc, const PupilGeom& g, double2* A, int* range_err,
                  hipStream_t st) {
     long n = (long)(2 * g.Nx + 1) * (2 * g.Ny + 1);
@@ -229,11 +340,14 @@ __global__ __launch_bounds__(256) void herm_sum_kernel(const double2* __restrict
     }
 }
 
-int herm_gemm_slices(int n, int m) {
-    const long nt = (n + TG_T - 1) / TG_T;
+int herm_gemm_slices(int n, int m, int has_imag) {
+    const long tile = has_imag ? TG_T : TR_T;
+    const long nt = (n + tile - 1) / tile;
     const long nblocks = nt * (nt + 1) / 2;
-    // about 4096 blocks in flight, slices of at least 256 terms
-    long ns = (4096 + nblocks - 1) / nblocks;
+    // about 4096 blocks in flight (2048 of the larger real tiles), slices of
+    // at least 256 terms
+    const long want = has_imag ? 4096 : 2048;
+    long ns = (want + nblocks - 1) / nblocks;
     ns = std::min(ns, (long)std::max(1, m / 256));
     ns = std::min(ns, 64L);
     return (int)std::max(1L, ns);
@@ -242,12 +356,17 @@ int herm_gemm_slices(int n, int m) {
 int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
                      int has_imag, double2* C, double2* P, int nslices, hipStream_t st) {
     if (n <= 0 || m <= 0 || nslices <= 0 || (nslices > 1 && !P)) return LITHO_EARG;
-    long nt = (n + TG_T - 1) / TG_T;
+    const long tile = has_imag ? TG_T : TR_T;
+    long nt = (n + tile - 1) / tile;
     long nblocks = nt * (nt + 1) / 2;
     const int kslice = (int)(((long)m + nslices - 1) / nslices + TG_K - 1) / TG_K * TG_K;
     const int ns = (m + kslice - 1) / kslice;
-    hipLaunchKernelGGL(herm_gemm_mfma_kernel, dim3((unsigned)nblocks, ns), dim3(256), 0, st, X, sx,
-                       Y, n, m, has_imag, kslice, C, ns > 1 ? P : nullptr);
+    if (has_imag)
+        hipLaunchKernelGGL(herm_gemm_mfma_kernel, dim3((unsigned)nblocks, ns), dim3(256), 0, st, X,
+                           sx, Y, n, m, has_imag, kslice, C, ns > 1 ? P : nullptr);
+    else
+        hipLaunchKernelGGL(herm_gemm_real_kernel, dim3((unsigned)nblocks, ns), dim3(256), 0, st, X,
+                           sx, Y, n, m, kslice, C, ns > 1 ? P : nullptr);
     if (ns > 1)
         hipLaunchKernelGGL(herm_sum_kernel, dim3(cdiv((long)n * n, 256)), dim3(256), 0, st, P, ns,
                            n, C);

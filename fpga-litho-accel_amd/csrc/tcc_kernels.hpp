@@ -22,8 +22,9 @@ int launch_conj_transpose_scale(const double2* A, const double* S, int n, int m,
                                 hipStream_t st);
 // C = (X diag(sx)) Y^H (sx may be null), Hermitian output, n x n; the m terms
 // split into nslices K slices (herm_gemm_slices) whose partial sums go
-// through P (nslices * n * n complex doubles; unused for one slice)
-int herm_gemm_slices(int n, int m);
+// through P (nslices * n * n complex doubles; unused for one slice); without
+// imaginary parts (has_imag = 0) the real-pupil kernel reads X.x, Y.x only
+int herm_gemm_slices(int n, int m, int has_imag);
 int launch_herm_gemm(const double2* X, const double* sx, const double2* Y, int n, int m,
                      int has_imag, double2* C, double2* P, int nslices, hipStream_t st);
 // Same contraction in split BF16 on the matrix cores (tcc_bf16.hip): part is
