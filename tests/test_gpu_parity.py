@@ -303,6 +303,48 @@ def test_single_kernel_and_empty_mask(gpu, orc, clip512):
     assert not z.any()
 
 
+@pytest.mark.parametrize("nk,B", [(1, 1), (3, 3), (24, 5)])
+def test_entry_split_socs(gpu, orc, base_source, nk, B):
+    """Launches of few 1024^2 clips split each clip's kernel entries over two
+    blocks (the second half's band spectrum in a second plane, added by the FI
+    column pass): a single kernel (one block of the pair gets no entry), an
+    odd entry count and an odd batch, against the FP64 oracle and against the
+    pinned one-block plan (shape 2) of the pipelined entry."""
+    case(f"entry-split SOCS 1024^2 K={nk} B={B}")
+    N = 1024
+    src, s = base_source
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, 24)
+    K, sc = K[:nk], sc[:nk]
+    masks = np.stack([gpu.manhattan_mask(N, 70 + i) for i in range(B)])
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm = ctx.alloc(masks.nbytes)
+        di = ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        ctx.socs_image_batch(bank, dm, B, N, di)  # few clips: split by default
+        split = ctx.d2h(np.zeros_like(masks), di)
+        ctx.set_plan(N, Nx, nk, 0, 144, 2, False)  # one block per clip
+        ctx.socs_image_batch_pipelined(bank, dm, B, N, di, 0, 1, False)
+        one = ctx.d2h(np.zeros_like(masks), di)
+        ctx.set_plan(N, Nx, nk, 0, 144, 3, False)  # the split, pinned
+        ctx.socs_image_batch_pipelined(bank, dm, B, N, di, 0, 1, False)
+        pinned = ctx.d2h(np.zeros_like(masks), di)
+        ctx.free(dm)
+        ctx.free(di)
+        bank.close()
+    finally:
+        ctx.close()
+    assert np.array_equal(split, pinned)
+    ref = orc.calc_socs(orc.mask_spectrum(masks[-1]), K.astype(np.complex128),
+                        sc.astype(np.float32), Nx, Nx, nthreads=8)
+    e = max_rel(split[-1], ref)
+    print(f"entry split K={nk} B={B}: {e:.2e} vs FP64, {max_rel(split, one):.2e} vs one block")
+    assert e < IMG_TOL
+    assert max_rel(split, one) < IMG_TOL
+
+
 def test_invalid_requests_raise(gpu, clip512):
     case("invalid shapes")
     with pytest.raises(gpu.LithoError):
