@@ -204,6 +204,45 @@ def mixed_size_bench(ctx, src, sigma, steps):
     return out
 
 
+def small_clip_bench(ctx, src, sigma, steps, with_oracle):
+    """Config 1: 512^2 clips at 4 nm (period 2048 nm), Nx = 17, K = 10 from the
+    GPU Gram route, plain FP32 on the 81^2 band grid; images/s over batches of
+    1024 clips resident in HBM, and one clip against the FP64 oracle."""
+    N1, B1, nk = 512, 1024, 10
+    Nx = L.compute_nx(NA, PIX_NM * N1, sigma, LAMBDA)
+    sc, K, flags = L.kernels_from_source(src, sigma, LAMBDA, NA, DEFOCUS, PIX_NM * N1,
+                                         PIX_NM * N1, Nx, Nx, nk)
+    bank = ctx.bank(K, sc, Nx, Nx)
+    masks = np.stack([L.manhattan_mask(N1, 9000 + i, WMIN, WMAX, FILL) for i in range(B1)])
+    dm, di = ctx.alloc(masks.nbytes), ctx.alloc(masks.nbytes)
+    ctx.h2d(dm, masks)
+    for _ in range(3):
+        ctx.socs_image_batch(bank, dm, B1, N1, di)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.socs_image_batch(bank, dm, B1, N1, di)
+    ctx.sync()
+    rate = B1 * steps / (time.perf_counter() - t)
+    out = {"N": N1, "Nx": Nx, "K": nk, "batch": B1, "conv_grid": 81, "flags": flags,
+           "value": round(rate, 1), "unit": "images/s",
+           "path_frac": round(8 * N1 * N1 * rate / 1e9 / HBM_PEAK_GBPS, 4)}
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        img = ctx.d2h(np.zeros((N1, N1), np.float32), di)
+        ref = O.calc_socs(O.mask_spectrum(masks[0]), K.astype(np.complex128),
+                          sc.astype(np.float32), Nx, Nx, nthreads=min(16, os.cpu_count() or 1))
+        out["max_rel_err_vs_fp64"] = float(np.abs(img.astype(np.float64) - ref).max()
+                                           / np.abs(ref).max())
+    log(f"config 1 (512^2, K=10): {rate:.1f} images/s"
+        + (f", {out['max_rel_err_vs_fp64']:.2e} vs FP64" if with_oracle else ""))
+    ctx.free(dm)
+    ctx.free(di)
+    bank.close()
+    return out
+
+
 def large_clip_bench(ctx, src, sigma, steps, with_oracle):
     """Config 5: 4096^2 clips at 4 nm (period 16384 nm), Nx = 137, K = 64 from
     the implicit TCC (source-space Gram route); the SOCS loop in FP32 on a
@@ -566,6 +605,7 @@ def main():
     out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
     mixed_sizes = None if args.no_mixed else mixed_size_bench(ctx, src, sigma,
                                                               max(3, args.steps // 4))
+    small = small_clip_bench(ctx, src, sigma, 10, not args.no_cpu)
     large = None if args.no_large else large_clip_bench(ctx, src, sigma, 4, not args.no_cpu)
     surrogate = surrogate_bench(ctx, src, sigma, args.surrogate_s) if args.surrogate_s > 0 else None
 
@@ -672,6 +712,7 @@ def main():
                   "scope": "host wall clock (ms) inside the library: pupil matrix, contraction, "
                            "eigensolve + kernel extraction"},
         "batching_form": mixed_sizes,
+        "small_clip_form": small,
         "large_clip_form": large,
         "learned_form": surrogate,
         "cpu_baseline": cpu,
