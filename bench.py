@@ -397,6 +397,8 @@ def main():
                     help="skip the 4096^2 (config 5) measurement")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-size (batching form) measurement")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 512^2 (config 1) measurement")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 4096^2 (config 5) measurement and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -605,7 +607,7 @@ def main():
     out_imgs = ctx.d2h(np.zeros((min(B, 8), N, N), np.float32), d_img)
     mixed_sizes = None if args.no_mixed else mixed_size_bench(ctx, src, sigma,
                                                               max(3, args.steps // 4))
-    small = small_clip_bench(ctx, src, sigma, 10, not args.no_cpu)
+    small = None if args.no_small else small_clip_bench(ctx, src, sigma, 10, not args.no_cpu)
     large = None if args.no_large else large_clip_bench(ctx, src, sigma, 4, not args.no_cpu)
     surrogate = surrogate_bench(ctx, src, sigma, args.surrogate_s) if args.surrogate_s > 0 else None
 
