@@ -1949,18 +1949,25 @@ int socs_rows_per_block(int G, int Nx) {
     return 0;
 }
 
+typedef float f2nt __attribute__((ext_vector_type(2)));  // for 8-byte streaming stores
+
 // Column pass of the large-grid SOCS: one transform per (image b, entry k,
 // window column col): the inverse G-point FFT over ky of K_k M (W nonzero
 // inputs at indices 0..W-1, as socs_accum2), written to Yt[b][k][col][0..G).
-template <int P, int Q, int NZ, int NTH, int BK>
+// A block takes one column group of one entry for CB consecutive images
+// (grid z = ceil(B / CB)).  CB > 1: the lane's bank values stay in registers
+// for all CB images, and image c + 1's window values are loaded while image
+// c is transformed.
+template <int P, int Q, int NZ, int NTH, int BK, int CB>
 __global__ __launch_bounds__(NTH) void socs_cols_kernel(const float2* __restrict__ M,
                                                         const void* __restrict__ bank,
                                                         const float2* __restrict__ kscale, int nk,
-                                                        int Nx, float2* __restrict__ Yt) {
+                                                        int Nx, int B, float2* __restrict__ Yt) {
     using Geo = Fft2Geom<P, Q>;
     constexpr int G = Geo::G;
     constexpr int T = Geo::T;
     constexpr int SLOTS = NTH / T;
+    constexpr int S1 = Geo::S1;
     extern __shared__ float2 lds[];
     float2* twG = lds;
     float2* xbuf = lds + G;
@@ -1972,26 +1979,79 @@ __global__ __launch_bounds__(NTH) void socs_cols_kernel(const float2* __restrict
     const int cgrp = lin % gridDim.x, kb = lin / gridDim.x;
     const int col = cgrp * SLOTS + slot;
     const int k = kb % gridDim.y;
-    const long b = kb / gridDim.y;
+    const long b0 = (long)(kb / gridDim.y) * CB;
     const bool act = col < W;
-    const float2* Mb = M + b * (long)W * W;
     const long kbase = (long)k * W * W;
     float2* xb = xbuf + slot * Geo::XBUF;
     fill_twiddles<G>(twG, tid, NTH);
-    __syncthreads();
-    fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
-        float2 v = make_float2(0.f, 0.f);
-        if (act && n < W) {
-            const int j = n * W + col;
-            v = cmul(Bank<BK>::ld(bank, kscale, kbase + j, k), Mb[j]);
+    if constexpr (CB == 1) {
+        const float2* Mb = M + b0 * (long)W * W;
+        __syncthreads();
+        fft2_stage1<Geo, NZ, true>(t, xb, twG, [&](int n) {
+            float2 v = make_float2(0.f, 0.f);
+            if (act && n < W) {
+                const int j = n * W + col;
+                v = cmul(Bank<BK>::ld(bank, kscale, kbase + j, k), Mb[j]);
+            }
+            return v;
+        });
+        __syncthreads();
+        float2* yo = Yt + ((b0 * nk + k) * (long)W + (act ? col : 0)) * G;
+        fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) {
+            if (act) yo[g] = val;
+        });
+    } else {
+        // the lane's inputs: rows n = Q n1 + t + T s (n1 < NZ) of column col
+        float2 kv[S1][NZ], mv[S1][NZ];
+        auto load_m = [&](long b) {
+            const float2* Mb = M + b * (long)W * W;
+#pragma unroll
+            for (int s = 0; s < S1; s++)
+#pragma unroll
+                for (int n1 = 0; n1 < NZ; n1++) {
+                    const int n = Q * n1 + t + T * s;
+                    mv[s][n1] = act && n < W ? Mb[n * W + col] : make_float2(0.f, 0.f);
+                }
+        };
+#pragma unroll
+        for (int s = 0; s < S1; s++)
+#pragma unroll
+            for (int n1 = 0; n1 < NZ; n1++) {
+                const int n = Q * n1 + t + T * s;
+                kv[s][n1] = act && n < W ? Bank<BK>::ld(bank, kscale, kbase + n * W + col, k)
+                                         : make_float2(0.f, 0.f);
+            }
+        load_m(b0);
+        __syncthreads();
+        for (int c = 0; c < CB; c++) {
+            const long b = b0 + c;
+            if (b >= B) break;  // the same for the whole block
+            float2 xin[S1][NZ];
+#pragma unroll
+            for (int s = 0; s < S1; s++)
+#pragma unroll
+                for (int n1 = 0; n1 < NZ; n1++) xin[s][n1] = cmul(kv[s][n1], mv[s][n1]);
+            if (c + 1 < CB && b + 1 < B) load_m(b + 1);
+            if (c > 0) __syncthreads();  // the previous image's stage 2 has read xb
+            fft2_stage1_ij<Geo, NZ, true>(t, xb, twG, [&](int n1, int s) { return xin[s][n1]; });
+            __syncthreads();
+            float2* yo = Yt + ((b * nk + k) * (long)W + (act ? col : 0)) * G;
+            // Yt is read once, by the row pass: streaming stores
+            fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) {
+                f2nt w;
+                w.x = val.x;
+                w.y = val.y;
+                if (act) __builtin_nontemporal_store(w, (f2nt*)(yo + g));
+            });
         }
-        return v;
-    });
-    __syncthreads();
-    float2* yo = Yt + ((b * nk + k) * (long)W + (act ? col : 0)) * G;
-    fft2_stage2<Geo, true>(t, xb, [&](int, int g, float2 val) {
-        if (act) yo[g] = val;
-    });
+    }
+}
+
+// images per block of the column pass: 8 where a lane's stage-1 inputs fit in
+// registers three times over (bank, window, product), else 1
+template <int NZ, int S1>
+constexpr int cols_images_per_block() {
+    return NZ * S1 <= 12 ? 8 : 1;
 }
 
 template <int P, int Q, int GR, int NZ, int NTH, int BK>
@@ -2005,12 +2065,13 @@ static int launch_socs2_pre(const float2* M, const void* bank, const float2* ksc
     const size_t lds_c = (size_t)(G + SLOTS * Geo::XBUF) * sizeof(float2);
     const size_t lds_r = (size_t)(G + SLOTS * Geo::XBUF) * sizeof(float2);  // rows read in place
     if (lds_c > 160 * 1024 || lds_r > 160 * 1024) return LITHO_EARG;
-    auto kc = socs_cols_kernel<P, Q, NZ, NTH, BK>;
+    constexpr int CB = cols_images_per_block<NZ, Geo::S1>();
+    auto kc = socs_cols_kernel<P, Q, NZ, NTH, BK, CB>;
     auto kr = socs_accum2_kernel<P, Q, GR, NZ, NTH, BK, true>;
     LITHO_TRY(allow_full_lds((const void*)kc));
     LITHO_TRY(allow_full_lds((const void*)kr));
-    hipLaunchKernelGGL(kc, dim3(cdiv(W, SLOTS), nk, B), dim3(NTH), lds_c, st, M, bank, kscale, nk,
-                       Nx, Yt);
+    hipLaunchKernelGGL(kc, dim3(cdiv(W, SLOTS), nk, cdiv(B, CB)), dim3(NTH), lds_c, st, M, bank,
+                       kscale, nk, Nx, B, Yt);
     hipLaunchKernelGGL(kr, dim3((unsigned)((size_t)B * (G / GR))), dim3(NTH), lds_r, st, M, bank,
                        kscale, scales, nk, Nx, out, shift, tw, (const float2*)Yt);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;

@@ -278,6 +278,43 @@ def test_large_clip_576_and_pow2_grids_agree(gpu, base_source):
     assert max_rel(outs[("fp16", a)], outs[("fp16", p)]) < IMG_TOL
 
 
+@pytest.mark.parametrize("N,B", [(2048, 11), (4096, 9)])
+def test_large_grid_batch_matches_single_clips(gpu, orc, base_source, N, B):
+    """The large-grid column pass takes several clips per block: a batch whose
+    size is not a multiple of that count gives every clip the image it gets
+    alone (bit for bit), in FP32 and with the fp16 bank, and the last clip
+    matches the FP64 oracle."""
+    case(f"large-grid batch N={N} B={B}")
+    src, s = base_source
+    Nx = orc.compute_nx(BASE_NA, 4 * N, s, BASE_LAMBDA)
+    sc, K, _ = gpu.kernels_from_source(src, s, BASE_LAMBDA, BASE_NA, 0.0, 4 * N, 4 * N, Nx, Nx, 16)
+    masks = np.stack([gpu.manhattan_mask(N, 300 + i) for i in range(B)])
+    ctx = gpu.Context()
+    try:
+        bank = ctx.bank(K, sc, Nx, Nx)
+        dm, di = ctx.alloc(masks.nbytes), ctx.alloc(masks.nbytes)
+        ctx.h2d(dm, masks)
+        one = masks[0].nbytes
+        for flag in (0, gpu.LITHO_PREC_BANK_FP16):
+            ctx.socs_image_batch_pipelined(bank, dm, B, N, di, flag, 1, False)
+            batch = ctx.d2h(np.zeros_like(masks), di)
+            for b in (0, 7, B - 1):
+                ctx.socs_image_batch_pipelined(bank, type(dm)(dm.value + b * one), 1, N, di, flag,
+                                               1, False)
+                alone = ctx.d2h(np.zeros_like(masks[0]), di)
+                assert np.array_equal(batch[b], alone), (flag, b)
+            if flag == 0:
+                last = batch[B - 1]
+        ctx.free(dm)
+        ctx.free(di)
+        bank.close()
+    finally:
+        ctx.close()
+    ref = orc.calc_socs(orc.mask_spectrum(masks[B - 1]), K.astype(np.complex128),
+                        sc.astype(np.float32), Nx, Nx, nthreads=16)
+    assert max_rel(last, ref) < IMG_TOL
+
+
 @pytest.mark.parametrize("N", [256, 2048])
 def test_other_clip_sizes(gpu, orc, base_source, N):
     case(f"socs_image N={N}")
