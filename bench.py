@@ -5,7 +5,8 @@ Workload (the production clip of the reference's SOCS path): 1024 x 1024
 binary masks at 4 nm/pixel (period 4096 nm), seeded Manhattan rectangles with
 sides of 12..45 pixels (48..180 nm) to 40 % fill; lambda 193 nm, NA 0.85, zero
 defocus, 41 x 41 annular source sigma 0.6-0.9 (572 points); Nx = 34 (69 x 69
-window), K = 24 kernels extracted on the GPU; 256 x 256 conv grid.
+window), K = 24 kernels extracted on the GPU; the 144 x 144 band conv grid
+(the reference's nextPow2 grid is 256: FI makes the image independent of it).
 
 A step is one pass of the hot path over one batch of B masks already resident
 in HBM: mask spectrum window -> SOCS accumulate -> Fourier interpolation ->

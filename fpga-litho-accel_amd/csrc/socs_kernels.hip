@@ -365,8 +365,10 @@ __device__ __forceinline__ void wave_sync() {
 // (column-major per entry); the block loads its GR rows of every entry from
 // there instead of recomputing every column (grids of 512 points and up,
 // where a block holds only GR of the G rows).
+// With PRE the block is held to 128 VGPRs, four wavefronts per SIMD (one
+// spilled register, outside the entry loop): 3 % faster at the 576 grid.
 template <int P, int Q, int GR, int NZ, int NTH, int BK = 0, bool PRE = false>
-__global__ __launch_bounds__(NTH) void socs_accum2_kernel(const float2* __restrict__ M,
+__global__ __launch_bounds__(NTH, PRE ? 4 : 1) void socs_accum2_kernel(const float2* __restrict__ M,
                                                           const void* __restrict__ bank,
                                                           const float2* __restrict__ kscale,
                                                           const float* __restrict__ scales,
@@ -1576,16 +1578,26 @@ __global__ __launch_bounds__(256) void image_rows_kernel(const float2* __restric
 // sequence), three radix-16 stages in registers, one pair per 256-thread block:
 //   n = t + 256 n3 (t = n1 + 16 n2), k = a + 16 c + 256 d
 //   A: thread t, DFT16 over n3 -> a, times W_4096^(t a)
-//   B: thread (a, n1), DFT16 over n2 -> c, times W_256^(n1 c)
-//   C: thread (a, c), DFT16 over n1 -> d: X[a + 16 c + 256 d]
-// with two transposes through LDS.  The inter-stage twiddles come from two
-// 64-entry tables, W_4096^m = W_64^(m >> 6) W_4096^(m & 63).  The inverse is
-// the same algebra with conjugate roots (input index k, output index n).
+//   B: thread (a, n1) = (tid >> 4, tid & 15), DFT16 over n2 -> c, times W_256^(n1 c)
+//   C: thread (a, c) = (tid & 15, tid >> 4), DFT16 over n1 -> d: X[a + 16 c + 256 d]
+// with two transposes through LDS.  Stage C's thread order makes a + 16 c =
+// tid, so thread tid ends with X[tid + 256 d]: a wavefront holds 64
+// consecutive outputs of every d and stores them without a reordering pass.
+// Each transpose moves one float component at a time through a single
+// 17 KB plane (write x, read x, write y, read y), so a block takes 18 KB of
+// LDS and eight blocks (32 wavefronts) fit a CU.  The inter-stage twiddles
+// of a thread are the powers 1..15 of one root per stage (f4k_powers), the
+// root taken from two 64-entry tables, W_4096^m = W_64^(m >> 6) W_4096^(m & 63).
+// The inverse is the same algebra with conjugate roots (input index k,
+// output index n).  Plane pitches (4-byte words): [a][t] at 272 (= 16 mod
+// 64, so the stage-B reads of lanes n1 = 0..15 of four a fall on 64
+// distinct banks), [n1][16 c + a] at 260 (= 4 mod 64: the stage-B writes
+// likewise); the stage-A writes and stage-C reads are consecutive.
 // ---------------------------------------------------------------------------
-constexpr int F4K_P1 = 257;  // pitch of the [a][t] transpose (t < 256)
-constexpr int F4K_P2 = 17;   // pitch of the [a c][n1] transpose
-constexpr int F4K_BUF = 256 * F4K_P2 > 16 * F4K_P1 ? 256 * F4K_P2 : 16 * F4K_P1;
-static_assert(F4K_BUF >= 4096 + 4096 / 32, "room for the padded pixel-order copy");
+constexpr int F4K_PA = 272;  // pitch of the [a][t] plane (t < 256)
+constexpr int F4K_PB = 260;  // pitch of the [n1][16 c + a] plane
+constexpr int F4K_BUF = 16 * F4K_PA / 2;  // the plane, in float2 (also used as float2 after the FFT)
+static_assert(16 * F4K_PA >= 16 * F4K_PB && F4K_BUF >= 2 * (256 + 256 / 32), "transpose plane");
 constexpr int F4K_NEG = 256 + 256 / 32;  // the d = 15 band after the d = 0 band (padded)
 
 __device__ __forceinline__ void f4k_tables(float2* __restrict__ lo, float2* __restrict__ hi,
@@ -1599,37 +1611,71 @@ __device__ __forceinline__ void f4k_tables(float2* __restrict__ lo, float2* __re
     }
 }
 
+// v[j] *= w^j (j = 1..15) for w = W_4096^m (conjugate when INV), m < 4096:
+// one table product for w, the other powers as products of depth <= 4
+// (w^2, w^3 = w^2 w, w^4 = (w^2)^2, w^(4+j) = w^4 w^j, w^(8+j) = w^8 w^j).
 template <bool INV>
-__device__ __forceinline__ float2 f4k_tw(const float2* __restrict__ lo,
-                                         const float2* __restrict__ hi, int m, float2 v) {
-    const float2 w = cmul(hi[(m >> 6) & 63], lo[m & 63]);
-    return cmul_w<false>(v, INV ? make_float2(w.x, -w.y) : w);
+__device__ __forceinline__ void f4k_powers(const float2* __restrict__ lo,
+                                           const float2* __restrict__ hi, int m,
+                                           float2 (&v)[16]) {
+    float2 w1 = cmul(hi[m >> 6], lo[m & 63]);
+    if (INV) w1.y = -w1.y;
+    const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
+    const float2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3), w8 = cmul(w4, w4);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], w5);
+    v[6] = cmul(v[6], w6);
+    v[7] = cmul(v[7], w7);
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w8, w1));
+    v[10] = cmul(v[10], cmul(w8, w2));
+    v[11] = cmul(v[11], cmul(w8, w3));
+    v[12] = cmul(v[12], cmul(w8, w4));
+    v[13] = cmul(v[13], cmul(w8, w5));
+    v[14] = cmul(v[14], cmul(w8, w6));
+    v[15] = cmul(v[15], cmul(w8, w7));
 }
 
-// v (thread t: inputs t + 256 n3, n3 = 0..15) -> v (thread (a, c) = (tid >> 4,
-// tid & 15): outputs a + 16 c + 256 d, d = 0..15).  All 256 threads call it.
+// v (thread t: inputs t + 256 n3, n3 = 0..15) -> v (thread tid: outputs tid +
+// 256 d, d = 0..15).  All 256 threads call it; buf is the F4K_BUF plane.  A
+// component is overwritten only after its old value has been written out.
 template <bool INV>
 __device__ __forceinline__ void fft4096_pair(int tid, float2 (&v)[16], float2* __restrict__ buf,
                                              const float2* __restrict__ lo,
                                              const float2* __restrict__ hi) {
+    float* pl = reinterpret_cast<float*>(buf);
     dft_reg<16, INV>(v);
+    f4k_powers<INV>(lo, hi, tid, v);  // W_4096^(t a)
+    const int a = tid >> 4, r = tid & 15;  // stage B: (a, n1 = r)
 #pragma unroll
-    for (int a = 1; a < 16; a++) v[a] = f4k_tw<INV>(lo, hi, tid * a, v[a]);
-#pragma unroll
-    for (int a = 0; a < 16; a++) buf[a * F4K_P1 + tid] = v[a];
+    for (int k = 0; k < 16; k++) pl[k * F4K_PA + tid] = v[k].x;
     __syncthreads();
-    const int a = tid >> 4, r = tid & 15;  // stage B: (a, n1 = r); stage C: (a, c = r)
 #pragma unroll
-    for (int n2 = 0; n2 < 16; n2++) v[n2] = buf[a * F4K_P1 + r + 16 * n2];
+    for (int n2 = 0; n2 < 16; n2++) v[n2].x = pl[a * F4K_PA + r + 16 * n2];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) pl[k * F4K_PA + tid] = v[k].y;
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; n2++) v[n2].y = pl[a * F4K_PA + r + 16 * n2];
     dft_reg<16, INV>(v);
-#pragma unroll
-    for (int c = 1; c < 16; c++) v[c] = f4k_tw<INV>(lo, hi, 16 * r * c, v[c]);
+    f4k_powers<INV>(lo, hi, 16 * r, v);  // W_256^(n1 c)
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 16; c++) buf[(16 * a + c) * F4K_P2 + r] = v[c];
+    for (int c = 0; c < 16; c++) pl[r * F4K_PB + 16 * c + a] = v[c].x;
+    __syncthreads();
+    // stage C: thread (a, c) = (tid & 15, tid >> 4) reads its 16 n1 values
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) v[n1].x = pl[n1 * F4K_PB + tid];
     __syncthreads();
 #pragma unroll
-    for (int n1 = 0; n1 < 16; n1++) v[n1] = buf[(16 * a + r) * F4K_P2 + n1];
+    for (int c = 0; c < 16; c++) pl[r * F4K_PB + 16 * c + a] = v[c].y;
+    __syncthreads();
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) v[n1].y = pl[n1 * F4K_PB + tid];
     dft_reg<16, INV>(v);
 }
 
@@ -1656,9 +1702,8 @@ __global__ __launch_bounds__(256) void rows4096_kernel(const float* __restrict__
     // X[k] for |k| < 256 in natural order: d = 0 (k = a + 16 c) and d = 15
     // (k - 4096); entry m at m + m / 32 (lanes 16 apart on different banks)
     __syncthreads();
-    const int m0 = (tid >> 4) + 16 * (tid & 15);
-    buf[m0 + (m0 >> 5)] = v[0];
-    buf[F4K_NEG + m0 + (m0 >> 5)] = v[15];
+    buf[tid + (tid >> 5)] = v[0];             // X[tid]
+    buf[F4K_NEG + tid + (tid >> 5)] = v[15];  // X[tid + 3840] = X[tid - 256]
     __syncthreads();
     auto at = [&](int k) {  // X[k mod 4096], |k| < 256
         const int m = k >= 0 ? k : k + 256;  // -k' -> 4096 - k' = 3840 + (256 - k')
@@ -1711,24 +1756,13 @@ __global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict
     f4k_tables(lo, hi, tid);
     __syncthreads();
     fft4096_pair<true>(tid, v, buf, lo, hi);
-    // thread (a, c) holds pixels a + 16 c + 256 d: back through LDS in pixel
-    // order, so each store instruction writes 1 KB of a row (pixel n at
-    // n + n / 32: lanes 16 pixels apart fall on different banks)
-    const int n0 = (tid >> 4) + 16 * (tid & 15);
-    __syncthreads();
-#pragma unroll
-    for (int d = 0; d < 16; d++) {
-        const int n = n0 + 256 * d;
-        buf[n + (n >> 5)] = v[d];
-    }
-    __syncthreads();
+    // thread tid holds pixels tid + 256 d: each store instruction of a
+    // wavefront writes 256 contiguous bytes of a row
     float* o1 = img + (2 * p) * (long)L + tid;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int n = tid + 256 * j;
-        const float2 z = buf[n + (n >> 5)];
-        __builtin_nontemporal_store(z.x, o1 + 256 * j);
-        __builtin_nontemporal_store(z.y, o1 + L + 256 * j);
+    for (int d = 0; d < 16; d++) {
+        __builtin_nontemporal_store(v[d].x, o1 + 256 * d);
+        __builtin_nontemporal_store(v[d].y, o1 + L + 256 * d);
     }
 }
 
@@ -1749,19 +1783,13 @@ __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __rest
     f4k_tables(lo, hi, tid);
     __syncthreads();
     fft4096_pair<false>(tid, v, buf, lo, hi);
-    __syncthreads();
-    const int m0 = (tid >> 4) + 16 * (tid & 15);
-    buf[m0 + (m0 >> 5)] = v[0];            // X[m0]
-    buf[F4K_NEG + m0 + (m0 >> 5)] = v[15];  // X[m0 + 3840]
-    __syncthreads();
+    // thread tid holds X[tid] (d = 0) and X[tid - 256] (d = 15); every ky in
+    // [kymin, kymin + cy) (inside (-256, 256)) is one of them
     const float inv = 1.0f / ((float)N * (float)N);
-    float2* Mb = M + b * (long)cy * cx;
-    for (int j = tid; j < cy; j += 256) {
-        const int ky = kymin + j;
-        const int m = ky >= 0 ? ky : ky + 256;
-        const float2 x = buf[(ky >= 0 ? 0 : F4K_NEG) + m + (m >> 5)];
-        Mb[(long)j * cx + col] = cscale(x, (ky & 1) ? -inv : inv);
-    }
+    float2* Mb = M + b * (long)cy * cx + col;
+    const int j0 = tid - kymin, j1 = tid - 256 - kymin;
+    if (j0 >= 0 && j0 < cy) Mb[(long)j0 * cx] = cscale(v[0], (tid & 1) ? -inv : inv);
+    if (j1 >= 0 && j1 < cy) Mb[(long)j1 * cx] = cscale(v[15], (tid & 1) ? -inv : inv);
 }
 
 // fi_cols4096: P [B][G][nb] -> U [B][4096][nb] as fi_cols_kernel: the G-point
@@ -1815,8 +1843,8 @@ __global__ __launch_bounds__(256) void fi_cols4096_kernel(const float2* __restri
         v[d] = x;
     }
     fft4096_pair<true>(tid, v, buf, lo, hi);
-    // thread (a, c): rows y = a + 16 c + 256 d
-    const int y0 = (tid >> 4) + 16 * (tid & 15);
+    // thread tid: rows y = tid + 256 d
+    const int y0 = tid;
     float2* Uc = U + b * (long)N * nb + col;
     const float sg = (col & 1) ? -1.f : 1.f;
 #pragma unroll
