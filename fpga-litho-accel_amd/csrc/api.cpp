@@ -1375,10 +1375,22 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
     void *R = ws[0], *M = ws[1], *T = ws[2], *P = ws[3], *U = ws[4];
     static const char* names[] = {"mask_rows", "mask_cols", "socs_accum", "fi_rows", "fi_cols",
                                   "image_rows"};
+    // 4096^2 clips: the row-DFT window R and the FI column output U stored
+    // transposed per clip, so the column passes read and write whole columns
+    // (per 32 clips at K = 64: mask columns 237 -> 68 us, FI columns 256 ->
+    // 104 us; the mask rows 410 -> 467 us, the image rows, which load their
+    // rows' values of U in tiles of two row pairs, 433 -> 538 us)
+    const bool tr = N == 4096 && Nx < 256;
     if (timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
-    LITHO_TRY(launch_rows_dft(N, d_mask, (float2*)R, (long)B * N, -Nx, W, true, c->tw, st));
+    if (tr)
+        LITHO_TRY(launch_rows4096_tr(d_mask, (float2*)R, B, -Nx, W, true, st));
+    else
+        LITHO_TRY(launch_rows_dft(N, d_mask, (float2*)R, (long)B * N, -Nx, W, true, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[1], st));
-    LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
+    if (tr)
+        LITHO_TRY(launch_mask_cols4096_tr((float2*)R, (float2*)M, B, W, -Nx, W, st));
+    else
+        LITHO_TRY(launch_mask_cols(N, (float2*)R, (float2*)M, B, W, -Nx, W, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[2], st));
     // a real mask's window spectrum, conjugate-symmetric to FP32 round-off: the
     // paired set (cross-talk of that order, far below the FP32 image error)
@@ -1387,9 +1399,15 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
     if (timing) LITHO_HIP(hipEventRecord(c->ev[3], st));
     LITHO_TRY(launch_rows_dft(G, (float*)T, (float2*)P, (long)B * G, 0, band + 1, false, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[4], st));
-    LITHO_TRY(launch_fi_cols(G, N, (float2*)P, (float2*)U, B, band, 0, c->tw, st));
+    if (tr)
+        LITHO_TRY(launch_fi_cols4096_tr(G, (float2*)P, (float2*)U, B, band, 0, c->tw, st));
+    else
+        LITHO_TRY(launch_fi_cols(G, N, (float2*)P, (float2*)U, B, band, 0, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[5], st));
-    LITHO_TRY(launch_image_rows(N, (float2*)U, d_image, (long)B * N, band, c->tw, st));
+    if (tr)
+        LITHO_TRY(launch_image4096_tr((float2*)U, d_image, B, band, st));
+    else
+        LITHO_TRY(launch_image_rows(N, (float2*)U, d_image, (long)B * N, band, c->tw, st));
     if (timing) LITHO_HIP(hipEventRecord(c->ev[6], st));
     c->nstage = 6;
     for (int i = 0; i < 6; i++) c->stage_names[i] = names[i];

@@ -1680,16 +1680,20 @@ __device__ __forceinline__ void fft4096_pair(int tid, float2 (&v)[16], float2* _
 }
 
 // Mask rows of 4096 pixels -> window R [rows][count] (kx = kmin + j, |kx| < 256),
-// as rows_dft_kernel: two real rows per transform, (-1)^kx when ALT.
-template <bool ALT>
+// as rows_dft_kernel: two real rows per transform, (-1)^kx when ALT.  TR: R is
+// stored transposed per clip of ncl rows, R [clip][j][y], so that the column
+// pass reads each column contiguously; a block writes its two rows' values as
+// one 16-byte pair per j, and the row pairs are taken in XCD order, so the
+// eight pairs that share a 128-byte line of R run on one XCD (one L2).
+template <bool ALT, bool TR = false>
 __global__ __launch_bounds__(256) void rows4096_kernel(const float* __restrict__ in,
                                                        float2* __restrict__ out, int nrows,
-                                                       int kmin, int count) {
+                                                       int kmin, int count, int ncl) {
     constexpr int L = 4096;
     __shared__ float2 buf[F4K_BUF];
     __shared__ float2 lo[64], hi[64];
     const int tid = threadIdx.x;
-    const long p = blockIdx.x;  // row pair
+    const long p = TR ? xcd_block_order() : (long)blockIdx.x;  // row pair
     const float* r0 = in + 2 * p * (long)L + tid;
     float2 v[16];
 #pragma unroll
@@ -1719,8 +1723,13 @@ __global__ __launch_bounds__(256) void rows4096_kernel(const float* __restrict__
             x1 = cscale(x1, -1.f);
             x2 = cscale(x2, -1.f);
         }
-        out[(2 * p) * (long)count + j] = x1;
-        out[(2 * p + 1) * (long)count + j] = x2;
+        if (TR) {
+            const long b = (2 * p) / ncl, y = 2 * p - b * ncl;  // y even, ncl even
+            *(float4*)(out + (b * count + j) * ncl + y) = make_float4(x1.x, x1.y, x2.x, x2.y);
+        } else {
+            out[(2 * p) * (long)count + j] = x1;
+            out[(2 * p + 1) * (long)count + j] = x2;
+        }
     }
 }
 
@@ -1766,6 +1775,65 @@ __global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict
     }
 }
 
+// U transposed per clip, U [clip][nb][4096] -> image rows, NP row pairs (2 NP
+// consecutive rows of one clip) per block: the block first copies its rows'
+// values of every kx into LDS, 16 bytes per lane and 8 NP bytes per kx, so the
+// loads cover whole line segments, then runs its NP inverse transforms from
+// there (the c2r of two rows as one complex transform, as image4096_kernel).
+// Dynamic LDS: nb * (2 NP + 1) float2 (pitch padded by one).
+template <int NP>
+__global__ __launch_bounds__(256) void image4096_tiles_kernel(const float2* __restrict__ U,
+                                                              float* __restrict__ img, int band) {
+    constexpr int L = 4096, R2 = 2 * NP, SP = R2 + 1;
+    static_assert(R2 % 2 == 0 && (256 % (R2 / 2)) == 0, "16-byte pieces per kx");
+    __shared__ float2 buf[F4K_BUF];
+    __shared__ float2 lo[64], hi[64];
+    extern __shared__ float2 spec[];  // [nb][SP]
+    const int tid = threadIdx.x;
+    const int nb = band + 1;
+    const int lin = xcd_block_order();  // neighbouring tiles (one line) on one XCD
+    constexpr int TPC = L / R2;        // tiles per clip
+    const long b = lin / TPC;
+    const int y0 = (lin - (int)b * TPC) * R2;
+    const float2* Ub = U + b * (long)nb * L + y0;
+    constexpr int PK = R2 / 2;  // 16-byte pieces per kx
+    for (int i = tid; i < nb * PK; i += 256) {
+        const int kx = i / PK, q = i - kx * PK;
+        const float4 v = *(const float4*)(Ub + (long)kx * L + 2 * q);
+        spec[kx * SP + 2 * q] = make_float2(v.x, v.y);
+        spec[kx * SP + 2 * q + 1] = make_float2(v.z, v.w);
+    }
+    f4k_tables(lo, hi, tid);
+    __syncthreads();
+    for (int pp = 0; pp < NP; pp++) {
+        float2 v[16];
+#pragma unroll
+        for (int d = 0; d < 16; d++) {
+            const int k = tid + 256 * d;
+            float2 sv = make_float2(0.f, 0.f);
+            if (k <= band) {
+                const float2 a1 = spec[k * SP + 2 * pp], a2 = spec[k * SP + 2 * pp + 1];
+                sv = k == 0 ? make_float2(a1.x, a2.x) : make_float2(a1.x - a2.y, a1.y + a2.x);
+            } else if (L - k <= band) {
+                const float2 a1 = spec[(L - k) * SP + 2 * pp], a2 = spec[(L - k) * SP + 2 * pp + 1];
+                sv = make_float2(a1.x + a2.y, a2.x - a1.y);
+            }
+            v[d] = sv;
+        }
+        if (pp > 0) __syncthreads();  // the previous transform's last reads of buf
+        fft4096_pair<true>(tid, v, buf, lo, hi);
+        float* o1 = img + (b * L + y0 + 2 * pp) * (long)L + tid;
+#pragma unroll
+        for (int d = 0; d < 16; d++) {
+            __builtin_nontemporal_store(v[d].x, o1 + 256 * d);
+            __builtin_nontemporal_store(v[d].y, o1 + L + 256 * d);
+        }
+    }
+}
+
+// TR: R [clip][cx][4096] (the transposed form of rows4096_kernel<.., true>),
+// each column read contiguously.
+template <bool TR = false>
 __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __restrict__ R,
                                                             float2* __restrict__ M, int cx,
                                                             int kymin, int cy) {
@@ -1776,10 +1844,16 @@ __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __rest
     const int lin = xcd_block_order();
     const int col = lin % cx;
     const long b = lin / cx;
-    const float2* Rc = R + b * (long)N * cx + col;
     float2 v[16];
+    if (TR) {
+        const float2* Rc = R + (b * cx + col) * (long)N + tid;
 #pragma unroll
-    for (int n3 = 0; n3 < 16; n3++) v[n3] = Rc[(long)(tid + 256 * n3) * cx];
+        for (int n3 = 0; n3 < 16; n3++) v[n3] = Rc[256 * n3];
+    } else {
+        const float2* Rc = R + b * (long)N * cx + col;
+#pragma unroll
+        for (int n3 = 0; n3 < 16; n3++) v[n3] = Rc[(long)(tid + 256 * n3) * cx];
+    }
     f4k_tables(lo, hi, tid);
     __syncthreads();
     fft4096_pair<false>(tid, v, buf, lo, hi);
@@ -1795,7 +1869,8 @@ __global__ __launch_bounds__(256) void mask_cols4096_kernel(const float2* __rest
 // fi_cols4096: P [B][G][nb] -> U [B][4096][nb] as fi_cols_kernel: the G-point
 // forward transform of a column in LDS, the band |ky| <= band embedded in a
 // 4096-point spectrum with the centring signs, the inverse on the register core.
-template <int G>
+// TR: U is written transposed, U [clip][nb][4096], each column contiguously.
+template <int G, bool TR = false>
 __global__ __launch_bounds__(256) void fi_cols4096_kernel(const float2* __restrict__ P,
                                                           float2* __restrict__ U, int band,
                                                           int yalt,
@@ -1844,11 +1919,16 @@ __global__ __launch_bounds__(256) void fi_cols4096_kernel(const float2* __restri
     }
     fft4096_pair<true>(tid, v, buf, lo, hi);
     // thread tid: rows y = tid + 256 d
-    const int y0 = tid;
-    float2* Uc = U + b * (long)N * nb + col;
     const float sg = (col & 1) ? -1.f : 1.f;
+    if (TR) {
+        float2* Uc = U + (b * nb + col) * (long)N + tid;
 #pragma unroll
-    for (int d = 0; d < 16; d++) Uc[(long)(y0 + 256 * d) * nb] = cscale(v[d], sg);
+        for (int d = 0; d < 16; d++) Uc[256 * d] = cscale(v[d], sg);
+    } else {
+        float2* Uc = U + b * (long)N * nb + col;
+#pragma unroll
+        for (int d = 0; d < 16; d++) Uc[(long)(tid + 256 * d) * nb] = cscale(v[d], sg);
+    }
 }
 
 // ===========================================================================
@@ -1877,10 +1957,10 @@ int launch_rows_dft(int L, const float* in, float2* out, long nrows, int kmin, i
         const dim3 grid((unsigned)(nrows / 2));
         if (alt)
             hipLaunchKernelGGL((rows4096_kernel<true>), grid, dim3(256), 0, st, in, out,
-                               (int)nrows, kmin, count);
+                               (int)nrows, kmin, count, 0);
         else
             hipLaunchKernelGGL((rows4096_kernel<false>), grid, dim3(256), 0, st, in, out,
-                               (int)nrows, kmin, count);
+                               (int)nrows, kmin, count, 0);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
 #define LITHO_RF2(p, q, nth)                                                                  \
@@ -1926,8 +2006,8 @@ int launch_mask_cols(int N, const float2* R, float2* M, int B, int cx, int kymin
                      const float2* tw, hipStream_t st) {
     if (B <= 0 || B > 65535 || cx <= 0 || cy <= 0) return LITHO_EARG;
     if (N == 4096 && kymin > -256 && kymin + cy - 1 < 256) {
-        hipLaunchKernelGGL(mask_cols4096_kernel, dim3(cx, B), dim3(256), 0, st, R, M, cx, kymin,
-                           cy);
+        hipLaunchKernelGGL((mask_cols4096_kernel<false>), dim3(cx, B), dim3(256), 0, st, R, M, cx,
+                           kymin, cy);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
     switch (N) {
@@ -2287,8 +2367,8 @@ int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, 
                       hipStream_t st) {
     if (nrows % 2 || 2 * band + 1 > N || nrows > 0x7fffffffL) return LITHO_EARG;
     if (N == 4096) {
-        hipLaunchKernelGGL(image4096_kernel, dim3((unsigned)(nrows / 2)), dim3(256), 0, st, U, img,
-                           (int)nrows, band);
+        hipLaunchKernelGGL(image4096_kernel, dim3((unsigned)(nrows / 2)), dim3(256), 0, st, U,
+                           img, (int)nrows, band);
         return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
     }
 #define LITHO_IR2(p, q, nz, nth)                                                          \
@@ -2321,6 +2401,74 @@ int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, 
     }
 }
 
+
+// The 4096^2 image path with R and U stored transposed per clip ([kx][y]):
+// the column transforms read and write whole columns contiguously, the row
+// transforms move one 16-byte pair per kept frequency (row pairs in XCD
+// order).  Mask rows: window |kx| < 256 (kmin = -Nx, count = 2 Nx + 1).
+int launch_rows4096_tr(const float* in, float2* out, int B, int kmin, int count, bool alt,
+                       hipStream_t st) {
+    if (B <= 0 || count <= 0 || kmin <= -256 || kmin + count - 1 >= 256) return LITHO_EARG;
+    const dim3 grid((unsigned)((long)B * 2048));  // row pairs
+    if (alt)
+        hipLaunchKernelGGL((rows4096_kernel<true, true>), grid, dim3(256), 0, st, in, out,
+                           B * 4096, kmin, count, 4096);
+    else
+        hipLaunchKernelGGL((rows4096_kernel<false, true>), grid, dim3(256), 0, st, in, out,
+                           B * 4096, kmin, count, 4096);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_mask_cols4096_tr(const float2* R, float2* M, int B, int cx, int kymin, int cy,
+                            hipStream_t st) {
+    if (B <= 0 || B > 65535 || cx <= 0 || cy <= 0 || kymin <= -256 || kymin + cy - 1 >= 256)
+        return LITHO_EARG;
+    hipLaunchKernelGGL((mask_cols4096_kernel<true>), dim3(cx, B), dim3(256), 0, st, R, M, cx,
+                       kymin, cy);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+int launch_fi_cols4096_tr(int G, const float2* P, float2* U, int B, int band, int yalt,
+                          const float2* tw, hipStream_t st) {
+    if (B <= 0 || B > 65535 || band < 0 || 2 * band + 1 > G) return LITHO_EARG;
+    const dim3 grid(band + 1, B);
+#define LITHO_FI4K(g)                                                                          \
+    case g:                                                                                    \
+        hipLaunchKernelGGL((fi_cols4096_kernel<g, true>), grid, dim3(256), 0, st, P, U, band,  \
+                           yalt, tw);                                                          \
+        return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+    switch (G) {
+        LITHO_FI4K(32)
+        LITHO_FI4K(64)
+        LITHO_FI4K(128)
+        LITHO_FI4K(256)
+        LITHO_FI4K(512)
+        LITHO_FI4K(576)
+        LITHO_FI4K(1024)
+        default: return LITHO_EARG;
+    }
+#undef LITHO_FI4K
+}
+
+template <int NP>
+static int launch_image4096_tiles(const float2* U, float* img, int B, int band, hipStream_t st) {
+    // static (transpose plane, tables) + dynamic LDS within the default 64 KB
+    // per block, so the kernel needs no raised limit (hipFuncSetAttribute
+    // refuses a 160 KB dynamic limit beside static LDS)
+    const size_t lds = (size_t)(band + 1) * (2 * NP + 1) * sizeof(float2);
+    if (lds + (F4K_BUF + 128) * sizeof(float2) > 64 * 1024) return LITHO_EARG;
+    auto kern = image4096_tiles_kernel<NP>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((long)B * (4096 / (2 * NP)))), dim3(256), lds, st,
+                       U, img, band);
+    return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
+}
+
+// two row pairs per block (5 blocks per CU: 538 us per 32 clips at K = 64,
+// 550 with four pairs, 672 with eight, 641 reading one row pair per block)
+int launch_image4096_tr(const float2* U, float* img, int B, int band, hipStream_t st) {
+    if (B <= 0 || band < 0 || band >= 512) return LITHO_EARG;
+    return launch_image4096_tiles<2>(U, img, B, band, st);
+}
 
 // ===========================================================================
 // band pipeline launchers

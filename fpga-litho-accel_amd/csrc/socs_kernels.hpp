@@ -31,6 +31,15 @@ int launch_fi_cols(int G, int N, const float2* P, float2* U, int B, int band, in
 // FI last axis: U [nrows][band+1] -> image [nrows][N]
 int launch_image_rows(int N, const float2* U, float* img, long nrows, int band, const float2* tw,
                       hipStream_t st);
+// 4096^2 clips with the row-DFT window and the FI column output stored
+// transposed per clip: R [B][count][4096] (|kx| < 256), U [B][band+1][4096].
+int launch_rows4096_tr(const float* in, float2* out, int B, int kmin, int count, bool alt,
+                       hipStream_t st);
+int launch_mask_cols4096_tr(const float2* R, float2* M, int B, int cx, int kymin, int cy,
+                            hipStream_t st);
+int launch_fi_cols4096_tr(int G, const float2* P, float2* U, int B, int band, int yalt,
+                          const float2* tw, hipStream_t st);
+int launch_image4096_tr(const float2* U, float* img, int B, int band, hipStream_t st);
 
 // Band pipeline (image path, square conv grid G = P*P in {36, 81, 144}); h16
 // ... lean: smaller blocks (less LDS) so SOCS and FFT blocks can share a CU;
