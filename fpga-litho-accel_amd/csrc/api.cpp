@@ -1379,7 +1379,7 @@ int image_batch_enqueue(litho_ctx* c, const litho_bank* bank, const float* d_mas
     // transposed per clip, so the column passes read and write whole columns
     // (per 32 clips at K = 64: mask columns 237 -> 68 us, FI columns 256 ->
     // 104 us; the mask rows 410 -> 467 us, the image rows, which load their
-    // rows' values of U in tiles of two row pairs, 433 -> 538 us)
+    // rows' values of U in tiles of two row pairs, 433 -> 506 us)
     const bool tr = N == 4096 && Nx < 256;
     if (timing) LITHO_HIP(hipEventRecord(c->ev[0], st));
     if (tr)

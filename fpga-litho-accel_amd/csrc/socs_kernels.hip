@@ -1775,58 +1775,80 @@ __global__ __launch_bounds__(256) void image4096_kernel(const float2* __restrict
     }
 }
 
-// U transposed per clip, U [clip][nb][4096] -> image rows, NP row pairs (2 NP
-// consecutive rows of one clip) per block: the block first copies its rows'
-// values of every kx into LDS, 16 bytes per lane and 8 NP bytes per kx, so the
-// loads cover whole line segments, then runs its NP inverse transforms from
-// there (the c2r of two rows as one complex transform, as image4096_kernel).
-// Dynamic LDS: nb * (2 NP + 1) float2 (pitch padded by one).
-template <int NP>
+// U transposed per clip, U [clip][nb][4096] -> image rows.  A block takes TPB
+// consecutive tiles of NP row pairs (2 NP rows) of one clip.  For each tile it
+// copies the rows' values of every kx into LDS (16-byte pieces, 16 NP bytes per
+// kx, so the loads cover whole line segments) and runs its NP inverse
+// transforms from there (the c2r of two rows as one complex transform, as
+// image4096_kernel); the next tile's values are loaded into registers while
+// the current tile's transforms run.  Dynamic LDS: nb * (2 NP + 1) float2
+// (pitch padded by one).  band < 512 (PT pieces per thread).
+template <int NP, int TPB>
 __global__ __launch_bounds__(256) void image4096_tiles_kernel(const float2* __restrict__ U,
                                                               float* __restrict__ img, int band) {
-    constexpr int L = 4096, R2 = 2 * NP, SP = R2 + 1;
-    static_assert(R2 % 2 == 0 && (256 % (R2 / 2)) == 0, "16-byte pieces per kx");
+    constexpr int L = 4096, R2 = 2 * NP, SP = R2 + 1, PK = NP;  // PK 16-byte pieces per kx
+    constexpr int PT = (512 * PK + 255) / 256;                 // pieces per thread, nb <= 512
+    constexpr int BPC = L / R2 / TPB;                           // blocks per clip
     __shared__ float2 buf[F4K_BUF];
     __shared__ float2 lo[64], hi[64];
     extern __shared__ float2 spec[];  // [nb][SP]
     const int tid = threadIdx.x;
     const int nb = band + 1;
-    const int lin = xcd_block_order();  // neighbouring tiles (one line) on one XCD
-    constexpr int TPC = L / R2;        // tiles per clip
-    const long b = lin / TPC;
-    const int y0 = (lin - (int)b * TPC) * R2;
-    const float2* Ub = U + b * (long)nb * L + y0;
-    constexpr int PK = R2 / 2;  // 16-byte pieces per kx
-    for (int i = tid; i < nb * PK; i += 256) {
-        const int kx = i / PK, q = i - kx * PK;
-        const float4 v = *(const float4*)(Ub + (long)kx * L + 2 * q);
-        spec[kx * SP + 2 * q] = make_float2(v.x, v.y);
-        spec[kx * SP + 2 * q + 1] = make_float2(v.z, v.w);
-    }
-    f4k_tables(lo, hi, tid);
-    __syncthreads();
-    for (int pp = 0; pp < NP; pp++) {
-        float2 v[16];
+    const long b = blockIdx.x / BPC;
+    const int yb = (int)(blockIdx.x - b * BPC) * TPB * R2;
+    const float2* Ub = U + b * (long)nb * L;
+    float4 pre[PT];
+    auto load = [&](int y0) {
 #pragma unroll
-        for (int d = 0; d < 16; d++) {
-            const int k = tid + 256 * d;
-            float2 sv = make_float2(0.f, 0.f);
-            if (k <= band) {
-                const float2 a1 = spec[k * SP + 2 * pp], a2 = spec[k * SP + 2 * pp + 1];
-                sv = k == 0 ? make_float2(a1.x, a2.x) : make_float2(a1.x - a2.y, a1.y + a2.x);
-            } else if (L - k <= band) {
-                const float2 a1 = spec[(L - k) * SP + 2 * pp], a2 = spec[(L - k) * SP + 2 * pp + 1];
-                sv = make_float2(a1.x + a2.y, a2.x - a1.y);
+        for (int i = 0; i < PT; i++) {
+            const int idx = tid + 256 * i;
+            if (idx < nb * PK) {
+                const int kx = idx / PK, q = idx - kx * PK;
+                pre[i] = *(const float4*)(Ub + (long)kx * L + y0 + 2 * q);
             }
-            v[d] = sv;
         }
-        if (pp > 0) __syncthreads();  // the previous transform's last reads of buf
-        fft4096_pair<true>(tid, v, buf, lo, hi);
-        float* o1 = img + (b * L + y0 + 2 * pp) * (long)L + tid;
+    };
+    load(yb);
+    f4k_tables(lo, hi, tid);
+    for (int j = 0; j < TPB; j++) {
+        const int y0 = yb + j * R2;
+        // every thread is past the previous tile's transforms (they end in
+        // block barriers after their last reads of spec), so spec is free
 #pragma unroll
-        for (int d = 0; d < 16; d++) {
-            __builtin_nontemporal_store(v[d].x, o1 + 256 * d);
-            __builtin_nontemporal_store(v[d].y, o1 + L + 256 * d);
+        for (int i = 0; i < PT; i++) {
+            const int idx = tid + 256 * i;
+            if (idx < nb * PK) {
+                const int kx = idx / PK, q = idx - kx * PK;
+                spec[kx * SP + 2 * q] = make_float2(pre[i].x, pre[i].y);
+                spec[kx * SP + 2 * q + 1] = make_float2(pre[i].z, pre[i].w);
+            }
+        }
+        __syncthreads();  // spec written; the previous transform's reads of buf done
+        if (j + 1 < TPB) load(y0 + R2);
+        for (int pp = 0; pp < NP; pp++) {
+            float2 v[16];
+#pragma unroll
+            for (int d = 0; d < 16; d++) {
+                const int k = tid + 256 * d;
+                float2 sv = make_float2(0.f, 0.f);
+                if (k <= band) {
+                    const float2 a1 = spec[k * SP + 2 * pp], a2 = spec[k * SP + 2 * pp + 1];
+                    sv = k == 0 ? make_float2(a1.x, a2.x) : make_float2(a1.x - a2.y, a1.y + a2.x);
+                } else if (L - k <= band) {
+                    const float2 a1 = spec[(L - k) * SP + 2 * pp],
+                                 a2 = spec[(L - k) * SP + 2 * pp + 1];
+                    sv = make_float2(a1.x + a2.y, a2.x - a1.y);
+                }
+                v[d] = sv;
+            }
+            if (pp > 0) __syncthreads();  // the previous transform's last reads of buf
+            fft4096_pair<true>(tid, v, buf, lo, hi);
+            float* o1 = img + (b * L + y0 + 2 * pp) * (long)L + tid;
+#pragma unroll
+            for (int d = 0; d < 16; d++) {
+                __builtin_nontemporal_store(v[d].x, o1 + 256 * d);
+                __builtin_nontemporal_store(v[d].y, o1 + L + 256 * d);
+            }
         }
     }
 }
@@ -2450,24 +2472,25 @@ int launch_fi_cols4096_tr(int G, const float2* P, float2* U, int B, int band, in
 #undef LITHO_FI4K
 }
 
-template <int NP>
+template <int NP, int TPB>
 static int launch_image4096_tiles(const float2* U, float* img, int B, int band, hipStream_t st) {
     // static (transpose plane, tables) + dynamic LDS within the default 64 KB
     // per block, so the kernel needs no raised limit (hipFuncSetAttribute
     // refuses a 160 KB dynamic limit beside static LDS)
     const size_t lds = (size_t)(band + 1) * (2 * NP + 1) * sizeof(float2);
     if (lds + (F4K_BUF + 128) * sizeof(float2) > 64 * 1024) return LITHO_EARG;
-    auto kern = image4096_tiles_kernel<NP>;
-    hipLaunchKernelGGL(kern, dim3((unsigned)((long)B * (4096 / (2 * NP)))), dim3(256), lds, st,
-                       U, img, band);
+    auto kern = image4096_tiles_kernel<NP, TPB>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((long)B * (4096 / (2 * NP * TPB)))), dim3(256), lds,
+                       st, U, img, band);
     return hipGetLastError() == hipSuccess ? LITHO_OK : LITHO_EHIP;
 }
 
-// two row pairs per block (5 blocks per CU: 538 us per 32 clips at K = 64,
-// 550 with four pairs, 672 with eight, 641 reading one row pair per block)
+// tiles of two row pairs, eight tiles per block (506 us per 32 clips at K =
+// 64; one tile per block 538, four pairs per tile 550, eight 672; reading one
+// row pair's values per block, one 16-byte piece from each of 275 lines, 641)
 int launch_image4096_tr(const float2* U, float* img, int B, int band, hipStream_t st) {
     if (B <= 0 || band < 0 || band >= 512) return LITHO_EARG;
-    return launch_image4096_tiles<2>(U, img, B, band, st);
+    return launch_image4096_tiles<2, 8>(U, img, B, band, st);
 }
 
 // ===========================================================================
