@@ -117,12 +117,15 @@ def test_socs_tmpimg_hls_entry(gpu, orc, clip512):
     assert max_rel(t, to) < IMG_TOL
 
 
-def test_fi(gpu, orc, clip512):
-    case("FI 128 -> 512")
+@pytest.mark.parametrize("N", [512, 4096])
+def test_fi(gpu, orc, clip512, N):
+    """FI of a 128^2 tmpImgp to N^2; at 4096 it runs the 4096-point column and
+    image-row kernels on row-major U (the layout of litho_fi)."""
+    case(f"FI 128 -> {N}")
     m = gpu.manhattan_mask(clip512["N"], 5)
     to = orc.socs_tmpimg(orc.mask_spectrum(m), clip512["K"], clip512["sc"].astype(np.float32), clip512["Nx"],
                          clip512["Nx"])
-    assert max_rel(gpu.FI(to, 512, 2 * clip512["Nx"]), orc.fi(to, 512, 512)) < IMG_TOL
+    assert max_rel(gpu.FI(to, N, 2 * clip512["Nx"]), orc.fi(to, N, N)) < IMG_TOL
 
 
 def test_calc_socs_from_spectrum(gpu, orc, clip512):
