@@ -244,6 +244,59 @@ def small_clip_bench(ctx, src, sigma, steps, with_oracle):
     return out
 
 
+def v18_entry_bench(ctx, reps=50):
+    """The HLS v18 accelerator entry at the reference's own operating point:
+    its golden_1024 optics (1 nm pixels, 1024 x 1024 nm period, NA 0.8,
+    defocus 0.2, 101^2 annulus 0.6-0.9, so nx = ny = 8), K = 10 kernels from
+    the GPU, the fixed 128 x 128 tmpImgp (v18 .h L34-35), FI excluded.  Host
+    entry: litho_socs_tmpimg on host buffers, as a v18 caller calls it (the
+    1024^2 spectrum uploaded, the SOCS loop, both outputs read back); device
+    entry: litho_socs_tmpimg_dev with the kernels resident in a bank and the
+    spectrum in HBM, calls enqueued back to back.  The reference's figures for
+    this call: the FPGA SOCS loop's C-synthesis estimate, 10.57 ms at 250 MHz
+    (doc/论文/第五章.md L193-216), and the CPU Module 7, 35.6 ms on one thread
+    (data/pc_cpp/run_log_10kernels.txt L112, FI included)."""
+    src = L.createSource(101, "Annular", 0.6, 0.9)
+    s = L.getOuterSigma(src)
+    nx = L.compute_nx(0.8, 1024, s, LAMBDA)
+    nk = 10
+    sc, K, _ = L.kernels_from_source(src, s, LAMBDA, 0.8, 0.2, 1024, 1024, nx, nx, nk)
+    mf = L.FT_r2c_padded(L.manhattan_mask(1024, 77))
+    G = L.tmpimg_grid(nx)
+    L.calc_socs_hls(mf, K, sc, nx, nx)  # first call: context and tables
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.calc_socs_hls(mf, K, sc, nx, nx)
+    host_ms = (time.perf_counter() - t0) / reps * 1e3
+    bank = ctx.bank(K, sc, nx, nx)
+    mr, mi = L._planes(mf)
+    dr, di = ctx.alloc(mr.nbytes), ctx.alloc(mi.nbytes)
+    dt, do = ctx.alloc(G * G * 4), ctx.alloc(G * G * 4)
+    try:
+        ctx.h2d(dr, mr)
+        ctx.h2d(di, mi)
+
+        def call():
+            L._check(L.lib().litho_socs_tmpimg_dev(ctx.h, bank, dr, di, 1024, 1024, G, G, dt, do),
+                     "litho_socs_tmpimg_dev")
+
+        call()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        ctx.sync()
+        dev_us = (time.perf_counter() - t0) / reps * 1e6
+    finally:
+        for q in (dr, di, dt, do):
+            ctx.free(q)
+        bank.close()
+    return {"nx": nx, "K": nk, "grid": G, "host_entry_ms": round(host_ms, 3),
+            "device_entry_us": round(dev_us, 1),
+            "reference": "FPGA SOCS loop 10.57 ms per call (C-synthesis estimate, 250 MHz, "
+                         "FI excluded); CPU Module 7 35.6 ms on one thread (FI included)"}
+
+
 def large_clip_bench(ctx, src, sigma, steps, with_oracle):
     """Config 5: 4096^2 clips at 4 nm (period 16384 nm), Nx = 137, K = 64 from
     the implicit TCC (source-space Gram route); the SOCS loop in FP32 on a
@@ -400,6 +453,8 @@ def main():
                     help="skip the mixed-size (batching form) measurement")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 512^2 (config 1) measurement")
+    ap.add_argument("--no-v18", action="store_true",
+                    help="skip the timing of the HLS v18 entry (litho_socs_tmpimg)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 4096^2 (config 5) measurement and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -610,6 +665,10 @@ def main():
                                                               max(3, args.steps // 4))
     small = None if args.no_small else small_clip_bench(ctx, src, sigma, 10, not args.no_cpu)
     large = None if args.no_large else large_clip_bench(ctx, src, sigma, 4, not args.no_cpu)
+    v18 = None if args.no_v18 else v18_entry_bench(ctx)
+    if v18:
+        log(f"v18 entry (nx = ny = {v18['nx']}, K = {v18['K']}, {v18['grid']}^2 tmpImgp): host "
+            f"{v18['host_entry_ms']} ms per call, device-resident {v18['device_entry_us']} us")
     surrogate = surrogate_bench(ctx, src, sigma, args.surrogate_s) if args.surrogate_s > 0 else None
 
     cpu = None
@@ -717,6 +776,7 @@ def main():
         "batching_form": mixed_sizes,
         "small_clip_form": small,
         "large_clip_form": large,
+        "v18_entry": v18,
         "learned_form": surrogate,
         "cpu_baseline": cpu,
     }
