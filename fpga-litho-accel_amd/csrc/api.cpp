@@ -2188,6 +2188,84 @@ int litho_image_tcc_rect(const float* mask, int Lx, int Ly, const double* tcc_re
     return LITHO_OK;
 }
 
+int litho_mask_spectrum_rect(const float* mask, int Lx, int Ly, float* mskf_r, float* mskf_i) {
+    if (!mask || !mskf_r || !mskf_i || !is_pow2(Lx) || !is_pow2(Ly) || Lx < 32 || Ly < 32 ||
+        Lx > 4096 || Ly > 4096)
+        return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    const size_t LL = (size_t)Lx * Ly;
+    Dev dm, dR, dM;
+    LITHO_TRY(dm.alloc(LL * sizeof(float)));
+    LITHO_TRY(dR.alloc(LL * sizeof(float2)));
+    LITHO_TRY(dM.alloc(LL * sizeof(float2)));
+    LITHO_HIP(hipMemcpyAsync(dm.p, mask, LL * sizeof(float), hipMemcpyHostToDevice, st));
+    LITHO_TRY(launch_rows_dft(Lx, dm.as<float>(), dR.as<float2>(), Ly, -Lx / 2, Lx, true, c->tw,
+                              st));
+    LITHO_TRY(launch_mask_cols(Ly, dR.as<float2>(), dM.as<float2>(), 1, Lx, -Ly / 2, Ly, c->tw,
+                               st));
+    std::vector<float2> h(LL);
+    LITHO_HIP(hipMemcpyAsync(h.data(), dM.p, LL * sizeof(float2), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    // the column pass divides by Ly^2, FT_r2c_padded by Lx Ly (L327-329)
+    const float s = (float)Ly / (float)Lx;
+    for (size_t i = 0; i < LL; i++) {
+        mskf_r[i] = h[i].x * s;
+        mskf_i[i] = h[i].y * s;
+    }
+    return LITHO_OK;
+}
+
+int litho_socs_image_rect(const float* mask, int Lx, int Ly, const float* krn_r,
+                          const float* krn_i, const float* scales, int nk, int Nx, int Ny,
+                          float* image) {
+    if (!mask || !krn_r || !krn_i || !scales || !image || nk <= 0 || Nx < 0 || Ny < 0 ||
+        !is_pow2(Lx) || !is_pow2(Ly) || Lx < 32 || Ly < 32 || Lx > 4096 || Ly > 4096)
+        return LITHO_EARG;
+    // the SOCS loop on one square grid G >= 4 max(Nx, Ny) + 1 (the intensity is
+    // band-limited to 2 Nx x 2 Ny, so any grid holding the band gives the same
+    // image after FI as calcSOCS's nextPow2 grid per axis, L833-834)
+    const int m = std::max(Nx, Ny), W = 2 * m + 1, band = 2 * m;
+    const int G = std::max(32, next_pow2(4 * m + 1));
+    if (G > 1024 || G > Lx || G > Ly) return LITHO_EARG;
+    litho_ctx* c;
+    LITHO_TRY(default_ctx(&c));
+    hipStream_t st = c->stream;
+    // the window's column pass divides by Ly^2 where FT_r2c_padded divides by
+    // Lx Ly: the intensity, quadratic in the window, takes (Ly / Lx)^2 in the
+    // weights
+    const double q = (double)Ly / Lx;
+    std::vector<float> sc(scales, scales + nk);
+    for (float& v : sc) v = (float)(v * q * q);
+    litho_bank* bank = nullptr;
+    LITHO_TRY(litho_bank_create(c, krn_r, krn_i, sc.data(), nk, Nx, Ny, &bank));
+    std::unique_ptr<litho_bank, int (*)(litho_bank*)> guard(bank, litho_bank_destroy);
+    const size_t LL = (size_t)Lx * Ly;
+    Dev dm, dR, dM, dT, dP, dU, dI;
+    LITHO_TRY(dm.alloc(LL * sizeof(float)));
+    LITHO_TRY(dR.alloc((size_t)Ly * W * sizeof(float2)));
+    LITHO_TRY(dM.alloc((size_t)W * W * sizeof(float2)));
+    LITHO_TRY(dT.alloc((size_t)G * G * sizeof(float)));
+    LITHO_TRY(dP.alloc((size_t)G * (band + 1) * sizeof(float2)));
+    LITHO_TRY(dU.alloc((size_t)Ly * (band + 1) * sizeof(float2)));
+    LITHO_TRY(dI.alloc(LL * sizeof(float)));
+    LITHO_HIP(hipMemcpyAsync(dm.p, mask, LL * sizeof(float), hipMemcpyHostToDevice, st));
+    // window (2m+1)^2 of the centred spectrum; the padded kernels are zero
+    // outside their own (2Ny+1) x (2Nx+1)
+    LITHO_TRY(launch_rows_dft(Lx, dm.as<float>(), dR.as<float2>(), Ly, -m, W, true, c->tw, st));
+    LITHO_TRY(launch_mask_cols(Ly, dR.as<float2>(), dM.as<float2>(), 1, W, -m, W, c->tw, st));
+    // tmpImgp (centred G x G), then FI to Ly x Lx as litho_fi
+    LITHO_TRY(tmpimg_enqueue(c, bank->full, m, dM.as<float2>(), G, G, nullptr, dT.as<float>(),
+                             nullptr, st));
+    LITHO_TRY(launch_rows_dft(G, dT.as<float>(), dP.as<float2>(), G, 0, band + 1, true, c->tw, st));
+    LITHO_TRY(launch_fi_cols(G, Ly, dP.as<float2>(), dU.as<float2>(), 1, band, 1, c->tw, st));
+    LITHO_TRY(launch_image_rows(Lx, dU.as<float2>(), dI.as<float>(), Ly, band, c->tw, st));
+    LITHO_HIP(hipMemcpyAsync(image, dI.p, LL * sizeof(float), hipMemcpyDeviceToHost, st));
+    LITHO_HIP(hipStreamSynchronize(st));
+    return LITHO_OK;
+}
+
 int litho_image_tcc(const float* mask, int N, const double* tcc_re, const double* tcc_im, int Nx,
                     int Ny, float* image) {
     if (!mask || !tcc_re || !tcc_im || !image || Nx < 0 || Ny < 0) return LITHO_EARG;

@@ -32,7 +32,7 @@ EXPORTS = [
     "litho_manhattan_mask", "litho_mask_spectrum", "litho_tcc_build", "litho_kernels_topk",
     "litho_kernels_from_source", "litho_setup_timing", "litho_tmpimg_grid", "litho_socs_tmpimg",
     "litho_socs_tmpimg_on_grid", "litho_socs_tmpimg_dev", "litho_socs_image", "litho_fi",
-    "litho_image_tcc", "litho_image_tcc_rect",
+    "litho_image_tcc", "litho_image_tcc_rect", "litho_mask_spectrum_rect", "litho_socs_image_rect",
     "litho_bank_create", "litho_bank_destroy", "litho_bank_info", "litho_socs_image_batch",
     "litho_socs_image_batch_noalloc",
     "litho_ctx_set_path", "litho_ctx_set_timing",
@@ -116,6 +116,8 @@ def lib():
         L.litho_compute_nx.argtypes = [f, i, f, f]
         L.litho_manhattan_mask.argtypes = [_f32p, i, C.c_ulonglong, i, i, f]
         L.litho_mask_spectrum.argtypes = [_f32p, i, _f32p, _f32p]
+        L.litho_mask_spectrum_rect.argtypes = [_f32p, i, i, _f32p, _f32p]
+        L.litho_socs_image_rect.argtypes = [_f32p, i, i, _f32p, _f32p, _f32p, i, i, i, _f32p]
         L.litho_tcc_build.argtypes = [C.POINTER(LithoOptics), _f32p, i, _f64p, _f64p]
         L.litho_kernels_topk.argtypes = [_f64p, _f64p, i, i, i, i, _f64p, _f32p, _f32p,
                                          C.POINTER(i)]
@@ -226,13 +228,17 @@ def optics(wavelength, NA, defocus, outer_sigma, Lx, Ly, Nx, Ny):
 
 # ---- Module 3 ---------------------------------------------------------------
 def FT_r2c_padded(mask):
-    """Full centred mask spectrum / N^2 (L255-331), complex64 N x N."""
+    """Full centred mask spectrum / (Lx Ly) (L255-331), complex64 Ly x Lx."""
     mask = np.ascontiguousarray(mask, np.float32)
-    N = mask.shape[0]
-    r = np.zeros(N * N, np.float32)
-    i = np.zeros(N * N, np.float32)
-    _check(lib().litho_mask_spectrum(mask.ravel(), N, r, i), "litho_mask_spectrum")
-    return (r + 1j * i).astype(np.complex64).reshape(N, N)
+    Ly, Lx = mask.shape
+    r = np.zeros(Lx * Ly, np.float32)
+    i = np.zeros(Lx * Ly, np.float32)
+    if Lx == Ly:
+        _check(lib().litho_mask_spectrum(mask.ravel(), Lx, r, i), "litho_mask_spectrum")
+    else:
+        _check(lib().litho_mask_spectrum_rect(mask.ravel(), Lx, Ly, r, i),
+               "litho_mask_spectrum_rect")
+    return (r + 1j * i).astype(np.complex64).reshape(Ly, Lx)
 
 
 # ---- Module 4 ---------------------------------------------------------------
@@ -367,15 +373,19 @@ def calcImage(mask, tcc, Nx, Ny):
 
 
 def socs_image(mask, krns, scales, Nx, Ny):
-    """Mask -> aerial image on the GPU (Modules 3 + 7 fused pipeline)."""
+    """Mask [Ly][Lx] -> aerial image on the GPU (Modules 3 + 7 fused pipeline)."""
     mask = np.ascontiguousarray(mask, np.float32)
-    N = mask.shape[0]
+    Ly, Lx = mask.shape
     kr, ki = _planes(krns)
     sc = np.ascontiguousarray(scales, np.float32)
-    out = np.zeros(N * N, np.float32)
-    _check(lib().litho_socs_image(mask.ravel(), N, kr, ki, sc, sc.size, Nx, Ny, out),
-           "litho_socs_image")
-    return out.reshape(N, N)
+    out = np.zeros(Lx * Ly, np.float32)
+    if Lx == Ly:
+        _check(lib().litho_socs_image(mask.ravel(), Lx, kr, ki, sc, sc.size, Nx, Ny, out),
+               "litho_socs_image")
+    else:
+        _check(lib().litho_socs_image_rect(mask.ravel(), Lx, Ly, kr, ki, sc, sc.size, Nx, Ny,
+                                           out), "litho_socs_image_rect")
+    return out.reshape(Ly, Lx)
 
 
 # ---- device-resident batch path ----------------------------------------------

@@ -97,6 +97,10 @@ int litho_manhattan_mask(float* mask, int N, unsigned long long seed, int wmin, 
 /* ---- Module 3: FT_r2c_padded, litho_thread.cpp L255-331 ------------------- */
 /* Full centred spectrum of an N x N real mask (N even, <= 4096). */
 int litho_mask_spectrum(const float* mask, int N, float* mskf_r, float* mskf_i);
+/* The same for a rectangular Ly x Lx clip (the reference's fftLx x fftLy,
+ * litho_thread.cpp L1269-1278): mask and spectrum [Ly][Lx], divided by Lx Ly,
+ * Lx and Ly powers of two in 32..4096. */
+int litho_mask_spectrum_rect(const float* mask, int Lx, int Ly, float* mskf_r, float* mskf_i);
 
 /* ---- Module 4: calcTCC, litho_thread.cpp L441-532 ------------------------- */
 /* TCC = A diag(S) A^H with the reference's float pupil arithmetic, built on
@@ -176,6 +180,14 @@ int litho_socs_tmpimg_dev(litho_ctx* ctx, const litho_bank* bank, const float* d
 /* Mask -> aerial image: mask spectrum, SOCS loop and FI (Modules 3 + 7). */
 int litho_socs_image(const float* mask, int N, const float* krn_r, const float* krn_i,
                      const float* scales, int nk, int Nx, int Ny, float* image);
+/* The same on a rectangular Ly x Lx clip (calcSOCS with fftLx != fftLy,
+ * L813-1040): mask and image [Ly][Lx], Lx and Ly powers of two in 32..4096, the
+ * SOCS loop on one square grid G = max(32, nextPow2(4 max(Nx, Ny) + 1)) <=
+ * min(1024, Lx, Ly) (the image does not depend on the grid once it holds the
+ * band). */
+int litho_socs_image_rect(const float* mask, int Lx, int Ly, const float* krn_r,
+                          const float* krn_i, const float* scales, int nk, int Nx, int Ny,
+                          float* image);
 /* FI alone, litho_thread.cpp L692-772: G x G (zero frequency centred, as
  * tmpImgp) -> N x N band-limited interpolation. */
 int litho_fi(const float* tmpImgp, int G, int N, int band, float* image);

@@ -628,6 +628,33 @@ def test_tcc_direct_image_general_shapes(gpu, orc, shape, nxy, period):
     assert max_rel(img, ref) < IMG_TOL
 
 
+@pytest.mark.parametrize("shape,nxy,period", [((64, 128), (4, 2), (512, 256)),
+                                              ((256, 64), (3, 8), (256, 1024)),
+                                              ((512, 1024), (17, 8), (2048, 1024)),
+                                              ((256, 4096), (8, 4), (896, 448)),
+                                              ((4096, 256), (4, 8), (448, 896))])
+def test_socs_image_rect(gpu, orc, shape, nxy, period):
+    """calcSOCS on a rectangular Ly x Lx clip with Nx != Ny: mask spectrum
+    (rectangular FT_r2c_padded) against the oracle, then the image against the
+    oracle's calcSOCS (its own nextPow2 grid per axis) fed the same kernels."""
+    Ly, Lx = shape
+    Nx, Ny = nxy
+    case(f"socs_image_rect {Ly}x{Lx}, Nx={Nx}, Ny={Ny}")
+    src = orc.source_annular(41, 0.6, 0.9)
+    s = orc.outer_sigma(src)
+    T = orc.calc_tcc(src, s, BASE_LAMBDA, BASE_NA, 0.0, period[0], period[1], Nx, Ny)
+    sc, K, _ = orc.calc_kernels(T, 12, Nx, Ny)
+    m = gpu.manhattan_mask(max(Lx, Ly), 93)[:Ly, :Lx]
+    mf = orc.mask_spectrum(m)
+    spec = gpu.FT_r2c_padded(m)
+    assert spec.shape == (Ly, Lx)
+    assert np.abs(spec - mf).max() < 1e-5 * np.abs(mf).max()
+    img = gpu.socs_image(m, K, sc, Nx, Ny)
+    ref = orc.calc_socs(mf, K, np.asarray(sc, np.float32), Nx, Ny)
+    assert img.shape == (Ly, Lx)
+    assert max_rel(img, ref) < IMG_TOL
+
+
 def test_large_clip_implicit_tcc(gpu, orc, base_source):
     """4096^2 clip at 4 nm (period 16384 nm): Nx = 137, a 75625-dim TCC that is
     never formed; the source-space Gram route gives K = 64 kernels (checked
