@@ -274,17 +274,20 @@ int main(int argc, char** argv) {
     const float lambda = (float)jnum(cfg, "optics.wavelength", 193.0);
     const float defocus = (float)jnum(cfg, "optics.defocus", 0.0);
     int nk = (int)jnum(cfg, "kernel.count", 10);
-    if (Lx <= 0 || Ly != Lx || msx <= 0 || msy <= 0 || srcSize <= 0 || NA <= 0.f || pix <= 0) {
-        std::cerr << "error: invalid or unsupported mask/source/optics parameters "
-                     "(square periods only)\n";
+    if (Lx <= 0 || Ly <= 0 || msx <= 0 || msy <= 0 || srcSize <= 0 || NA <= 0.f || pix <= 0) {
+        std::cerr << "error: invalid or unsupported mask/source/optics parameters\n";
         return 1;
     }
-    const int N = (int)std::lround(Lx / pix);  // pixels per side
-    if (std::fabs(N * pix - Lx) > 1e-6 || msx > N || msy > N) {
-        std::cerr << "error: period " << Lx << " nm is not a whole number of " << pix
-                  << " nm pixels covering the mask\n";
+    // pixels per side (physLx x physLy, L1228-1233); Lx != Ly runs the
+    // rectangular entries, which take power-of-two sides
+    const int N = (int)std::lround(Lx / pix), NyP = (int)std::lround(Ly / pix);
+    if (std::fabs(N * pix - Lx) > 1e-6 || std::fabs(NyP * pix - Ly) > 1e-6 || msx > N ||
+        msy > NyP) {
+        std::cerr << "error: period " << Lx << " x " << Ly << " nm is not a whole number of "
+                  << pix << " nm pixels covering the mask\n";
         return 1;
     }
+    const bool rect = NyP != N;
     // Module 1: source
     std::vector<float> src((size_t)srcSize * srcSize, 0.f);
     if (stype == "Annular")
@@ -313,26 +316,37 @@ int main(int argc, char** argv) {
     }
     const float sigma = litho_outer_sigma(src.data(), srcSize);
     const int Nx = litho_compute_nx(NA, Lx, sigma, lambda);
-    const int W = 2 * Nx + 1, n = W * W;
-    int G = 1;
+    const int Ny = rect ? litho_compute_nx(NA, Ly, sigma, lambda) : Nx;  // L1300-1301
+    const int W = 2 * Nx + 1, H = 2 * Ny + 1, n = W * H;
+    int G = 1, Gy = 1;
     while (G < 4 * Nx + 1) G *= 2;
+    while (Gy < 4 * Ny + 1) Gy *= 2;
     if (nk > n) nk = n;  // L1456
-    if (verbose)
-        std::cout << "[CONFIG] period " << Lx << " nm, " << N << "x" << N << " pixels of " << pix
-                  << " nm, NA " << NA << ", lambda " << lambda << ", defocus " << defocus
-                  << "\n[PARAM] outer sigma " << sigma << ", Nx = Ny = " << Nx << ", TCC "
-                  << n << "x" << n << ", conv grid " << G << ", kernels " << nk << "\n";
+    if (verbose) {
+        std::cout << "[CONFIG] period " << Lx;
+        if (rect) std::cout << " x " << Ly;
+        std::cout << " nm, " << N << "x" << NyP << " pixels of " << pix << " nm, NA " << NA
+                  << ", lambda " << lambda << ", defocus " << defocus << "\n[PARAM] outer sigma "
+                  << sigma;
+        if (rect)
+            std::cout << ", Nx = " << Nx << ", Ny = " << Ny;
+        else
+            std::cout << ", Nx = Ny = " << Nx;
+        std::cout << ", TCC " << n << "x" << n << ", conv grid " << G;
+        if (rect) std::cout << "x" << Gy;
+        std::cout << ", kernels " << nk << "\n";
+    }
     if (dry) return 0;
     fs::create_directories(out / "kernels");
     {
         std::ofstream m(out / "fft_meta.txt");
-        m << "physical_size_x " << N << "\nphysical_size_y " << N << "\nfft_size_x " << N
-          << "\nfft_size_y " << N << "\nNx " << Nx << "\nNy " << Nx << "\nconv_size_x "
-          << 4 * Nx + 1 << "\nconv_size_y " << 4 * Nx + 1 << "\nfft_conv_size_x " << G
-          << "\nfft_conv_size_y " << G << "\n";
+        m << "physical_size_x " << N << "\nphysical_size_y " << NyP << "\nfft_size_x " << N
+          << "\nfft_size_y " << NyP << "\nNx " << Nx << "\nNy " << Ny << "\nconv_size_x "
+          << 4 * Nx + 1 << "\nconv_size_y " << 4 * Ny + 1 << "\nfft_conv_size_x " << G
+          << "\nfft_conv_size_y " << Gy << "\n";
     }
     // Module 2: mask (createMask, common/mask.cpp L31-60)
-    std::vector<float> inner((size_t)msx * msy, 0.f), mask((size_t)N * N, 0.f);
+    std::vector<float> inner((size_t)msx * msy, 0.f), mask((size_t)N * NyP, 0.f);
     if (mtype == "LineSpace") {
         const int lw = (int)jnum(cfg, "mask.lineSpace.lineWidth", 128);
         const int sw = (int)jnum(cfg, "mask.lineSpace.spaceWidth", 128);
@@ -355,17 +369,20 @@ int main(int argc, char** argv) {
         std::cerr << "error: unknown mask type " << mtype << "\n";
         return 1;
     }
-    const int oy = (N - msy) / 2, ox = (N - msx) / 2;
+    const int oy = (NyP - msy) / 2, ox = (N - msx) / 2;
     for (int y = 0; y < msy; y++)
         for (int x = 0; x < msx; x++)
             mask[(size_t)(y + oy) * N + x + ox] = inner[(size_t)y * msx + x] * dose;
 
     long tm[8] = {0};
     // Module 3: mask spectrum
-    std::vector<float> mr((size_t)N * N), mi((size_t)N * N);
+    std::vector<float> mr((size_t)N * NyP), mi((size_t)N * NyP);
     {
         Timer t;
-        CHECK(litho_mask_spectrum(mask.data(), N, mr.data(), mi.data()));
+        if (rect)
+            CHECK(litho_mask_spectrum_rect(mask.data(), N, NyP, mr.data(), mi.data()));
+        else
+            CHECK(litho_mask_spectrum(mask.data(), N, mr.data(), mi.data()));
         tm[3] = t.us();
     }
     write_bin(out / "mskf_r.bin", mr.data(), mr.size());
@@ -373,7 +390,7 @@ int main(int argc, char** argv) {
 
     std::vector<float> kr((size_t)nk * n), ki((size_t)nk * n), sc(nk);
     std::vector<double> scd(nk);
-    LithoOptics op{lambda, NA, defocus, sigma, Lx, Ly, Nx, Nx};
+    LithoOptics op{lambda, NA, defocus, sigma, Lx, Ly, Nx, Ny};
     if (loadDir.empty()) {
         std::vector<double> tr, ti;
         if (!noTcc) {
@@ -388,10 +405,16 @@ int main(int argc, char** argv) {
             write_bin(out / "tcc_r.bin", f32.data(), f32.size());
             for (size_t i = 0; i < f32.size(); i++) f32[i] = (float)ti[i];
             write_bin(out / "tcc_i.bin", f32.data(), f32.size());
-            if (N >= 256 && N <= 1024 && (N & (N - 1)) == 0) {
+            const auto p2 = [](int v) { return v >= 256 && v <= 1024 && (v & (v - 1)) == 0; };
+            if (p2(N) && p2(NyP)) {
                 Timer t5;
-                std::vector<float> img((size_t)N * N);
-                CHECK(litho_image_tcc(mask.data(), N, tr.data(), ti.data(), Nx, Nx, img.data()));
+                std::vector<float> img((size_t)N * NyP);
+                if (rect)
+                    CHECK(litho_image_tcc_rect(mask.data(), N, NyP, tr.data(), ti.data(), Nx, Ny,
+                                               img.data()));
+                else
+                    CHECK(litho_image_tcc(mask.data(), N, tr.data(), ti.data(), Nx, Nx,
+                                          img.data()));
                 tm[5] = t5.us();
                 write_bin(out / "aerial_image_tcc_direct.bin", img.data(), img.size());
             }
@@ -400,7 +423,7 @@ int main(int argc, char** argv) {
         int flags = 0;
         Timer t;
         if (!noTcc)
-            CHECK(litho_kernels_topk(tr.data(), ti.data(), n, Nx, Nx, nk, scd.data(), kr.data(),
+            CHECK(litho_kernels_topk(tr.data(), ti.data(), n, Nx, Ny, nk, scd.data(), kr.data(),
                                      ki.data(), &flags));
         else
             CHECK(litho_kernels_from_source(&op, src.data(), srcSize, nk, scd.data(), kr.data(),
@@ -417,7 +440,7 @@ int main(int argc, char** argv) {
         }
         write_bin(out / "scales.bin", sc.data(), nk);
         std::ofstream info(out / "kernels" / "kernel_info.txt");
-        info << "# Kernel Information\n## Kernel Size and Count\n- Kernel Size: " << W << "x" << W
+        info << "# Kernel Information\n## Kernel Size and Count\n- Kernel Size: " << W << "x" << H
              << "\n- Number of Kernels: " << nk << "\n## Eigenvalues of Each Kernel\n";
         for (int k = 0; k < nk; k++) info << "Eigenvalue " << k << ": " << sc[k] << "\n";
     } else {
@@ -438,13 +461,19 @@ int main(int argc, char** argv) {
     // Module 7: SOCS (HLS-shaped tmpImgp and the full image)
     {
         Timer t;
-        std::vector<float> tmp((size_t)G * G);
+        std::vector<float> tmp((size_t)G * Gy);
         CHECK(litho_socs_tmpimg_on_grid(mr.data(), mi.data(), kr.data(), ki.data(), sc.data(),
-                                        tmp.data(), nullptr, nk, Nx, Nx, N, N, G, G));
-        write_bin(out / ("tmpImgp_full_" + std::to_string(G) + ".bin"), tmp.data(), tmp.size());
-        std::vector<float> img((size_t)N * N);
-        CHECK(litho_socs_image(mask.data(), N, kr.data(), ki.data(), sc.data(), nk, Nx, Nx,
-                               img.data()));
+                                        tmp.data(), nullptr, nk, Nx, Ny, N, NyP, G, Gy));
+        write_bin(out / ("tmpImgp_full_" + std::to_string(G) +
+                         (rect ? "x" + std::to_string(Gy) : std::string()) + ".bin"),
+                  tmp.data(), tmp.size());
+        std::vector<float> img((size_t)N * NyP);
+        if (rect)
+            CHECK(litho_socs_image_rect(mask.data(), N, NyP, kr.data(), ki.data(), sc.data(), nk,
+                                        Nx, Ny, img.data()));
+        else
+            CHECK(litho_socs_image(mask.data(), N, kr.data(), ki.data(), sc.data(), nk, Nx, Nx,
+                                   img.data()));
         tm[7] = t.us();
         write_bin(out / "aerial_image_socs_kernel.bin", img.data(), img.size());
         if (verbose) {

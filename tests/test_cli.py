@@ -25,6 +25,10 @@ def test_dry_run_parameters():
     assert "512x512 pixels of 4 nm" in out and "Nx = Ny = 17" in out and "conv grid 128" in out
     out = run(os.path.join(CFG, "fixture_512_linespace.json"), "/tmp/unused", "--dry-run").stdout
     assert "Nx = Ny = 4" in out and "TCC 81x81" in out
+    # a rectangular period: Nx, Ny and the conv grid per axis (L1300-1301, L833-834)
+    out = run(os.path.join(CFG, "linespace_512x256_rect.json"), "/tmp/unused", "--dry-run").stdout
+    assert "512x256 pixels of 1 nm" in out and "Nx = 4, Ny = 2" in out
+    assert "TCC 45x45" in out and "conv grid 32x16" in out
 
 
 def test_bad_inputs_fail_cleanly(tmp_path):
@@ -93,3 +97,35 @@ def test_full_flow_512_source_route(tmp_path, orc):
     img = rd(out / "aerial_image_socs_kernel.bin", (N, N))
     assert max_rel(img, orc.calc_socs(mf.astype(np.complex128), K, sc, Nx, Nx)) < 1e-5
     assert rd(out / "tmpImgp_full_128.bin", (128, 128)).min() >= 0
+
+
+@pytest.mark.gpu
+def test_full_flow_rectangular_period(tmp_path, orc):
+    """A 512 x 256 nm period: every BIN output against the oracle on the same
+    mask, optics and kernels (Nx = 4, Ny = 2, conv grid 32 x 16)."""
+    out = tmp_path / "rect"
+    run(os.path.join(CFG, "linespace_512x256_rect.json"), str(out), "--quiet")
+    Lx, Ly, Nx, Ny, nk = 512, 256, 4, 2, 12
+    meta = (out / "fft_meta.txt").read_text()
+    assert "physical_size_y 256" in meta and "Ny 2" in meta and "fft_conv_size_y 16" in meta
+    src = orc.source_annular(101, 0.6, 0.9)
+    m = np.zeros((Ly, Lx), np.float32)
+    m[64:192, 128:384] = orc.linespace(256, 128, 16, 24)
+    mf = orc.mask_spectrum(m)
+    assert max_rel(rd(out / "mskf_r.bin", (Ly, Lx)) + 1j * rd(out / "mskf_i.bin", (Ly, Lx)),
+                   mf) < 1e-5
+    T = orc.calc_tcc(src, orc.outer_sigma(src), 193.0, 0.8, 0.2, Lx, Ly, Nx, Ny)
+    assert max_rel(np.fromfile(out / "tcc_r.bin", np.float32), T.real.ravel()) < 1e-5
+    assert max_rel(rd(out / "aerial_image_tcc_direct.bin", (Ly, Lx)),
+                   orc.calc_image_tcc(mf, T, Nx, Ny)) < 1e-5
+    sc = np.fromfile(out / "scales.bin", np.float32)
+    ref_sc, _, _ = orc.calc_kernels(T, nk, Nx, Ny)
+    assert np.all(np.abs(sc - ref_sc) / ref_sc < 1e-5)
+    K = np.stack([np.fromfile(out / "kernels" / f"krn_{k}_r.bin", np.float32) +
+                  1j * np.fromfile(out / "kernels" / f"krn_{k}_i.bin", np.float32)
+                  for k in range(nk)])
+    assert K.shape == (nk, (2 * Nx + 1) * (2 * Ny + 1))
+    assert max_rel(rd(out / "tmpImgp_full_32x16.bin", (16, 32)),
+                   orc.socs_tmpimg(mf, K, sc, Nx, Ny)) < 1e-5
+    assert max_rel(rd(out / "aerial_image_socs_kernel.bin", (Ly, Lx)),
+                   orc.calc_socs(mf, K, sc, Nx, Ny)) < 1e-5
