@@ -235,6 +235,40 @@ def test_socs_full_rank_equals_tcc_direct(orc, fix_tcc):
     assert max_rel(socs, ref) < 1e-6
 
 
+def test_mask_spectrum_rectangular_direct_formula(orc):
+    # Ly x Lx: out[Ly/2+ky][Lx/2+kx] = (1/(Lx Ly)) sum mask e^{-2 pi i (kx x'/Lx + ky y'/Ly)}
+    Ly, Lx = 8, 16
+    m = (np.random.default_rng(2).random((Ly, Lx)) > 0.5).astype(np.float32)
+
+    def E(n):
+        k = np.arange(n) - n // 2
+        return np.exp(-2j * np.pi * np.outer(k, k) / n)
+
+    direct = E(Ly) @ m.astype(np.float64) @ E(Lx).T / (Lx * Ly)
+    assert max_rel(orc.mask_spectrum(m), direct) < 1e-13
+
+
+def test_rectangular_clip_full_rank_and_transpose(orc):
+    """A 64 x 128 clip (period 256 x 512 nm, Nx = 4, Ny = 2): SOCS with every
+    kernel equals the TCC-direct image, and the clip transposed with the
+    periods and Nx, Ny swapped gives the transposed image (the annulus and the
+    in-focus pupil are symmetric under x <-> y), which an Lx/Ly or Nx/Ny mix-up
+    anywhere in the rectangular path breaks."""
+    src = orc.source_annular(41, 0.6, 0.9)
+    s = orc.outer_sigma(src)
+    m = np.zeros((64, 128), np.float32)
+    m[16:48, 24:104] = orc.linespace(80, 32, 6, 10)
+    T = orc.calc_tcc(src, s, 193.0, 0.8, 0.0, 512, 256, 4, 2)
+    mf = orc.mask_spectrum(m)
+    img = orc.calc_image_tcc(mf, T, 4, 2)
+    assert img.shape == (64, 128) and img.max() > 0
+    sc, K, _ = orc.calc_kernels(T, T.shape[0], 4, 2)
+    assert max_rel(orc.calc_socs(mf, K, sc.astype(np.float32), 4, 2), img) < 1e-6
+    Tt = orc.calc_tcc(src, s, 193.0, 0.8, 0.0, 256, 512, 2, 4)
+    imgt = orc.calc_image_tcc(orc.mask_spectrum(np.ascontiguousarray(m.T)), Tt, 2, 4)
+    assert max_rel(imgt, img.T) < 1e-6
+
+
 def test_fi_is_exact_on_band_limited_grids(orc):
     G, N, b = 32, 96, 7
     rng = np.random.default_rng(5)
